@@ -1,0 +1,99 @@
+/*
+ * xca_amd — C ABI of the MI355X pseudo-particle self-energy engine.
+ *
+ * Drop-in boundary for triqs_xca's DiagramEvaluator (c++/triqs_xca/block_sparse_backbone.hpp:23-165)
+ * and dyson_it_ppsc (c++/triqs_xca/dlr_dyson_ppsc.hpp:45-230). Plain pointers and sizes only; no
+ * exception crosses the ABI: every entry point returns a status (0 = success, < 0 = error) or a
+ * handle (NULL on error), and xca_last_error() returns the message of the last failure on the
+ * calling thread, with the reference's message text where the reference raises.
+ *
+ * Array layouts follow nda C order as in the reference: Green's-function blocks (r, b, b) indexed
+ * t*b*b + i*b + j, hybridization coefficients (p, n, n), topology (m, 2) int32. Time nodes are in
+ * relative format (t in [0, 1/2] as t, t in (1/2, 1] as t - 1). Poles are in energy units and are
+ * multiplied by beta inside, as hyb.cpp:18 does. All device arithmetic is real FP64.
+ */
+#ifndef XCA_AMD_H
+#define XCA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *xca_last_error(void);
+
+/* DLR grid for cutoff lambda = beta * w_max and tolerance eps. Writes r real-frequency nodes
+ * (dimensionless) and r imaginary-time nodes (relative format) and returns r, or < 0 if r > max_r.
+ * Replaces cppdlr::build_dlr_rf + imtime_ops node selection used at
+ * c++/triqs_xca/block_sparse_backbone.cpp:68-70 and python/triqs_xca/block_sparse_solver.py:105,136. */
+int xca_dlr_build(double lambda, double eps, int max_r, double *rf_out, double *it_out);
+
+/* Fermionic kernel K(t, om) = -exp(-om t)/(1 + exp(-om)), t in relative format (cppdlr::k_it as
+ * called at c++/triqs_xca/hyb.cpp:37-41). */
+double xca_dlr_kernel(double t, double om);
+
+/* One impurity problem: DLR grid, hybridization poles/coefficients and the block-sparse
+ * annihilation/creation operators in the Fock basis (what get_operators builds from atom_diag,
+ * c++/triqs_xca/atom_diag_utils.cpp:39-185). c_conn[o*nblocks + j] is the block that c_o maps block
+ * j to (-1: none); c_mats concatenates the (dims[c_conn], dims[j]) matrices in (o, j) order over the
+ * connected j. The same for cdag. */
+typedef struct xca_problem {
+  double beta;
+  int r;
+  const double *it;      /* (r) relative times */
+  const double *rf;      /* (r) dimensionless real frequencies */
+  int p;
+  const double *poles;   /* (p) energy units */
+  const double *coeffs;  /* (p, n, n) */
+  int n;                 /* number of flavours */
+  int nblocks;
+  const int *dims;       /* (nblocks) */
+  const int *c_conn;     /* (n, nblocks) */
+  const int *cdag_conn;  /* (n, nblocks) */
+  const double *c_mats;
+  const double *cdag_mats;
+} xca_problem;
+
+typedef struct xca_ctx xca_ctx;
+
+/* Build an evaluator on GPU `device` and upload operators, barred operators, hybridization and
+ * kernel tables; they stay resident until xca_ctx_destroy. Replaces
+ * DiagramEvaluator(beta, Lambda, eps, hyb_poles, hyb_coeffs, Fq)
+ * (c++/triqs_xca/block_sparse_backbone.cpp:63-85). NULL on error (shape errors, cross-set
+ * hybridization: "couples different symmetry sets", no GPU). */
+xca_ctx *xca_ctx_create(const xca_problem *desc, int device);
+void xca_ctx_destroy(xca_ctx *ctx);
+
+/* get_num_self_energy_backbones (c++/triqs_xca/block_sparse_backbone.cpp:721-724):
+ * 2^m n^(m-1) p^(m-1). */
+int64_t xca_num_backbones(xca_ctx *ctx, int m);
+/* Number of symmetry sets found from the annihilator connections. */
+int xca_num_sym_sets(xca_ctx *ctx);
+
+/* compute_self_energy(G, topology[, f_ix_vec]) (c++/triqs_xca/block_sparse_backbone.cpp:369-459):
+ * G and sigma_out are the concatenated (r, b, b) blocks (host memory). fix == NULL evaluates every
+ * flat backbone index; otherwise the nfix listed ones (error "f_ix out of range" if any is outside
+ * [0, xca_num_backbones)). The result carries the reference's per-backbone signs but not the
+ * order/topology prefactor applied by the Python caller (block_sparse_solver.py:618,636). */
+int xca_sigma(xca_ctx *ctx, const double *G, int m, const int *topo, const int32_t *fix, int64_t nfix, double *sigma_out);
+
+/* Device-resident form for benchmarking and the iteration loop: upload G once, evaluate any number
+ * of topologies into the device Sigma accumulator with weight `scale` (Sigma += scale * S_topo),
+ * then download. xca_sigma_zero clears the accumulator. */
+int xca_set_G(xca_ctx *ctx, const double *G);
+int xca_sigma_zero(xca_ctx *ctx);
+int xca_sigma_accumulate(xca_ctx *ctx, int m, const int *topo, double scale);
+int xca_get_sigma(xca_ctx *ctx, double *sigma_out);
+int xca_synchronize(xca_ctx *ctx);
+
+/* Statistics of the last accumulate call: number of (path, pole tuple) items evaluated, canonical
+ * multiply-add count, and the device time of the chain kernel in milliseconds. */
+int xca_last_stats(xca_ctx *ctx, int64_t *items, double *macs, double *kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

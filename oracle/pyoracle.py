@@ -1,0 +1,189 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+ORACLE — test infrastructure only: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg as the checker; never by the product package.
+"""
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_build" / "liboracle.so"
+
+d_p = C.POINTER(C.c_double)
+i_p = C.POINTER(C.c_int)
+
+
+class OrcProblem(C.Structure):
+    _fields_ = [
+        ("beta", C.c_double), ("r", C.c_int), ("it", d_p), ("rf", d_p),
+        ("p", C.c_int), ("poles", d_p), ("coeffs", d_p),
+        ("n", C.c_int), ("nblocks", C.c_int), ("dims", i_p),
+        ("c_conn", i_p), ("cdag_conn", i_p), ("c_mats", d_p), ("cdag_mats", d_p),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.POINTER(OrcProblem), C.c_int]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_num_backbones.restype = C.c_long
+        L.orc_num_backbones.argtypes = [C.c_void_p, C.c_int]
+        L.orc_num_sym_sets.argtypes = [C.c_void_p]
+        L.orc_sigma.argtypes = [C.c_void_p, d_p, C.c_int, i_p, i_p, C.c_long, d_p]
+        L.orc_vals2coefs.argtypes = [C.c_int, d_p, d_p, d_p, C.c_int]
+        L.orc_convolve.argtypes = [C.c_double, C.c_int, d_p, d_p, d_p, d_p, C.c_int, C.c_int, C.c_int, d_p]
+        L.orc_reflect.argtypes = [C.c_int, d_p, d_p, d_p, C.c_int, d_p]
+        L.orc_interp.argtypes = [C.c_int, d_p, d_p, d_p, C.c_int, d_p, C.c_int, d_p]
+        L.orc_dyson.argtypes = [C.c_double, C.c_int, d_p, d_p, C.c_int, d_p, d_p, C.c_double, d_p]
+        L.orc_backbone.argtypes = [i_p, C.c_int, C.c_int, d_p, C.c_int, C.c_int, i_p, i_p, i_p, i_p]
+        _lib = L
+    return _lib
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _chk(st):
+    if st != 0:
+        raise OracleError(lib().orc_last_error().decode())
+
+
+def _z(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _dp(a):
+    return a.ctypes.data_as(d_p)
+
+
+def _ip(a):
+    return a.ctypes.data_as(i_p)
+
+
+# ------------------------------------------------------------------ DLR unit operations
+
+def vals2coefs(it, rf, vals):
+    v = _z(vals).copy()
+    m = v.size // len(it)
+    _chk(lib().orc_vals2coefs(len(it), _dp(np.ascontiguousarray(it)), _dp(np.ascontiguousarray(rf)), _dp(v), m))
+    return v
+
+
+def convolve(beta, it, rf, f, g):
+    f, g = _z(f), _z(g)
+    r, a, b = f.shape
+    _, b2, c = g.shape
+    out = np.zeros((r, a, c), dtype=np.complex128)
+    _chk(lib().orc_convolve(beta, r, _dp(np.ascontiguousarray(it)), _dp(np.ascontiguousarray(rf)), _dp(f), _dp(g), a, b, c, _dp(out)))
+    return out
+
+
+def reflect(it, rf, vals):
+    v = _z(vals)
+    out = np.zeros_like(v)
+    _chk(lib().orc_reflect(len(it), _dp(np.ascontiguousarray(it)), _dp(np.ascontiguousarray(rf)), _dp(v), v.size // len(it), _dp(out)))
+    return out
+
+
+def interp(it, rf, vals, ts):
+    v = _z(vals)
+    ts = np.ascontiguousarray(ts, dtype=np.float64)
+    m = v.size // len(it)
+    out = np.zeros((len(ts),) + v.shape[1:], dtype=np.complex128)
+    _chk(lib().orc_interp(len(it), _dp(np.ascontiguousarray(it)), _dp(np.ascontiguousarray(rf)), _dp(v), m, _dp(ts), len(ts), _dp(out)))
+    return out
+
+
+def dyson(beta, it, rf, g0, sig, eta):
+    g0, sig = _z(g0), _z(sig)
+    r, b, _ = g0.shape
+    out = np.zeros_like(g0)
+    _chk(lib().orc_dyson(beta, r, _dp(np.ascontiguousarray(it)), _dp(np.ascontiguousarray(rf)), b, _dp(g0), _dp(sig), eta, _dp(out)))
+    return out
+
+
+def backbone(topology, n, poles_dimless, f_ix):
+    topo = np.ascontiguousarray(topology, dtype=np.int32)
+    m = topo.shape[0]
+    poles = np.ascontiguousarray(poles_dimless, dtype=np.float64)
+    vert = np.zeros((2 * m, 5), dtype=np.int32)
+    edges = np.zeros((2 * m - 1, max(m - 1, 1)), dtype=np.int32)
+    pref = np.zeros((max(m - 1, 1), 2), dtype=np.int32)
+    misc = np.zeros(2 * m + 1, dtype=np.int32)
+    _chk(lib().orc_backbone(_ip(topo), m, n, _dp(poles), len(poles), f_ix, _ip(vert), _ip(edges), _ip(pref), _ip(misc)))
+    return dict(bar=vert[:, 0], dag=vert[:, 1], hyb_ind=vert[:, 2], Ksign=vert[:, 3], orb=vert[:, 4],
+                edges=edges[:, : m - 1], pref_Ksign=pref[: m - 1, 0], pref_Kexp=pref[: m - 1, 1],
+                prefactor_sign=int(misc[0]), fb=misc[1:1 + m], pole_inds=misc[1 + m:2 * m])
+
+
+# ------------------------------------------------------------------ evaluator
+
+class OracleEvaluator:
+    """CPU restatement of DiagramEvaluator for one ImpurityProblem (xca_amd.models)."""
+
+    def __init__(self, prob, complex_arith=True):
+        ad = prob.ad
+        self.prob = prob
+        self._keep = []
+
+        def keep(a):
+            self._keep.append(a)
+            return a
+
+        it = keep(np.ascontiguousarray(prob.grid.it, dtype=np.float64))
+        rf = keep(np.ascontiguousarray(prob.grid.rf, dtype=np.float64))
+        poles = keep(np.ascontiguousarray(prob.poles, dtype=np.float64))
+        coeffs = keep(_z(prob.coeffs))
+        dims = keep(np.ascontiguousarray(ad.dims, dtype=np.int32))
+        cc = keep(np.ascontiguousarray(ad.c_conn, dtype=np.int32))
+        dc = keep(np.ascontiguousarray(ad.cdag_conn, dtype=np.int32))
+        cm, dm = ad.packed_ops()
+        cm, dm = keep(_z(cm)), keep(_z(dm))
+        d = OrcProblem(prob.beta, prob.grid.r, _dp(it), _dp(rf), len(poles), _dp(poles), _dp(coeffs), prob.n, ad.nblocks, _ip(dims),
+                       _ip(cc), _ip(dc), _dp(cm), _dp(dm))
+        h = lib().orc_create(C.byref(d), 1 if complex_arith else 0)
+        if not h:
+            raise OracleError(lib().orc_last_error().decode())
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def num_backbones(self, m):
+        return lib().orc_num_backbones(self.h, m)
+
+    def num_sym_sets(self):
+        return lib().orc_num_sym_sets(self.h)
+
+    def sigma(self, G, topology, fix=None):
+        """compute_self_energy(G, topology[, f_ix_vec]); G: list of (r, b, b) blocks."""
+        prob = self.prob
+        g = _z(np.concatenate([np.asarray(b).ravel() for b in G]))
+        topo = np.ascontiguousarray(topology, dtype=np.int32)
+        out = np.zeros_like(g)
+        if fix is None:
+            st = lib().orc_sigma(self.h, _dp(g), topo.shape[0], _ip(topo), None, 0, _dp(out))
+        else:
+            fx = np.ascontiguousarray(fix, dtype=np.int32)
+            st = lib().orc_sigma(self.h, _dp(g), topo.shape[0], _ip(topo), _ip(fx), len(fx), _dp(out))
+        _chk(st)
+        return prob.unpack(out)

@@ -1,0 +1,185 @@
+"""Fock-space input builder: restates what triqs.atom_diag supplies to the reference's
+DiagramEvaluator (c++/triqs_xca/atom_diag_utils.cpp:39-343) and the atomic pseudo-particle
+propagator of python/triqs_xca/block_sparse_solver.py:1305-1328.
+
+Conventions:
+  * flavour o is bit o of the Fock-state index; c_o|s> = (-1)^{#occupied flavours below o} |s - o>
+    (the sign convention pinned by test/c++/test_atom_diag_utils.cpp:104-121);
+  * blocks are the sectors of the chosen conserved quantities ('NSz': particle number and S_z,
+    'N': particle number, 'none': one block), ordered by their first Fock state; states inside a
+    block are in increasing Fock order;
+  * operator and propagator blocks are in the Fock basis of each block, as get_operators_impl
+    (atom_diag_utils.cpp:133-148) and ad_to_atom_prop build them.
+"""
+import itertools
+
+import numpy as np
+import scipy.sparse as sp
+
+
+class FockSpace:
+    def __init__(self, n):
+        self.n = n
+        self.dim = 1 << n
+        s = np.arange(self.dim)
+        self._c = []
+        for o in range(n):
+            occ = (s >> o) & 1
+            src = s[occ == 1]
+            dst = src ^ (1 << o)
+            below = src & ((1 << o) - 1)
+            sign = np.where(np.array([bin(x).count("1") for x in below]) % 2 == 0, 1.0, -1.0)
+            self._c.append(sp.csr_matrix((sign, (dst, src)), shape=(self.dim, self.dim)))
+
+    def c(self, o):
+        return self._c[o]
+
+    def cdag(self, o):
+        return self._c[o].T.tocsr()
+
+    def nop(self, o):
+        return (self.cdag(o) @ self.c(o)).tocsr()
+
+
+class AtomDiag:
+    """Block-diagonalised local Hamiltonian with block-sparse c and c^dagger.
+
+    H: (dim, dim) real matrix (dense or sparse) in the Fock basis; spin[o] = +1/-1 for S_z blocks.
+    """
+
+    def __init__(self, fs, H, qn="NSz", spin=None):
+        self.fs = fs
+        n, dim = fs.n, fs.dim
+        H = sp.csr_matrix(H)
+        states = np.arange(dim)
+        N = np.array([bin(s).count("1") for s in states])
+        if qn == "NSz":
+            spin = np.asarray(spin)
+            sz = np.array([sum(spin[o] for o in range(n) if (s >> o) & 1) for s in states])
+            keys = list(zip(N, sz))
+        elif qn == "N":
+            keys = list(N)
+        elif qn == "none":
+            keys = [0] * dim
+        else:
+            raise ValueError(qn)
+        order = {}
+        for s, k in zip(states, keys):
+            order.setdefault(k, []).append(s)
+        self.fock_states = [np.array(v) for v in order.values()]
+        self.nblocks = len(self.fock_states)
+        self.dims = np.array([len(f) for f in self.fock_states], dtype=np.int32)
+        self.block_of = np.empty(dim, dtype=np.int64)
+        self.pos_in_block = np.empty(dim, dtype=np.int64)
+        for b, f in enumerate(self.fock_states):
+            self.block_of[f] = b
+            self.pos_in_block[f] = np.arange(len(f))
+        Hc = H.tocoo()
+        if np.any(self.block_of[Hc.row] != self.block_of[Hc.col]):
+            raise ValueError("Hamiltonian couples different blocks of the chosen quantum numbers")
+        Hd = H.toarray()
+        self.H_blocks = [Hd[np.ix_(f, f)] for f in self.fock_states]
+        self.eigvals, self.eigvecs = [], []
+        for hb in self.H_blocks:
+            E, U = np.linalg.eigh(hb)
+            self.eigvals.append(E)
+            self.eigvecs.append(U)
+        self.gs_energy = min(E.min() for E in self.eigvals)
+        self.energies = [E - self.gs_energy for E in self.eigvals]
+        # block-sparse operators
+        self.c_conn = -np.ones((n, self.nblocks), dtype=np.int32)
+        self.cdag_conn = -np.ones((n, self.nblocks), dtype=np.int32)
+        self.c_blocks = {}
+        self.cdag_blocks = {}
+        for o in range(n):
+            for which, conn, store in ((fs.c(o), self.c_conn, self.c_blocks), (fs.cdag(o), self.cdag_conn, self.cdag_blocks)):
+                M = which.tocsc()
+                for j, fj in enumerate(self.fock_states):
+                    sub = M[:, fj].tocoo()
+                    if sub.nnz == 0:
+                        continue
+                    tgt = np.unique(self.block_of[sub.row])
+                    if len(tgt) != 1:
+                        raise ValueError("operator maps one block into several blocks")
+                    t = int(tgt[0])
+                    conn[o, j] = t
+                    store[(o, j)] = M[self.fock_states[t], :][:, fj].toarray()
+
+    def packed_ops(self):
+        """Concatenated c and c^dagger blocks in (o, j) order, as xca_problem expects."""
+        c = [self.c_blocks[(o, j)].ravel() for o in range(self.fs.n) for j in range(self.nblocks) if self.c_conn[o, j] >= 0]
+        d = [self.cdag_blocks[(o, j)].ravel() for o in range(self.fs.n) for j in range(self.nblocks) if self.cdag_conn[o, j] >= 0]
+        return np.concatenate(c) if c else np.zeros(0), np.concatenate(d) if d else np.zeros(0)
+
+    def atomic_G0(self, beta, tau):
+        """G0_b(tau) = -U exp(-tau (E - eta0)) U^T and eta0 = -ln Z0 / beta with ground-state-shifted
+        energies (block_sparse_solver.py:1305-1328). tau: absolute times (r,). Returns (blocks, eta0)."""
+        Z0 = sum(np.exp(-beta * E).sum() for E in self.energies)
+        eta0 = -np.log(Z0) / beta
+        G = []
+        for E, U in zip(self.energies, self.eigvecs):
+            d = -np.exp(-tau[:, None] * (E - eta0)[None, :])
+            G.append(np.einsum("ia,ta,ja->tij", U, d, U))
+        return G, eta0
+
+    def dense_H(self):
+        dim = self.fs.dim
+        Hd = np.zeros((dim, dim))
+        for f, hb in zip(self.fock_states, self.H_blocks):
+            Hd[np.ix_(f, f)] = hb
+        return Hd
+
+
+# --------------------------------------------------------------------------- Hamiltonians
+
+def kanamori_hamiltonian(fs, norb, U, J, mu):
+    """Kanamori interaction with spin flip and pair hopping, U' = U - 2J, flavours ordered
+    (do, 0..norb-1), (up, 0..norb-1) (test/c++/block_sparse_utils.cpp:189-208)."""
+    do = lambda o: o
+    up = lambda o: norb + o
+    nn = lambda a: fs.nop(a)
+    cd, c = fs.cdag, fs.c
+    Up = U - 2 * J
+    H = sp.csr_matrix((fs.dim, fs.dim))
+    for o in range(norb):
+        H = H - mu * (nn(up(o)) + nn(do(o)))
+        H = H + U * (nn(up(o)) @ nn(do(o)))
+    for o1 in range(norb):
+        for o2 in range(norb):
+            if o1 == o2:
+                continue
+            H = H + 0.5 * Up * (nn(up(o1)) @ nn(do(o2)) + nn(do(o1)) @ nn(up(o2)))
+            H = H + 0.5 * (Up - J) * (nn(up(o1)) @ nn(up(o2)) + nn(do(o1)) @ nn(do(o2)))
+            H = H - J * (cd(up(o1)) @ c(do(o1)) @ cd(do(o2)) @ c(up(o2)))
+            H = H - J * (cd(up(o1)) @ cd(do(o1)) @ c(up(o2)) @ c(do(o2)))
+    return H.tocsr()
+
+
+def kanamori_spins(norb):
+    return np.array([-1] * norb + [1] * norb)
+
+
+# --------------------------------------------------------------------------- hybridization
+
+def semicircle_delta_tau(tau_abs, beta, D=1.0, t2=0.25, nquad=4000):
+    """Delta(tau) = t2 * int rho(e) K(tau/beta, beta e) de for the semicircular DOS of half-bandwidth
+    D, rho(e) = 2 sqrt(D^2 - e^2)/(pi D^2); Gauss-Chebyshev (second kind) quadrature."""
+    k = np.arange(1, nquad + 1)
+    th = k * np.pi / (nquad + 1)
+    x = np.cos(th)
+    w = np.pi / (nquad + 1) * np.sin(th) ** 2
+    e = D * x
+    w = w * (2.0 / np.pi)
+    t = np.asarray(tau_abs)[:, None] / beta
+    om = beta * e[None, :]
+    # K(t, om) for t in [0, 1], om any sign, without overflow
+    Kv = np.where(om >= 0, -np.exp(-t * om) / (1 + np.exp(-np.abs(om))), -np.exp((1 - t) * om) / (1 + np.exp(om)))
+    return t2 * (Kv * w[None, :]).sum(axis=1)
+
+
+def dlr_hybridization(grid, delta_vals):
+    """Poles (energy units) and DLR coefficients (p, n, n) of a hybridization given by its values
+    (r, n, n) at the DLR nodes, without pole compression (block_sparse_solver.py:181-185)."""
+    coeffs = grid.vals2coefs(delta_vals)
+    poles = grid.rf / grid.beta
+    return poles, coeffs

@@ -1,0 +1,137 @@
+"""Impurity problems used by the tests and the benchmark: the reference's test models and the five
+benchmark workloads (single-orbital Anderson NCA, 2- and 3-orbital Kanamori OCA/TCA, serving mix,
+5-orbital Slater-Condon)."""
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.sparse as sp
+
+from .atom import AtomDiag, FockSpace, dlr_hybridization, kanamori_hamiltonian, kanamori_spins, semicircle_delta_tau
+from .dlr import DLRGrid, kernel
+
+
+@dataclass
+class ImpurityProblem:
+    grid: DLRGrid
+    ad: AtomDiag
+    poles: np.ndarray          # (p,) energy units
+    coeffs: np.ndarray         # (p, n, n) real
+    G0: list = field(default_factory=list)   # atomic propagator blocks (r, b, b)
+    eta0: float = 0.0
+    name: str = ""
+
+    @property
+    def beta(self):
+        return self.grid.beta
+
+    @property
+    def n(self):
+        return self.ad.fs.n
+
+    @property
+    def dims(self):
+        return self.ad.dims
+
+    def pack(self, blocks):
+        return np.concatenate([np.ascontiguousarray(b, dtype=np.float64).ravel() for b in blocks])
+
+    def unpack(self, flat):
+        out, off = [], 0
+        r = self.grid.r
+        for d in self.dims:
+            sz = r * d * d
+            out.append(flat[off:off + sz].reshape(r, d, d))
+            off += sz
+        return out
+
+    def perturbed_G(self, seed=20260101, amp=1e-3):
+        """Atomic propagator with DLR coefficients scaled by (1 + u), u ~ U[-amp, amp] iid per
+        (block, coefficient, i, j)."""
+        rng = np.random.default_rng(seed)
+        G = []
+        for g in self.G0:
+            c = self.grid.vals2coefs(g)
+            c = c * (1.0 + rng.uniform(-amp, amp, size=c.shape))
+            G.append(self.grid.coefs2vals(c))
+        return G
+
+
+def _problem(grid, ad, delta_vals, name):
+    poles, coeffs = dlr_hybridization(grid, delta_vals)
+    G0, eta0 = ad.atomic_G0(grid.beta, grid.tau)
+    return ImpurityProblem(grid=grid, ad=ad, poles=poles, coeffs=np.real(coeffs), G0=G0, eta0=eta0, name=name)
+
+
+def one_fermion(beta=2.0, lam=40.0, eps=1e-10, pole=0.0):
+    """H = 0, one spinless fermion, one-pole hybridization with unit weight
+    (test/c++/block_sparse_utils.cpp:17-44)."""
+    grid = DLRGrid(beta, lam, eps)
+    fs = FockSpace(1)
+    ad = AtomDiag(fs, sp.csr_matrix((2, 2)), qn="N")
+    G0, eta0 = ad.atomic_G0(beta, grid.tau)
+    return ImpurityProblem(grid=grid, ad=ad, poles=np.array([pole]), coeffs=np.ones((1, 1, 1)), G0=G0, eta0=eta0, name="one_fermion")
+
+
+def two_fermion(beta=2.0, lam=40.0, eps=1e-12, U=3.0, mu=0.0, pole=0.0):
+    """H = -mu (n0 + n1) + U n0 n1, blocks by N, hybridization one pole with identity weight
+    (test/c++/block_sparse_utils.cpp:46-76)."""
+    grid = DLRGrid(beta, lam, eps)
+    fs = FockSpace(2)
+    H = -mu * (fs.nop(0) + fs.nop(1)) + U * (fs.nop(0) @ fs.nop(1))
+    ad = AtomDiag(fs, H, qn="N")
+    G0, eta0 = ad.atomic_G0(beta, grid.tau)
+    return ImpurityProblem(grid=grid, ad=ad, poles=np.array([pole]), coeffs=np.eye(2)[None], G0=G0, eta0=eta0, name="two_fermion")
+
+
+def n_fermion_h0(nflav, beta=2.0, lam=20.0, eps=1e-12):
+    """n flavours, H = 0, constant hybridization Delta = -1/2 on the diagonal (a single pole at 0 with
+    unit weight), one block: the analytic model of test/python/analytic_diagrams.py:82-122."""
+    grid = DLRGrid(beta, lam, eps)
+    fs = FockSpace(nflav)
+    ad = AtomDiag(fs, sp.csr_matrix((fs.dim, fs.dim)), qn="none")
+    G0, eta0 = ad.atomic_G0(beta, grid.tau)
+    return ImpurityProblem(grid=grid, ad=ad, poles=np.array([0.0]), coeffs=np.eye(nflav)[None], G0=G0, eta0=eta0, name=f"n{nflav}_h0")
+
+
+def anderson_nca(beta=10.0, lam=100.0, eps=1e-10, U=2.0, mu=1.0, t2=0.25):
+    """Single-orbital Anderson impurity, H = U n_up n_do - mu (n_up + n_do), blocks by N, semicircular
+    hybridization (half-bandwidth 1, t^2 = 0.25) expanded in DLR."""
+    grid = DLRGrid(beta, lam, eps)
+    fs = FockSpace(2)
+    H = U * (fs.nop(0) @ fs.nop(1)) - mu * (fs.nop(0) + fs.nop(1))
+    ad = AtomDiag(fs, H, qn="N")
+    d = semicircle_delta_tau(grid.tau, beta, t2=t2)
+    delta = d[:, None, None] * np.eye(2)[None]
+    return _problem(grid, ad, delta, "anderson_nca")
+
+
+def kanamori(norb, U, J, mu, beta, lam, eps=1e-10, t2=0.25, offdiag=0.0):
+    """Kanamori impurity with (N, S_z) blocks and a semicircular hybridization per flavour; offdiag
+    couples different orbitals of the same spin with relative weight `offdiag`."""
+    grid = DLRGrid(beta, lam, eps)
+    fs = FockSpace(2 * norb)
+    H = kanamori_hamiltonian(fs, norb, U, J, mu)
+    ad = AtomDiag(fs, H, qn="NSz", spin=kanamori_spins(norb))
+    d = semicircle_delta_tau(grid.tau, beta, t2=t2)
+    n = 2 * norb
+    M = np.zeros((n, n))
+    for a in range(n):
+        for b in range(n):
+            if a == b:
+                M[a, b] = 1.0
+            elif (a < norb) == (b < norb):
+                M[a, b] = offdiag
+    delta = d[:, None, None] * M[None]
+    return _problem(grid, ad, delta, f"kanamori{norb}")
+
+
+def oca_2orb():
+    """2-orbital Kanamori, U = 4, J = 0.5, half filling mu = (3U - 5J)/2, beta = 50, Lambda = 1000."""
+    U, J = 4.0, 0.5
+    return kanamori(2, U, J, (3 * U - 5 * J) / 2, beta=50.0, lam=1000.0)
+
+
+def tca_3orb():
+    """3-orbital Kanamori, U = 5, J = 0.7, filling 2 (mu = 4.35), beta = 100, Lambda = 2000,
+    off-diagonal inter-orbital hybridization 0.05."""
+    return kanamori(3, 5.0, 0.7, 4.35, beta=100.0, lam=2000.0, offdiag=0.05)
