@@ -1,0 +1,443 @@
+// HIP kernels of the pseudo-particle self-energy engine (gfx950, FP64 throughout).
+//
+// Layout of one tile in LDS: two slabs of RP x LDN doubles (RP = 16 * RB padded DLR rank; rows are
+// DLR nodes, columns are (item, row, column) triples of the per-item state T(t) for up to LDN
+// columns). Items of one tile share a program (one block path and rule class) and differ by their
+// pole tuple. Every node-space transform is an FP64 MFMA GEMM (v_mfma_f64_16x16x4f64) with the
+// DLR operator as the A operand (pre-packed in lane order in global memory) and the tile as the B
+// operand (LDS); each wave owns one 16-column block of the tile.
+//
+// Edge integration h = int_0^t GK(t-s) T(s) ds follows the time-ordered DLR convolution:
+//   g  = vals2coefs(T)   : (L^{-1} P) T by MFMA, then back substitution with U in 4-row steps
+//                          (4x4 diagonal-block inverses, rank-4 MFMA trailing updates) —
+//                          backward stable, unlike a product with the explicit inverse of cf2it;
+//   Wg = W g             : MFMA;
+//   c  = -(Wf o g + f o Wg),  d = f o g   (per node, per item: small products, vector ALU);
+//   h  = beta (cf2it c + tcf2it d)        : one MFMA GEMM with K = 2 RP.
+// f = vals2coefs(GK) and W f come from edge tables computed by tab_kernel with the same routines.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "engine.hpp"
+
+namespace xca::dev {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int kMaxB = 12;   // largest block handled by the tile kernels
+
+__device__ __forceinline__ v4d mfma_f64(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// acc[rb] += sum_{ks in [k0, k1)} A[rb][ks] * S[4 (ks - k0) + (lane >> 4)][c0 + (lane & 15)]
+// A fragments packed as Afrag[(rb * ldk + ks) * 64 + lane].
+template <int RB>
+__device__ __forceinline__ void gemm_acc(const double *__restrict__ Afrag, int ldk, int k0, int k1, const double *S, int ldn, int c0,
+                                         v4d (&acc)[RB], int lane) {
+  const int krow = lane >> 4, col = c0 + (lane & 15);
+  for (int ks = k0; ks < k1; ++ks) {
+    double b = S[(size_t)(4 * (ks - k0) + krow) * ldn + col];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) acc[rb] = mfma_f64(Afrag[((size_t)rb * ldk + ks) * 64 + lane], b, acc[rb]);
+  }
+}
+
+template <int RB>
+__device__ __forceinline__ void zero_acc(v4d (&acc)[RB]) {
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb) acc[rb] = v4d{0.0, 0.0, 0.0, 0.0};
+}
+
+// Accumulator layout of v_mfma_f64_16x16x4f64 (measured on gfx950): element (i, j) of the 16x16
+// block sits in lane j + 16 (i % 4), register i / 4. Row of register v in this lane:
+__device__ __forceinline__ int acc_row(int rb, int v, int lane) { return 16 * rb + 4 * v + (lane >> 4); }
+
+// store accumulators to LDS, column c0 + (lane & 15)
+template <int RB>
+__device__ __forceinline__ void store_acc(const v4d (&acc)[RB], double *S, int ldn, int c0, int lane) {
+  const int col = c0 + (lane & 15);
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) S[(size_t)acc_row(rb, v, lane) * ldn + col] = acc[rb][v];
+}
+
+// Back substitution U x = y on the accumulators (rows of y in acc, one 16-column block per wave),
+// 4 rows per step from the bottom. Rows 4s..4s+3 of step s live in register s % 4 of row block s / 4,
+// row 4s + k in lane group k: exactly the B-operand layout of the rank-4 trailing update, so the
+// solved rows feed the MFMA directly. The 4x4 diagonal block is applied through its inverse.
+template <int RB>
+__device__ __forceinline__ void usolve(v4d (&acc)[RB], const DlrFrag &D, int lane) {
+  const int k = lane >> 4, j = lane & 15;
+#pragma unroll
+  for (int s = 4 * RB - 1; s >= 0; --s) {
+    const int rbI = s >> 2, v = s & 3;
+    const double ys = acc[rbI][v];
+    const double y0 = __shfl(ys, j, 64), y1 = __shfl(ys, j + 16, 64), y2 = __shfl(ys, j + 32, 64), y3 = __shfl(ys, j + 48, 64);
+    const double *Di = D.Dinv + 16 * s + 4 * k;
+    const double x = Di[0] * y0 + Di[1] * y1 + Di[2] * y2 + Di[3] * y3;
+    acc[rbI][v] = x;
+    if (s > 0) {
+#pragma unroll
+      for (int rb = 0; rb <= rbI; ++rb) acc[rb] = mfma_f64(D.AU[((size_t)s * RB + rb) * 64 + lane], x, acc[rb]);
+    }
+  }
+}
+
+// Coefficients of the RP x 16 column block c0 of slab S: acc <- U^{-1} L^{-1} P S
+template <int RB>
+__device__ __forceinline__ void vals2coefs_block(const DlrFrag &D, const double *S, int ldn, int c0, v4d (&acc)[RB], int lane) {
+  zero_acc(acc);
+  gemm_acc<RB>(D.LiP, 4 * RB, 0, 4 * RB, S, ldn, c0, acc, lane);
+  usolve<RB>(acc, D, lane);
+}
+
+// -------------------------------------------------------------------------------------------
+// Edge-function tables: f = vals2coefs(G_b K...) and W f for every entry of a table job.
+
+template <int RB, int NW>
+__global__ void __launch_bounds__(64 * NW) tab_kernel(DlrFrag D, const DTabJob *__restrict__ jobs, const double *__restrict__ G,
+                                                      const double *__restrict__ kp, const double *__restrict__ km, int p,
+                                                      double *__restrict__ ftab) {
+  constexpr int RP = 16 * RB, NT = 64 * NW, LDN = 16 * NW + 8;
+  extern __shared__ double lds[];
+  double *SA = lds, *SB = lds + RP * LDN, *xch = lds + 2 * RP * LDN;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const DTabJob J = jobs[blockIdx.x];
+  const int r = D.r, b = J.b, bb = b * b, ncol = J.ne * bb;
+  for (int x = tid; x < RP * LDN; x += NT) { SA[x] = 0.0; SB[x] = 0.0; }
+  __syncthreads();
+  // GK(t) for each entry
+  for (int x = tid; x < r * ncol; x += NT) {
+    const int t = x / ncol, c = x % ncol, e = c / bb, ij = c % bb;
+    double v = G[J.g_off + (size_t)t * bb + ij];
+    int ee = J.e0 + e;
+    for (int q = 0; q < J.nlines; ++q) {
+      const int l = ee % p;
+      ee /= p;
+      v *= (J.lsign[q] > 0 ? kp : km)[(size_t)l * r + t];
+    }
+    SA[(size_t)t * LDN + c] = v;
+  }
+  __syncthreads();
+  v4d acc[RB];
+  const int c0 = 16 * wv;
+  vals2coefs_block<RB>(D, SA, LDN, c0, acc, lane);
+  store_acc<RB>(acc, SB, LDN, c0, lane);
+  // f to global
+  {
+    const int col = c0 + (lane & 15);
+    if (col < ncol) {
+      const int e = col / bb, ij = col % bb;
+      double *out = ftab + J.out_off + (size_t)(J.e0 + e) * 2 * r * bb;
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int t = acc_row(rb, v, lane);
+          if (t < r) out[(size_t)t * bb + ij] = acc[rb][v];
+        }
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  zero_acc(acc);
+  gemm_acc<RB>(D.W, 4 * RB, 0, 4 * RB, SB, LDN, c0, acc, lane);
+  {
+    const int col = c0 + (lane & 15);
+    if (col < ncol) {
+      const int e = col / bb, ij = col % bb;
+      double *out = ftab + J.out_off + (size_t)(J.e0 + e) * 2 * r * bb + (size_t)r * bb;
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int t = acc_row(rb, v, lane);
+          if (t < r) out[(size_t)t * bb + ij] = acc[rb][v];
+        }
+    }
+  }
+}
+
+// -------------------------------------------------------------------------------------------
+// Chain kernel: one workgroup per chunk of tiles of one program.
+
+
+template <int RB, int NW>
+__global__ void __launch_bounds__(64 * NW) chain_kernel(ChainArgs A) {
+  constexpr int RP = 16 * RB, NT = 64 * NW, NTILE = 16 * NW, LDN = NTILE + 8;
+  extern __shared__ double lds[];
+  double *S0 = lds, *S1 = lds + RP * LDN, *xch = lds + 2 * RP * LDN;
+  double *ipref = xch + 64 * NW;                               // per item prefactor (NTILE)
+  unsigned long long *imax = (unsigned long long *)(ipref + NTILE);  // per item max |T|
+  int *ipole = (int *)(imax + NTILE);                          // (NTILE, kMaxLines)
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int r = A.r;
+  const DlrFrag &D = A.D;
+  const I2 ch = A.chunks[blockIdx.x];
+  const int64_t slot = A.chunk_slot[blockIdx.x];
+
+  for (int x = tid; x < RP * LDN; x += NT) { S0[x] = 0.0; S1[x] = 0.0; }
+  bool first_tile = true;
+  for (int ti = ch.x; ti < ch.y; ++ti) {
+    const DTile tile = A.tiles[ti];
+    const DProg P = A.progs[tile.prog];
+    const int P_items = tile.count;
+    __syncthreads();
+    // pole tuple of each item, prefactor
+    for (int it = tid; it < P_items; it += NT) {
+      int u = tile.start + it;
+      double pref = 1.0;
+      int ls[kMaxLines];
+#pragma unroll
+      for (int x = 0; x < kMaxLines; ++x) {
+        ls[x] = 0;
+        if (x < P.nlines) {
+          ls[x] = A.plist[P.set_off[x] + u % P.set_len[x]];
+          u /= P.set_len[x];
+        }
+        ipole[it * kMaxLines + x] = ls[x];
+      }
+#pragma unroll
+      for (int x = 0; x < kMaxLines; ++x)
+        if (x < P.nlines && P.pk_exp[x] != 0)
+          pref *= A.kpow[((size_t)(P.pk_sign[x] > 0 ? 0 : 1) * A.maxexp + (P.pk_exp[x] - 1)) * A.p + ls[x]];
+      ipref[it] = pref * (double)P.sign;
+      imax[it] = 0ull;
+    }
+    // T = G_b1 for every item
+    {
+      const int b1 = P.b1, w = b1 * b1;
+      for (int x = tid; x < r * P_items * w; x += NT) {
+        const int t = x / (P_items * w), c = x % (P_items * w);
+        S0[(size_t)t * LDN + c] = A.G[P.g_init + (size_t)t * w + c % w];
+      }
+    }
+    __syncthreads();
+    double *Scur = S0, *Soth = S1;
+    int rows = P.b1, cols = P.b1;
+    for (int oi = 0; oi < P.nops; ++oi) {
+      const DOp op = A.ops[P.op0 + oi];
+      if (op.type == 0) {  // vertex: T' = K F T
+        const int bin = op.bin, bout = op.bout, win = bin * cols, wout = bout * cols;
+        for (int x = tid; x < r * P_items * cols; x += NT) {
+          const int t = x / (P_items * cols), rem = x % (P_items * cols), it = rem / cols, j = rem % cols;
+          double h[kMaxB];
+#pragma unroll
+          for (int k = 0; k < kMaxB; ++k) h[k] = (k < bin) ? Scur[(size_t)t * LDN + it * win + k * cols + j] : 0.0;
+          const double *F = A.opmats + op.fbase + (op.bar_line >= 0 ? (int64_t)ipole[it * kMaxLines + op.bar_line] * op.fstride_l : 0);
+          double kf = 1.0;
+          if (op.ks != 0) kf = (op.ks > 0 ? A.kp : A.km)[(size_t)ipole[it * kMaxLines + op.kx] * r + t];
+          for (int i = 0; i < bout; ++i) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < kMaxB; ++k)
+              if (k < bin) s += F[i * bin + k] * h[k];
+            Soth[(size_t)t * LDN + it * wout + i * cols + j] = kf * s;
+          }
+        }
+        rows = bout;
+        double *tmp = Scur; Scur = Soth; Soth = tmp;
+        __syncthreads();
+      } else if (op.type == 1) {  // edge integration
+        const int b = op.bin, w = b * cols;
+        const int c0 = 16 * wv;
+        v4d acc[RB];
+        // g = vals2coefs(T) -> Soth ; W g -> Scur (T is dead after the GEMM of this wave's block)
+        vals2coefs_block<RB>(D, Scur, LDN, c0, acc, lane);
+        store_acc<RB>(acc, Soth, LDN, c0, lane);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        zero_acc(acc);
+        gemm_acc<RB>(D.W, 4 * RB, 0, 4 * RB, Soth, LDN, c0, acc, lane);
+        store_acc<RB>(acc, Scur, LDN, c0, lane);
+        __syncthreads();
+        // c = -(Wf o g + f o Wg) -> Soth, d = f o g -> Scur
+        for (int x = tid; x < r * P_items * cols; x += NT) {
+          const int t = x / (P_items * cols), rem = x % (P_items * cols), it = rem / cols, j = rem % cols;
+          double g[kMaxB], wg[kMaxB];
+#pragma unroll
+          for (int k = 0; k < kMaxB; ++k) {
+            g[k] = (k < b) ? Soth[(size_t)t * LDN + it * w + k * cols + j] : 0.0;
+            wg[k] = (k < b) ? Scur[(size_t)t * LDN + it * w + k * cols + j] : 0.0;
+          }
+          int64_t e = op.ebase;
+#pragma unroll
+          for (int q = 0; q < kMaxLines; ++q)
+            if (q < op.nlines) e += (int64_t)ipole[it * kMaxLines + op.lines[q]] * op.estride[q];
+          const double *f = A.ftab + e + (size_t)t * b * b;
+          const double *wf = f + (size_t)r * b * b;
+          for (int i = 0; i < b; ++i) {
+            double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < kMaxB; ++k)
+              if (k < b) {
+                const double fv = f[i * b + k];
+                s1 += wf[i * b + k] * g[k] + fv * wg[k];
+                s2 += fv * g[k];
+              }
+            Soth[(size_t)t * LDN + it * w + i * cols + j] = -s1;
+            Scur[(size_t)t * LDN + it * w + i * cols + j] = s2;
+          }
+        }
+        __syncthreads();
+        // h = beta (cf2it c + tcf2it d) -> Soth (own columns only after own GEMM)
+        zero_acc(acc);
+        gemm_acc<RB>(D.M2, 8 * RB, 0, 4 * RB, Soth, LDN, c0, acc, lane);
+        gemm_acc<RB>(D.M2, 8 * RB, 4 * RB, 8 * RB, Scur, LDN, c0, acc, lane);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        store_acc<RB>(acc, Soth, LDN, c0, lane);
+        // keep the padding rows of the slab that held d zero for the next GEMM that reads it
+        double *tmp = Scur; Scur = Soth; Soth = tmp;
+        __syncthreads();
+      } else {  // zero-vertex pair
+        const int bv = op.bin, bvp = op.bout, b1 = op.b1, b0 = op.cols, qs = op.qs, win = bv * b1, wout = bvp * b0;
+        const int n = A.n;
+        const int *orbs = A.ops[P.op0 + oi].orb;
+        for (int x = tid; x < r * P_items * b0; x += NT) {
+          const int t = x / (P_items * b0), rem = x % (P_items * b0), it = rem / b0, j = rem % b0;
+          double out[kMaxB];
+#pragma unroll
+          for (int i = 0; i < kMaxB; ++i) out[i] = 0.0;
+          for (int a = 0; a < bv; ++a) {
+            double tk[kMaxSet];
+#pragma unroll
+            for (int kk = 0; kk < kMaxSet; ++kk) {
+              tk[kk] = 0.0;
+              if (kk < qs) {
+                const double *Fk = A.opmats + op.fbase + (int64_t)kk * b1 * b0;
+                double s = 0.0;
+                for (int c = 0; c < b1; ++c) s += Scur[(size_t)t * LDN + it * win + a * b1 + c] * Fk[c * b0 + j];
+                tk[kk] = s;
+              }
+            }
+            for (int mu = 0; mu < qs; ++mu) {
+              double tm = 0.0;
+#pragma unroll
+              for (int kk = 0; kk < kMaxSet; ++kk)
+                if (kk < qs) {
+                  const double hv = op.forward ? A.delta[((size_t)t * n + orbs[mu]) * n + orbs[kk]]
+                                               : -A.delta_refl[((size_t)t * n + orbs[kk]) * n + orbs[mu]];
+                  tm += hv * tk[kk];
+                }
+              const double *Fm = A.opmats + op.mubase + (int64_t)mu * bvp * bv;
+#pragma unroll
+              for (int i = 0; i < kMaxB; ++i)
+                if (i < bvp) out[i] += Fm[i * bv + a] * tm;
+            }
+          }
+#pragma unroll
+          for (int i = 0; i < kMaxB; ++i)
+            if (i < bvp) Soth[(size_t)t * LDN + it * wout + i * b0 + j] = out[i];
+        }
+        rows = bvp;
+        cols = b0;
+        double *tmp = Scur; Scur = Soth; Soth = tmp;
+        __syncthreads();
+      }
+    }
+    // final: per item max |T| (with the prefactor) and accumulation into the chunk's partial slot
+    const int b0 = P.b0, w = b0 * b0;
+    for (int x = tid; x < P_items * r; x += NT) {
+      const int it = x / r, t = x % r;
+      double mx = 0.0;
+      for (int c = 0; c < w; ++c) mx = fmax(mx, fabs(Scur[(size_t)t * LDN + it * w + c] * ipref[it]));
+      atomicMax(&imax[it], (unsigned long long)__double_as_longlong(mx));
+    }
+    __syncthreads();
+    double *out = A.partial + slot;
+    for (int x = tid; x < r * w; x += NT) {
+      const int t = x / w, c = x % w;
+      double s = first_tile ? 0.0 : out[x];
+      for (int it = 0; it < P_items; ++it)
+        if (__longlong_as_double((long long)imax[it]) > 1e-16) s += Scur[(size_t)t * LDN + it * w + c] * ipref[it];
+      out[x] = s;
+    }
+    first_tile = false;
+    (void)rows;
+  }
+}
+
+// Sigma[block] += scale * sum over the block's chunks (fixed order) of the partial slots.
+__global__ void reduce_kernel(const int64_t *__restrict__ slot, const int *__restrict__ red_chunks, const I2 *__restrict__ red_range,
+                              const int64_t *__restrict__ sig_off, const int *__restrict__ sig_size, const double *__restrict__ partial,
+                              double *__restrict__ sigma, double scale) {
+  const int blk = blockIdx.y;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= sig_size[blk]) return;
+  const I2 rg = red_range[blk];
+  double s = 0.0;
+  for (int c = rg.x; c < rg.y; ++c) s += partial[slot[red_chunks[c]] + x];
+  if (rg.y > rg.x) sigma[sig_off[blk] + x] += scale * s;
+}
+
+// -------------------------------------------------------------------------------------------
+// host-callable launchers
+
+template <int RB, int NW>
+static size_t lds_bytes() {
+  constexpr int RP = 16 * RB, NTILE = 16 * NW, LDN = NTILE + 8;
+  return sizeof(double) * ((size_t)2 * RP * LDN + 64 * NW + NTILE + NTILE) + sizeof(int) * NTILE * kMaxLines;
+}
+
+template <int RB, int NW>
+static hipError_t launch_tab_t(const DlrFrag &D, const DTabJob *jobs, int njobs, const double *G, const double *kp, const double *km, int p,
+                               double *ftab, hipStream_t s) {
+  constexpr int RP = 16 * RB, LDN = 16 * NW + 8;
+  size_t sh = sizeof(double) * ((size_t)2 * RP * LDN + 64 * NW);
+  hipError_t e = hipFuncSetAttribute((const void *)tab_kernel<RB, NW>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((tab_kernel<RB, NW>), dim3(njobs), dim3(64 * NW), sh, s, D, jobs, G, kp, km, p, ftab);
+  return hipGetLastError();
+}
+
+template <int RB, int NW>
+static hipError_t launch_chain_t(const ChainArgs &A, int nchunks, hipStream_t s) {
+  size_t sh = lds_bytes<RB, NW>();
+  hipError_t e = hipFuncSetAttribute((const void *)chain_kernel<RB, NW>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((chain_kernel<RB, NW>), dim3(nchunks), dim3(64 * NW), sh, s, A);
+  return hipGetLastError();
+}
+
+}  // namespace xca::dev
+
+// entry points used by the host orchestration (context.cpp)
+namespace xca::dev {
+
+int tile_columns() { return 128; }
+
+hipError_t launch_tab(int RB, const DlrFrag &D, const DTabJob *jobs, int njobs, const double *G, const double *kp, const double *km, int p,
+                      double *ftab, hipStream_t s) {
+  switch (RB) {
+    case 2: return launch_tab_t<2, 8>(D, jobs, njobs, G, kp, km, p, ftab, s);
+    case 3: return launch_tab_t<3, 8>(D, jobs, njobs, G, kp, km, p, ftab, s);
+    case 4: return launch_tab_t<4, 8>(D, jobs, njobs, G, kp, km, p, ftab, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_chain(int RB, const void *args, int nchunks, hipStream_t s) {
+  const ChainArgs &A = *static_cast<const ChainArgs *>(args);
+  switch (RB) {
+    case 2: return launch_chain_t<2, 8>(A, nchunks, s);
+    case 3: return launch_chain_t<3, 8>(A, nchunks, s);
+    case 4: return launch_chain_t<4, 8>(A, nchunks, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_reduce(const int64_t *slot, const int *red_chunks, const I2 *red_range, const int64_t *sig_off, const int *sig_size,
+                         int nblocks, int max_size, const double *partial, double *sigma, double scale, hipStream_t s) {
+  dim3 grid((max_size + 255) / 256, nblocks);
+  hipLaunchKernelGGL(reduce_kernel, grid, dim3(256), 0, s, slot, red_chunks, red_range, sig_off, sig_size, partial, sigma, scale);
+  return hipGetLastError();
+}
+
+size_t chain_args_size() { return sizeof(ChainArgs); }
+
+}  // namespace xca::dev

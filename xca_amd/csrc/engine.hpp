@@ -1,0 +1,72 @@
+// Device data structures shared by the host orchestration and the HIP kernels.
+#pragma once
+#include <cstdint>
+
+namespace xca::dev {
+
+constexpr int kMaxLines = 3;   // orders up to 4
+constexpr int kMaxSet = 8;     // largest symmetry set handled by the zero-vertex op
+
+// One vertex/edge/zero-vertex operation (see host/model.hpp for the semantics).
+struct DOp {
+  int type, bin, bout, cols;
+  int64_t fbase, fstride_l;      // V: operator matrix, bar stride per pole; Z: kappa operators
+  int bar_line, kx, ks, nlines;  // V: bar line, K line and sign; E: number of K lines
+  int lines[kMaxLines];
+  int pad0;
+  int64_t ebase;                 // E: edge-table entry 0 offset (doubles)
+  int64_t estride[kMaxLines];    // E: entry stride per line pole (doubles)
+  int64_t mubase;                // Z: mu operators
+  int qs, forward, b1, set;      // Z
+  int orb[kMaxSet];              // Z: orbitals of the set
+};
+
+struct DProg {
+  int op0, nops, sigma_block, b0;
+  int b1, g_init, sign, nlines;
+  int pk_sign[kMaxLines], pk_exp[kMaxLines];
+  int set_off[kMaxLines], set_len[kMaxLines];  // pole lists per line (into the pole-list array)
+  int64_t slot_base;                            // not used by the chain kernel
+  int items_per_tile, max_w;
+};
+
+struct DTile {
+  int prog, start, count, chunk;
+};
+
+// Edge-function table job: entries [e0, e0 + ne) of one table
+struct DTabJob {
+  int block, b, nlines, e0;
+  int ne, g_off, pad0, pad1;
+  int lsign[kMaxLines], pad2;
+  int64_t out_off;   // doubles, entry 0 of this table
+};
+
+struct I2 {
+  int x, y;
+};
+
+// DLR operator fragments for the FP64 MFMA path, packed in lane order (see engine.hip).
+struct DlrFrag {
+  const double *LiP;   // (RB, KS, 64): A fragments of L^{-1} P, KS = RP/4
+  const double *W;     // (RB, KS, 64)
+  const double *M2;    // (RB, 2*KS, 64): beta [cf2it | tcf2it]
+  const double *AU;    // upper-triangular trailing-update fragments, per (step, row block)
+  const double *Dinv;  // (RP/4, 16): inverses of the 4x4 diagonal blocks of U
+  int r, RP;
+};
+
+struct ChainArgs {
+  DlrFrag D;
+  const DOp *ops;
+  const DProg *progs;
+  const DTile *tiles;
+  const I2 *chunks;            // (tile begin, tile end)
+  const int64_t *chunk_slot;   // partial-slot offset (doubles) per chunk
+  const int *plist;            // pole lists
+  const double *G, *opmats, *ftab, *kp, *km, *delta, *delta_refl, *kpow;
+  double *partial;
+  int r, p, n, maxexp;
+};
+
+}  // namespace xca::dev

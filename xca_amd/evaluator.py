@@ -1,0 +1,140 @@
+"""Python facade of the device evaluator with the reference's method names and semantics.
+
+Mirrors triqs_xca.block_sparse.DiagramEvaluator (python bindings block_sparse.wrap.cxx:173-208, C++
+c++/triqs_xca/block_sparse_backbone.hpp:23-165): compute_self_energy(G, topology[, f_ix]),
+get_num_self_energy_backbones(topology), reset(). G and the result are lists of (r, b, b) blocks (the
+data of the reference's BlockGf). Inputs must be real (imaginary parts exactly zero); the device path
+computes in real FP64 and raises otherwise.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import XcaError, XcaProblem, as_f64, as_i32, check, dptr, i64_p, iptr, last_error, lib
+from .diag import all_connected_pairings
+
+
+def _real(a, what):
+    a = np.asarray(a)
+    if np.iscomplexobj(a):
+        if np.any(a.imag != 0):
+            raise XcaError(f"{what}: the device path is real FP64; imaginary parts must be zero")
+        a = a.real
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class DiagramEvaluator:
+    """Pseudo-particle self-energy evaluator on one GPU.
+
+    Parameters mirror DiagramEvaluator(hyb_poles, hyb_coeffs, tau_mesh, atom_diag): poles in energy
+    units (p,), coefficients (p, n, n), a DLR grid (xca_amd.dlr.DLRGrid) and the block-sparse operators
+    of an xca_amd.atom.AtomDiag.
+    """
+
+    def __init__(self, hyb_poles, hyb_coeffs, grid, ad, device=0):
+        self.grid = grid
+        self.ad = ad
+        self.beta = grid.beta
+        self.r = grid.r
+        self.n = ad.fs.n
+        self.dims = np.asarray(ad.dims, dtype=np.int32)
+        self._keep = []
+
+        def keep(a):
+            self._keep.append(a)
+            return a
+
+        poles = keep(as_f64(hyb_poles).ravel())
+        coeffs = keep(_real(hyb_coeffs, "hyb_coeffs"))
+        if coeffs.ndim != 3 or coeffs.shape[0] != len(poles) or coeffs.shape[1] != self.n or coeffs.shape[2] != self.n:
+            raise XcaError(f"DiagramEvaluator: hyb_coeffs must have shape (p, n, n) with p = hyb_poles.size() = {len(poles)} "
+                           f"and n = {self.n}, got {coeffs.shape}")
+        cm, dm = ad.packed_ops()
+        d = XcaProblem(self.beta, self.r, dptr(keep(as_f64(grid.it))), dptr(keep(as_f64(grid.rf))), len(poles), dptr(poles), dptr(coeffs),
+                       self.n, ad.nblocks, iptr(keep(as_i32(self.dims))), iptr(keep(as_i32(ad.c_conn))), iptr(keep(as_i32(ad.cdag_conn))),
+                       dptr(keep(_real(cm, "c"))), dptr(keep(_real(dm, "cdag"))))
+        h = lib().xca_ctx_create(C.byref(d), device)
+        if not h:
+            raise XcaError(last_error())
+        self._h = h
+        self.p = len(poles)
+        self.q = lib().xca_num_sym_sets(h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().xca_ctx_destroy(h)
+            self._h = None
+
+    # ---------------------------------------------------------------- helpers
+    def pack(self, G):
+        if len(G) != len(self.dims):
+            raise XcaError(f"expected {len(self.dims)} blocks, got {len(G)}")
+        out = []
+        for b, g in zip(self.dims, G):
+            g = _real(g, "G")
+            if g.shape != (self.r, b, b):
+                raise XcaError(f"G block has shape {g.shape}, expected {(self.r, b, b)}")
+            out.append(g.ravel())
+        return np.ascontiguousarray(np.concatenate(out))
+
+    def unpack(self, flat):
+        out, off = [], 0
+        for b in self.dims:
+            sz = self.r * b * b
+            out.append(flat[off:off + sz].reshape(self.r, b, b).copy())
+            off += sz
+        return out
+
+    # ---------------------------------------------------------------- reference API
+    def get_num_self_energy_backbones(self, topology):
+        return int(lib().xca_num_backbones(self._h, len(topology)))
+
+    def compute_self_energy(self, G, topology, f_ix=None):
+        g = self.pack(G)
+        topo = as_i32(topology)
+        out = np.zeros_like(g)
+        if f_ix is None:
+            st = lib().xca_sigma(self._h, dptr(g), topo.shape[0], iptr(topo), None, 0, dptr(out))
+        else:
+            fx = as_i32(np.atleast_1d(f_ix))
+            st = lib().xca_sigma(self._h, dptr(g), topo.shape[0], iptr(topo), fx.ctypes.data_as(C.POINTER(C.c_int32)), len(fx), dptr(out))
+        check(st)
+        return self.unpack(out)
+
+    def reset(self):
+        check(lib().xca_sigma_zero(self._h))
+
+    # ---------------------------------------------------------------- device-resident form
+    def set_G(self, G):
+        g = self.pack(G)
+        check(lib().xca_set_G(self._h, dptr(g)))
+
+    def accumulate(self, topology, scale):
+        topo = as_i32(topology)
+        check(lib().xca_sigma_accumulate(self._h, topo.shape[0], iptr(topo), float(scale)))
+
+    def get_sigma(self):
+        out = np.zeros(sum(self.r * b * b for b in self.dims))
+        check(lib().xca_get_sigma(self._h, dptr(out)))
+        return self.unpack(out)
+
+    def synchronize(self):
+        check(lib().xca_synchronize(self._h))
+
+    def last_stats(self):
+        items = C.c_int64(0)
+        macs = C.c_double(0)
+        ms = C.c_double(0)
+        check(lib().xca_last_stats(self._h, C.byref(items), C.byref(macs), C.byref(ms)))
+        return items.value, macs.value, ms.value
+
+    def pseudo_particle_self_energy(self, G, max_order):
+        """Sigma summed over orders 1..max_order and connected topologies with the solver's signs
+        (block_sparse_solver.py:597-638: net factor -s_topo per topology)."""
+        self.set_G(G)
+        self.reset()
+        for order in range(1, max_order + 1):
+            for sign, topo in all_connected_pairings(order):
+                self.accumulate(np.array(topo, dtype=np.int32), (-1) ** order * sign * (-1) ** (order + 1))
+        return self.get_sigma()
