@@ -62,7 +62,8 @@ template <class T> struct DevBuf {
   }
 };
 
-static constexpr int kChunkTiles = 8;
+// tiles per chunk: enough chunks to cover the GPU several times, at most 64 tiles per chunk
+static int chunk_tiles(int64_t total_tiles) { return (int)std::max<int64_t>(1, std::min<int64_t>(64, total_tiles / 2048)); }
 
 struct DevPlan {
   DevBuf<DOp> ops;
@@ -279,7 +280,16 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
       wanted[{(int)(rest / p_max), o_ix}].push_back((int)(rest % p_max));
     }
   }
-  std::vector<int> explicit_list;
+  // total tile estimate for the chunk size
+  int64_t est_tiles = 0;
+  for (auto &pg : plan.progs) {
+    std::vector<int> fb(m);
+    for (int i = 0, f = pg.fb_ix; i < m; ++i, f /= 2) fb[i] = f % 2;
+    int64_t nt = 1;
+    for (int x = 0; x < nl; ++x) nt *= (int64_t)c->ps.get(fb[x + 1], (pg.cls >> x) & 1).size();
+    int per = std::max(1, std::min(NT, NT / std::max(1, pg.max_w)));
+    est_tiles += (nt + per - 1) / per;
+  }
   for (auto &pg : plan.progs) {
     for (auto &op : pg.ops) {
       if (op.bin > 12 || op.bout > 12 || (op.type == OP_Z && op.qs > kMaxSet))
@@ -361,10 +371,11 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
     int64_t per = d.items_per_tile;
     if (!fix) {
       int64_t ntiles = (ntup + per - 1) / per;
-      for (int64_t t0 = 0; t0 < ntiles; t0 += kChunkTiles) {
+      const int ct = chunk_tiles(est_tiles);
+      for (int64_t t0 = 0; t0 < ntiles; t0 += ct) {
         I2 ch;
         ch.x = (int)dp->h_tiles.size();
-        for (int64_t t = t0; t < std::min(ntiles, t0 + kChunkTiles); ++t) {
+        for (int64_t t = t0; t < std::min(ntiles, t0 + ct); ++t) {
           DTile tl{pidx, (int)(t * per), (int)std::min<int64_t>(per, ntup - t * per), (int)dp->h_chunks.size()};
           dp->h_tiles.push_back(tl);
         }

@@ -26,21 +26,39 @@ namespace xca::dev {
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 constexpr int kMaxB = 12;   // largest block handled by the tile kernels
+#ifndef XCA_GEMM_UNROLL
+#define XCA_GEMM_UNROLL 2
+#endif
 
 __device__ __forceinline__ v4d mfma_f64(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// acc[rb] += sum_{ks in [k0, k1)} A[rb][ks] * S[4 (ks - k0) + (lane >> 4)][c0 + (lane & 15)]
-// A fragments packed as Afrag[(rb * ldk + ks) * 64 + lane].
+// acc[rb] += sum_{ks < NK} A[rb][koff + ks] * S[4 ks + (lane >> 4)][c0 + (lane & 15)]
+// A fragments packed as Afrag[(rb * ldk + ks) * 64 + lane]; the next step's A and B fragments are
+// loaded before the current step's MFMAs. NK = ceil(r / 4): the zero padding of the DLR rank up to
+// the 16-row blocks is skipped in the contraction.
 template <int RB>
-__device__ __forceinline__ void gemm_acc(const double *__restrict__ Afrag, int ldk, int k0, int k1, const double *S, int ldn, int c0,
+__device__ __forceinline__ void gemm_acc(const double *__restrict__ Afrag, int ldk, int koff, int NK, const double *S, int ldn, int c0,
                                          v4d (&acc)[RB], int lane) {
   const int krow = lane >> 4, col = c0 + (lane & 15);
-  for (int ks = k0; ks < k1; ++ks) {
-    double b = S[(size_t)(4 * (ks - k0) + krow) * ldn + col];
+  const double *Ap = Afrag + (size_t)koff * 64 + lane;
+  double ac[RB], an[RB], bc, bn = 0.0;
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb) acc[rb] = mfma_f64(Afrag[((size_t)rb * ldk + ks) * 64 + lane], b, acc[rb]);
+  for (int rb = 0; rb < RB; ++rb) { ac[rb] = Ap[(size_t)rb * ldk * 64]; an[rb] = 0.0; }
+  bc = S[(size_t)krow * ldn + col];
+#pragma unroll XCA_GEMM_UNROLL
+  for (int ks = 0; ks < NK; ++ks) {
+    if (ks + 1 < NK) {
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) an[rb] = Ap[((size_t)rb * ldk + ks + 1) * 64];
+      bn = S[(size_t)(4 * (ks + 1) + krow) * ldn + col];
+    }
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) acc[rb] = mfma_f64(ac[rb], bc, acc[rb]);
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) ac[rb] = an[rb];
+    bc = bn;
   }
 }
 
@@ -71,8 +89,10 @@ __device__ __forceinline__ void store_acc(const v4d (&acc)[RB], double *S, int l
 template <int RB>
 __device__ __forceinline__ void usolve(v4d (&acc)[RB], const DlrFrag &D, int lane) {
   const int k = lane >> 4, j = lane & 15;
+  const int smax = (D.r - 1) >> 2;  // steps above smax only see padding rows (identity, zero data)
 #pragma unroll
   for (int s = 4 * RB - 1; s >= 0; --s) {
+    if (s > smax) continue;
     const int rbI = s >> 2, v = s & 3;
     const double ys = acc[rbI][v];
     const double y0 = __shfl(ys, j, 64), y1 = __shfl(ys, j + 16, 64), y2 = __shfl(ys, j + 32, 64), y3 = __shfl(ys, j + 48, 64);
@@ -90,7 +110,7 @@ __device__ __forceinline__ void usolve(v4d (&acc)[RB], const DlrFrag &D, int lan
 template <int RB>
 __device__ __forceinline__ void vals2coefs_block(const DlrFrag &D, const double *S, int ldn, int c0, v4d (&acc)[RB], int lane) {
   zero_acc(acc);
-  gemm_acc<RB>(D.LiP, 4 * RB, 0, 4 * RB, S, ldn, c0, acc, lane);
+  gemm_acc<RB>(D.LiP, 4 * RB, 0, (D.r + 3) >> 2, S, ldn, c0, acc, lane);
   usolve<RB>(acc, D, lane);
 }
 
@@ -144,7 +164,7 @@ __global__ void __launch_bounds__(64 * NW) tab_kernel(DlrFrag D, const DTabJob *
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   zero_acc(acc);
-  gemm_acc<RB>(D.W, 4 * RB, 0, 4 * RB, SB, LDN, c0, acc, lane);
+  gemm_acc<RB>(D.W, 4 * RB, 0, (D.r + 3) >> 2, SB, LDN, c0, acc, lane);
   {
     const int col = c0 + (lane & 15);
     if (col < ncol) {
@@ -252,7 +272,7 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainArgs A) {
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         zero_acc(acc);
-        gemm_acc<RB>(D.W, 4 * RB, 0, 4 * RB, Soth, LDN, c0, acc, lane);
+        gemm_acc<RB>(D.W, 4 * RB, 0, (D.r + 3) >> 2, Soth, LDN, c0, acc, lane);
         store_acc<RB>(acc, Scur, LDN, c0, lane);
         __syncthreads();
         // c = -(Wf o g + f o Wg) -> Soth, d = f o g -> Scur
@@ -286,8 +306,8 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainArgs A) {
         __syncthreads();
         // h = beta (cf2it c + tcf2it d) -> Soth (own columns only after own GEMM)
         zero_acc(acc);
-        gemm_acc<RB>(D.M2, 8 * RB, 0, 4 * RB, Soth, LDN, c0, acc, lane);
-        gemm_acc<RB>(D.M2, 8 * RB, 4 * RB, 8 * RB, Scur, LDN, c0, acc, lane);
+        gemm_acc<RB>(D.M2, 8 * RB, 0, (D.r + 3) >> 2, Soth, LDN, c0, acc, lane);
+        gemm_acc<RB>(D.M2, 8 * RB, 4 * RB, (D.r + 3) >> 2, Scur, LDN, c0, acc, lane);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         store_acc<RB>(acc, Soth, LDN, c0, lane);
