@@ -221,12 +221,17 @@ void build_frags(xca_ctx *c) {
   all.insert(all.end(), fAU.begin(), fAU.end());
   size_t oD = all.size();
   all.insert(all.end(), fD.begin(), fD.end());
+  // -U strictly above the diagonal, packed by rows
+  size_t oU = all.size();
+  for (int i = 0; i < RP; ++i)
+    for (int k = i + 1; k < RP; ++k) all.push_back(-Up[(size_t)i * RP + k]);
   c->frag.upload(all, c->stream);
   c->D.LiP = c->frag.ptr + oLiP;
   c->D.W = c->frag.ptr + oW;
   c->D.M2 = c->frag.ptr + oM2;
   c->D.AU = c->frag.ptr + oAU;
   c->D.Dinv = c->frag.ptr + oD;
+  c->D.Upk = c->frag.ptr + oU;
   c->D.r = r;
   c->D.RP = RP;
 }
@@ -511,9 +516,10 @@ extern "C" xca_ctx *xca_ctx_create(const xca_problem *desc, int device) {
     std::vector<double> w(P.p);
     for (int l = 0; l < P.p; ++l) w[l] = P.poles[l] * P.beta;
     c->ps.init(w);
-    c->RB = (P.r + 15) / 16;
-    if (c->RB < 2) c->RB = 2;
-    if (c->RB > 4) throw std::invalid_argument("xca: DLR rank above 64 is not supported by the device path");
+    // padded DLR rank: 32 or 64 rows (16-row MFMA blocks; the row-partitioned GEMMs need the row-block
+    // count to divide the 8 waves of a workgroup)
+    c->RB = (P.r <= 32) ? 2 : 4;
+    if (P.r > 64) throw std::invalid_argument("xca: DLR rank above 64 is not supported by the device path");
     c->RP = 16 * c->RB;
     HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIP_CHECK(hipEventCreate(&c->ev0));

@@ -82,6 +82,84 @@ __device__ __forceinline__ void store_acc(const v4d (&acc)[RB], double *S, int l
     for (int v = 0; v < 4; ++v) S[(size_t)acc_row(rb, v, lane) * ldn + col] = acc[rb][v];
 }
 
+// Row-partitioned GEMM: the wave computes row block rb of the product for NCB column blocks
+// cb0 .. cb0 + NCB - 1: acc[c] += sum_{ks < NK} A[rb][koff + ks] * S[4 ks + (lane >> 4)][16 (cb0 + c) + (lane & 15)].
+// One A fragment (global, prefetched two steps ahead) feeds NCB MFMAs; B fragments come from LDS.
+template <int NCB>
+__device__ __forceinline__ void gemm_rows(const double *__restrict__ Afrag, int ldk, int rb, int koff, int NK, const double *S, int ldn,
+                                          int cb0, v4d (&acc)[NCB], int lane) {
+  const int krow = lane >> 4, col = 16 * cb0 + (lane & 15);
+  const double *Ap = Afrag + ((size_t)rb * ldk + koff) * 64 + lane;
+  double a0 = Ap[0], a1 = (NK > 1) ? Ap[64] : 0.0;
+#pragma unroll 2
+  for (int ks = 0; ks < NK; ++ks) {
+    const double an = (ks + 2 < NK) ? Ap[(size_t)(ks + 2) * 64] : 0.0;
+    double b[NCB];
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) b[c] = S[(size_t)(4 * ks + krow) * ldn + col + 16 * c];
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) acc[c] = mfma_f64(a0, b[c], acc[c]);
+    a0 = a1;
+    a1 = an;
+  }
+}
+
+template <int NCB>
+__device__ __forceinline__ void store_rows(const v4d (&acc)[NCB], double *S, int ldn, int rb, int cb0, int lane) {
+  const int col = 16 * cb0 + (lane & 15);
+#pragma unroll
+  for (int c = 0; c < NCB; ++c)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) S[(size_t)acc_row(rb, v, lane) * ldn + col + 16 * c] = acc[c][v];
+}
+
+template <int RB>
+__device__ __forceinline__ void load_acc(v4d (&acc)[RB], const double *S, int ldn, int c0, int lane) {
+  const int col = c0 + (lane & 15);
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[rb][v] = S[(size_t)acc_row(rb, v, lane) * ldn + col];
+}
+
+// Back substitution with U held in LDS: Ul = -U strictly above the diagonal, packed by rows
+// (row i holds columns i+1 .. RP-1 at offset i (2 RP - i - 1) / 2), Dl = inverses of the 4x4
+// diagonal blocks. Rows 4s..4s+3 of step s live in register s % 4 of row block s / 4, row 4s + k in
+// lane group k, which is the B-operand layout of the rank-4 trailing update.
+template <int RB>
+__device__ __forceinline__ void usolve_lds(v4d (&acc)[RB], const double *Ul, const double *Dl, int r, int lane) {
+  constexpr int RP = 16 * RB;
+  const int k = lane >> 4, j = lane & 15;
+  const int smax = (r - 1) >> 2;
+#pragma unroll
+  for (int s = 4 * RB - 1; s >= 0; --s) {
+    if (s > smax) continue;
+    const int rbI = s >> 2, v = s & 3;
+    const double ys = acc[rbI][v];
+    const double y0 = __shfl(ys, j, 64), y1 = __shfl(ys, j + 16, 64), y2 = __shfl(ys, j + 32, 64), y3 = __shfl(ys, j + 48, 64);
+    const double *Di = Dl + 16 * s + 4 * k;
+    const double x = Di[0] * y0 + Di[1] * y1 + Di[2] * y2 + Di[3] * y3;
+    acc[rbI][v] = x;
+    if (s > 0) {
+      const int col = 4 * s + k;
+#pragma unroll
+      for (int rb = 0; rb <= rbI; ++rb) {
+        const int row = 16 * rb + j;
+        const double a = (row < 4 * s) ? Ul[row * (2 * RP - row - 1) / 2 + (col - row - 1)] : 0.0;
+        acc[rb] = mfma_f64(a, x, acc[rb]);
+      }
+    }
+  }
+}
+
+// Copy the packed U and the diagonal-block inverses into LDS.
+template <int RB>
+__device__ __forceinline__ void load_u(const DlrFrag &D, double *Ul, double *Dl, int tid, int nt) {
+  constexpr int RP = 16 * RB, NU = RP * (RP - 1) / 2, ND = RP * 4;
+  for (int x = tid; x < NU; x += nt) Ul[x] = D.Upk[x];
+  for (int x = tid; x < ND; x += nt) Dl[x] = D.Dinv[x];
+}
+
 // Back substitution U x = y on the accumulators (rows of y in acc, one 16-column block per wave),
 // 4 rows per step from the bottom. Rows 4s..4s+3 of step s live in register s % 4 of row block s / 4,
 // row 4s + k in lane group k: exactly the B-operand layout of the rank-4 trailing update, so the
@@ -187,10 +265,13 @@ __global__ void __launch_bounds__(64 * NW) tab_kernel(DlrFrag D, const DTabJob *
 
 template <int RB, int NW>
 __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainArgs A) {
-  constexpr int RP = 16 * RB, NT = 64 * NW, NTILE = 16 * NW, LDN = NTILE + 8;
+  constexpr int RP = 16 * RB, NT = 64 * NW, NTILE = 16 * NW, LDN = NTILE + 4;
+  constexpr int NCB = NTILE / 16 * RB / NW;  // column blocks per wave in the row-partitioned GEMMs
+  static_assert(NW % RB == 0, "waves must tile the row blocks");
   extern __shared__ double lds[];
-  double *S0 = lds, *S1 = lds + RP * LDN, *xch = lds + 2 * RP * LDN;
-  double *ipref = xch + 64 * NW;                               // per item prefactor (NTILE)
+  double *S0 = lds, *S1 = lds + RP * LDN;
+  double *Ul = lds + 2 * RP * LDN, *Dl = Ul + RP * (RP - 1) / 2;
+  double *ipref = Dl + RP * 4;                                 // per item prefactor (NTILE)
   unsigned long long *imax = (unsigned long long *)(ipref + NTILE);  // per item max |T|
   int *ipole = (int *)(imax + NTILE);                          // (NTILE, kMaxLines)
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -200,6 +281,9 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainArgs A) {
   const int64_t slot = A.chunk_slot[blockIdx.x];
 
   for (int x = tid; x < RP * LDN; x += NT) { S0[x] = 0.0; S1[x] = 0.0; }
+  load_u<RB>(D, Ul, Dl, tid, NT);
+  const int rbw = wv % RB, cbw = (wv / RB) * NCB;  // this wave's row block and first column block
+  const int nk = (r + 3) >> 2;
   bool first_tile = true;
   for (int ti = ch.x; ti < ch.y; ++ti) {
     const DTile tile = A.tiles[ti];
@@ -264,16 +348,26 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainArgs A) {
       } else if (op.type == 1) {  // edge integration
         const int b = op.bin, w = b * cols;
         const int c0 = 16 * wv;
-        v4d acc[RB];
-        // g = vals2coefs(T) -> Soth ; W g -> Scur (T is dead after the GEMM of this wave's block)
-        vals2coefs_block<RB>(D, Scur, LDN, c0, acc, lane);
-        store_acc<RB>(acc, Soth, LDN, c0, lane);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        zero_acc(acc);
-        gemm_acc<RB>(D.W, 4 * RB, 0, (D.r + 3) >> 2, Soth, LDN, c0, acc, lane);
-        store_acc<RB>(acc, Scur, LDN, c0, lane);
+        {  // y = (L^{-1} P) T -> Soth
+          v4d acc[NCB];
+          zero_acc(acc);
+          gemm_rows<NCB>(D.LiP, 4 * RB, rbw, 0, nk, Scur, LDN, cbw, acc, lane);
+          store_rows<NCB>(acc, Soth, LDN, rbw, cbw, lane);
+        }
+        __syncthreads();
+        {  // g = U^{-1} y in place (one column block per wave)
+          v4d acc[RB];
+          load_acc<RB>(acc, Soth, LDN, c0, lane);
+          usolve_lds<RB>(acc, Ul, Dl, r, lane);
+          store_acc<RB>(acc, Soth, LDN, c0, lane);
+        }
+        __syncthreads();
+        {  // W g -> Scur (T is dead)
+          v4d acc[NCB];
+          zero_acc(acc);
+          gemm_rows<NCB>(D.W, 4 * RB, rbw, 0, nk, Soth, LDN, cbw, acc, lane);
+          store_rows<NCB>(acc, Scur, LDN, rbw, cbw, lane);
+        }
         __syncthreads();
         // c = -(Wf o g + f o Wg) -> Soth, d = f o g -> Scur
         for (int x = tid; x < r * P_items * cols; x += NT) {
@@ -304,13 +398,15 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainArgs A) {
           }
         }
         __syncthreads();
-        // h = beta (cf2it c + tcf2it d) -> Soth (own columns only after own GEMM)
-        zero_acc(acc);
-        gemm_acc<RB>(D.M2, 8 * RB, 0, (D.r + 3) >> 2, Soth, LDN, c0, acc, lane);
-        gemm_acc<RB>(D.M2, 8 * RB, 4 * RB, (D.r + 3) >> 2, Scur, LDN, c0, acc, lane);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        store_acc<RB>(acc, Soth, LDN, c0, lane);
+        // h = beta (cf2it c + tcf2it d) -> Soth, stored after every wave has read both slabs
+        {
+          v4d acc[NCB];
+          zero_acc(acc);
+          gemm_rows<NCB>(D.M2, 8 * RB, rbw, 0, nk, Soth, LDN, cbw, acc, lane);
+          gemm_rows<NCB>(D.M2, 8 * RB, rbw, 4 * RB, nk, Scur, LDN, cbw, acc, lane);
+          __syncthreads();
+          store_rows<NCB>(acc, Soth, LDN, rbw, cbw, lane);
+        }
         // keep the padding rows of the slab that held d zero for the next GEMM that reads it
         double *tmp = Scur; Scur = Soth; Soth = tmp;
         __syncthreads();
@@ -400,8 +496,8 @@ __global__ void reduce_kernel(const int64_t *__restrict__ slot, const int *__res
 
 template <int RB, int NW>
 static size_t lds_bytes() {
-  constexpr int RP = 16 * RB, NTILE = 16 * NW, LDN = NTILE + 8;
-  return sizeof(double) * ((size_t)2 * RP * LDN + 64 * NW + NTILE + NTILE) + sizeof(int) * NTILE * kMaxLines;
+  constexpr int RP = 16 * RB, NTILE = 16 * NW, LDN = NTILE + 4;
+  return sizeof(double) * ((size_t)2 * RP * LDN + RP * (RP - 1) / 2 + RP * 4 + NTILE + NTILE) + sizeof(int) * NTILE * kMaxLines;
 }
 
 template <int RB, int NW>
@@ -435,7 +531,6 @@ hipError_t launch_tab(int RB, const DlrFrag &D, const DTabJob *jobs, int njobs, 
                       double *ftab, hipStream_t s) {
   switch (RB) {
     case 2: return launch_tab_t<2, 8>(D, jobs, njobs, G, kp, km, p, ftab, s);
-    case 3: return launch_tab_t<3, 8>(D, jobs, njobs, G, kp, km, p, ftab, s);
     case 4: return launch_tab_t<4, 8>(D, jobs, njobs, G, kp, km, p, ftab, s);
     default: return hipErrorInvalidValue;
   }
@@ -445,7 +540,6 @@ hipError_t launch_chain(int RB, const void *args, int nchunks, hipStream_t s) {
   const ChainArgs &A = *static_cast<const ChainArgs *>(args);
   switch (RB) {
     case 2: return launch_chain_t<2, 8>(A, nchunks, s);
-    case 3: return launch_chain_t<3, 8>(A, nchunks, s);
     case 4: return launch_chain_t<4, 8>(A, nchunks, s);
     default: return hipErrorInvalidValue;
   }

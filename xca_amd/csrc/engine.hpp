@@ -53,6 +53,7 @@ struct DlrFrag {
   const double *M2;    // (RB, 2*KS, 64): beta [cf2it | tcf2it]
   const double *AU;    // upper-triangular trailing-update fragments, per (step, row block)
   const double *Dinv;  // (RP/4, 16): inverses of the 4x4 diagonal blocks of U
+  const double *Upk;   // -U strictly above the diagonal, packed by rows (RP (RP - 1) / 2)
   int r, RP;
 };
 
