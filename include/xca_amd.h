@@ -88,6 +88,24 @@ int xca_sigma_accumulate(xca_ctx *ctx, int m, const int *topo, double scale);
 int xca_get_sigma(xca_ctx *ctx, double *sigma_out);
 int xca_synchronize(xca_ctx *ctx);
 
+/* Pseudo-particle Dyson solver: one dyson_it_ppsc(beta, itops, G0_block) per block
+ * (c++/triqs_xca/dlr_dyson_ppsc.hpp:92-117, constructed at python/triqs_xca/block_sparse_solver.py:140).
+ * G0: concatenated (r, b, b) blocks of the atomic pseudo-particle propagator. */
+int xca_dyson_init(xca_ctx *ctx, const double *G0);
+
+/* dyson_it_ppsc::solve(sigma, eta) for every block followed by hermitization
+ * (dlr_dyson_ppsc.hpp:135-168, block_sparse_solver.py:572-590). sigma == NULL takes the device Sigma
+ * accumulator. G_out: concatenated blocks (host). */
+int xca_dyson_solve(xca_ctx *ctx, const double *sigma, double eta, double *G_out);
+
+/* BlockSparseSolver.solve(max_order, tol, maxiter, mix) with classic normalization
+ * (python/triqs_xca/block_sparse_solver.py:240-355, 427-449): Sigma over orders 1..max_order -> Dyson
+ * -> Z = -Tr G(beta), eta += ln|Z|/beta, G_new *= exp(-tau deta) -> stop when max|G_new - G| < tol,
+ * else G = mix G_new + (1 - mix) G. Starts from the G0 given to xca_dyson_init and pseudo-particle
+ * chemical potential eta0. Outputs the solver's G, eta, the iteration count and the last difference. */
+int xca_ppsc_solve(xca_ctx *ctx, double eta0, int max_order, double tol, int maxiter, double mix, double *G_out, double *eta_out,
+                   int *iters_out, double *diff_out);
+
 /* Statistics of the last accumulate call: number of (path, pole tuple) items evaluated, canonical
  * multiply-add count, and the device time of the chain kernel in milliseconds. */
 int xca_last_stats(xca_ctx *ctx, int64_t *items, double *macs, double *kernel_ms);

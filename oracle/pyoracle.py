@@ -187,3 +187,41 @@ class OracleEvaluator:
             st = lib().orc_sigma(self.h, _dp(g), topo.shape[0], _ip(topo), _ip(fx), len(fx), _dp(out))
         _chk(st)
         return prob.unpack(out)
+
+
+# ------------------------------------------------------------------ self-consistency loop
+
+def ppsc_solve(prob, max_order, tol=1e-10, maxiter=100, mix=1.0, complex_arith=False):
+    """BlockSparseSolver.solve with classic normalization (block_sparse_solver.py:240-355, 427-449,
+    572-590) on top of the oracle's Sigma and Dyson: returns (G blocks, eta, iterations, diff)."""
+    from xca_amd.diag import all_connected_pairings
+    orc = OracleEvaluator(prob, complex_arith=complex_arith)
+    grid = prob.grid
+    G = [np.array(g, dtype=float) for g in prob.G0]
+    eta = 0.0
+    topos = [(np.array(t, dtype=np.int32), (-1) ** o * s * (-1) ** (o + 1)) for o in range(1, max_order + 1)
+             for s, t in all_connected_pairings(o)]
+    diff = 0.0
+    it = 0
+    for it in range(1, maxiter + 1):
+        S = None
+        for t, fac in topos:
+            s = [x.real for x in orc.sigma(G, t)]
+            S = [fac * x for x in s] if S is None else [a + fac * x for a, x in zip(S, s)]
+        Gn = []
+        for g0, sb in zip(prob.G0, S):
+            g = dyson(prob.beta, grid.it, grid.rf, g0, sb, eta).real
+            Gn.append(0.5 * (g + g.transpose(0, 2, 1)))
+        tr = 0.0
+        for g in Gn:
+            ev = interp(grid.it, grid.rf, g, np.array([1.0]))[0].real
+            tr += np.trace(ev)
+        Z = -tr
+        deta = np.log(abs(Z)) / prob.beta
+        Gn = [g * np.exp(-grid.tau * deta)[:, None, None] for g in Gn]
+        eta += deta
+        diff = max(np.abs(a - b).max() for a, b in zip(Gn, G))
+        if diff < tol:
+            break
+        G = [mix * a + (1 - mix) * b for a, b in zip(Gn, G)]
+    return G, eta, it, diff

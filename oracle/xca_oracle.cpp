@@ -110,6 +110,18 @@ struct ImTimeOps {
     }
   }
 
+  // in place: solve cf2it^T y = b for one vector (LAPACK getrs with transposition)
+  template <class S> void solve_transposed(S *y) const {
+    for (int i = 0; i < r; ++i) {  // U^T z = b
+      for (int k = 0; k < i; ++k) y[i] -= lu[(size_t)k * r + i] * y[k];
+      y[i] /= lu[(size_t)i * r + i];
+    }
+    for (int i = r - 1; i >= 0; --i)  // L^T w = z (unit diagonal)
+      for (int k = i + 1; k < r; ++k) y[i] -= lu[(size_t)k * r + i] * y[k];
+    for (int k = r - 1; k >= 0; --k)  // undo the row interchanges in reverse order
+      if (piv[k] != k) std::swap(y[k], y[piv[k]]);
+  }
+
   // out (r, m) = M (r x r) * in (r, m)
   template <class S> void apply(const std::vector<double> &M, const S *in, S *out, int m) const {
     for (int i = 0; i < r; ++i) {
@@ -753,23 +765,31 @@ template <class S> struct DiagramEvaluator {
 // ---------------------------------------------------------------- Dyson (dlr_dyson_ppsc.hpp:92-168)
 
 // Matrix of X -> convolve_to(beta, fc, vals2coefs(X)) acting on one column of X(t) (b x b blocks),
-// indexed (t, i) x (t', i'), built from the same formula as convolve_to.
+// indexed (t, i) x (t', i'). The coefficient-to-value convolution matrix Mc (columns: unit coefficient
+// vectors through convolve_to) is composed with vals2coefs from the right by a transposed LU solve,
+// C^T = cf2it^{-T} Mc^T, as every other coefficient transform here goes through the LU factors.
 template <class S>
 std::vector<S> convmat(const ImTimeOps &ops, double beta, const std::vector<S> &fc, int b) {
   int r = ops.r, N = r * b;
-  std::vector<S> M((size_t)N * N, S(0));
-  // columns: unit vectors e_(t', i')
+  std::vector<S> Mc((size_t)N * N, S(0));
   std::vector<S> g((size_t)r * b), res;
-  for (int tp = 0; tp < r; ++tp)
+  for (int kp = 0; kp < r; ++kp)
     for (int ip = 0; ip < b; ++ip) {
       std::fill(g.begin(), g.end(), S(0));
-      g[(size_t)tp * b + ip] = S(1);
-      ops.vals2coefs(g.data(), b);
+      g[(size_t)kp * b + ip] = S(1);
       res = ops.convolve_to(beta, fc, g, b, b, 1);
       for (int t = 0; t < r; ++t)
-        for (int i = 0; i < b; ++i) M[((size_t)t * b + i) * N + (size_t)tp * b + ip] = res[(size_t)t * b + i];
+        for (int i = 0; i < b; ++i) Mc[((size_t)t * b + i) * N + (size_t)kp * b + ip] = res[(size_t)t * b + i];
     }
-  return M;
+  // each row (t, i) and column orbital i': y(t') solves cf2it^T y = Mc[(t,i), (:, i')]
+  std::vector<S> y(r);
+  for (size_t row = 0; row < (size_t)N; ++row)
+    for (int ip = 0; ip < b; ++ip) {
+      for (int k = 0; k < r; ++k) y[k] = Mc[row * N + (size_t)k * b + ip];
+      ops.solve_transposed(y.data());
+      for (int k = 0; k < r; ++k) Mc[row * N + (size_t)k * b + ip] = y[k];
+    }
+  return Mc;
 }
 
 template <class S> void lu_solve_inplace(std::vector<S> &A, int N, std::vector<S> &B, int nrhs) {

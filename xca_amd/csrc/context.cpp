@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/xca_amd.h"
+#include "ctx.hpp"
 #include "engine.hpp"
 #include "host/errors.hpp"
 #include "host/linalg.hpp"
@@ -22,6 +23,7 @@ int tile_columns();
 hipError_t launch_tab(int RB, const DlrFrag &D, const DTabJob *jobs, int njobs, const double *G, const double *kp, const double *km, int p,
                       double *ftab, hipStream_t s);
 hipError_t launch_chain(int RB, const void *args, int nchunks, hipStream_t s);
+hipError_t launch_fill(int64_t n, double v, double *p, hipStream_t s);
 hipError_t launch_reduce(const int64_t *slot, const int *red_chunks, const I2 *red_range, const int64_t *sig_off, const int *sig_size,
                          int nblocks, int max_size, const double *partial, double *sigma, double scale, hipStream_t s);
 size_t chain_args_size();
@@ -31,36 +33,6 @@ size_t chain_args_size();
 namespace xca {
 
 using namespace xca::dev;
-
-#define HIP_CHECK(x)                                                                                          \
-  do {                                                                                                        \
-    hipError_t e_ = (x);                                                                                      \
-    if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x); \
-  } while (0)
-
-template <class T> struct DevBuf {
-  T *ptr = nullptr;
-  size_t n = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf &) = delete;
-  DevBuf &operator=(const DevBuf &) = delete;
-  ~DevBuf() { release(); }
-  void release() {
-    if (ptr) (void)hipFree(ptr);
-    ptr = nullptr;
-    n = 0;
-  }
-  void alloc(size_t count) {
-    if (count <= n && ptr) return;
-    release();
-    n = std::max<size_t>(count, 1);
-    HIP_CHECK(hipMalloc(&ptr, n * sizeof(T)));
-  }
-  void upload(const std::vector<T> &v, hipStream_t s) {
-    alloc(v.size());
-    if (!v.empty()) HIP_CHECK(hipMemcpyAsync(ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
-  }
-};
 
 // tiles per chunk: enough chunks to cover the GPU several times, at most 64 tiles per chunk
 static int chunk_tiles(int64_t total_tiles) { return (int)std::max<int64_t>(1, std::min<int64_t>(64, total_tiles / 2048)); }
@@ -92,26 +64,6 @@ struct DevPlan {
 
 using namespace xca;
 
-struct xca_ctx {
-  Model M;
-  PoleSets ps;
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int RB = 0, RP = 0, maxexp = 8;
-  std::vector<int> plist_h;
-  int plist_off[4];  // fwd_a, fwd_b, bwd_a, bwd_b
-  DevBuf<double> opmats, kp, km, delta, delta_refl, kpow, G, sigma, frag;
-  DevBuf<int> plist;
-  DevBuf<int64_t> sig_off;
-  DevBuf<int> sig_size;
-  DlrFrag D{};
-  std::vector<int> G_zero;
-  bool have_G = false;
-  std::map<std::vector<int>, std::unique_ptr<DevPlan>> plans;
-  int64_t last_items = 0;
-  double last_macs = 0, last_ms = 0;
-};
 
 namespace {
 
@@ -492,6 +444,71 @@ void accumulate(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t 
 
 }  // namespace
 
+namespace xca {
+
+// Connected pairings of 2*order vertices with their parity sign (python/triqs_xca/diag.py:48-127):
+// vertex 0 is paired first with each later vertex, the sign flips for an even distance, and a
+// pairing is kept when every line is reachable through crossings from the line attached to vertex 0.
+static void pairings(int parity, std::vector<std::pair<int, int>> &cur, std::vector<int> rest,
+                     std::vector<std::pair<int, std::vector<std::pair<int, int>>>> &out) {
+  if (rest.empty()) {
+    out.push_back({parity, cur});
+    return;
+  }
+  for (size_t i = 1; i < rest.size(); ++i) {
+    std::vector<int> r2;
+    for (size_t k = 1; k < rest.size(); ++k)
+      if (k != i) r2.push_back(rest[k]);
+    cur.push_back({rest[0], rest[i]});
+    pairings(i % 2 == 0 ? -parity : parity, cur, r2, out);
+    cur.pop_back();
+  }
+}
+
+static bool connected(const std::vector<std::pair<int, int>> &pr) {
+  auto cross = [](std::pair<int, int> a, std::pair<int, int> b) {
+    return (a.first < b.first && b.first < a.second && a.second < b.second) || (b.first < a.first && a.first < b.second && b.second < a.second);
+  };
+  std::vector<std::pair<int, int>> stack, rest;
+  for (auto &p : pr) ((p.first <= 0) != (p.second <= 0) ? stack : rest).push_back(p);
+  while (!stack.empty()) {
+    auto pc = stack.front();
+    stack.erase(stack.begin());
+    for (int i = (int)rest.size() - 1; i >= 0; --i)
+      if (cross(pc, rest[i])) {
+        stack.push_back(rest[i]);
+        rest.erase(rest.begin() + i);
+      }
+  }
+  return rest.empty();
+}
+
+std::vector<std::pair<int, std::vector<int>>> connected_topologies(int order) {
+  std::vector<std::pair<int, std::vector<std::pair<int, int>>>> all;
+  std::vector<std::pair<int, int>> cur;
+  std::vector<int> v(2 * order);
+  for (int i = 0; i < 2 * order; ++i) v[i] = i;
+  pairings(1, cur, v, all);
+  std::vector<std::pair<int, std::vector<int>>> out;
+  for (auto &pp : all)
+    if (connected(pp.second)) {
+      std::vector<int> t;
+      for (auto &p : pp.second) { t.push_back(p.first); t.push_back(p.second); }
+      out.push_back({pp.first, t});
+    }
+  return out;
+}
+
+// Sigma accumulator <- sum over orders 1..max_order and connected topologies with the solver's net
+// factor (-1)^order * s * (-1)^(order+1) = -s (block_sparse_solver.py:609-638)
+void sigma_full_order(xca_ctx *c, int max_order) {
+  HIP_CHECK(launch_fill(c->M.G_size, 0.0, c->sigma.ptr, c->stream));
+  for (int order = 1; order <= max_order; ++order)
+    for (auto &st : connected_topologies(order)) accumulate(c, order, st.second.data(), nullptr, 0, -(double)st.first);
+}
+
+}  // namespace xca
+
 // ================================================================================== C ABI
 
 #define XCA_TRY(body)                  \
@@ -552,7 +569,7 @@ extern "C" xca_ctx *xca_ctx_create(const xca_problem *desc, int device) {
     c->sig_size.upload(ss, s);
     c->G.alloc(c->M.G_size);
     c->sigma.alloc(c->M.G_size);
-    HIP_CHECK(hipMemsetAsync(c->sigma.ptr, 0, c->M.G_size * sizeof(double), s));
+    HIP_CHECK(launch_fill(c->M.G_size, 0.0, c->sigma.ptr, s));
     build_frags(c);
     HIP_CHECK(hipStreamSynchronize(s));
     return c;
@@ -568,6 +585,7 @@ extern "C" void xca_ctx_destroy(xca_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->plans.clear();
+  xca_dyson_release(c);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -597,7 +615,7 @@ extern "C" int xca_set_G(xca_ctx *c, const double *G) {
 
 extern "C" int xca_sigma_zero(xca_ctx *c) {
   XCA_TRY({
-    HIP_CHECK(hipMemsetAsync(c->sigma.ptr, 0, c->M.G_size * sizeof(double), c->stream));
+    HIP_CHECK(launch_fill(c->M.G_size, 0.0, c->sigma.ptr, c->stream));
     return 0;
   })
 }
@@ -630,7 +648,7 @@ extern "C" int xca_synchronize(xca_ctx *c) {
 extern "C" int xca_sigma(xca_ctx *c, const double *G, int m, const int *topo, const int32_t *fix, int64_t nfix, double *out) {
   XCA_TRY({
     if (xca_set_G(c, G) < 0) return -1;
-    HIP_CHECK(hipMemsetAsync(c->sigma.ptr, 0, c->M.G_size * sizeof(double), c->stream));
+    HIP_CHECK(launch_fill(c->M.G_size, 0.0, c->sigma.ptr, c->stream));
     accumulate(c, m, topo, fix, fix ? nfix : 0, 1.0);
     return xca_get_sigma(c, out);
   })

@@ -138,3 +138,25 @@ class DiagramEvaluator:
             for sign, topo in all_connected_pairings(order):
                 self.accumulate(np.array(topo, dtype=np.int32), (-1) ** order * sign * (-1) ** (order + 1))
         return self.get_sigma()
+
+    # ---------------------------------------------------------------- Dyson and self-consistency
+    def dyson_init(self, G0):
+        """One dyson_it_ppsc per block built from the atomic propagator G0 (list of blocks)."""
+        self._G0 = self.pack(G0)
+        check(lib().xca_dyson_init(self._h, dptr(self._G0)))
+
+    def dyson_solve(self, Sigma, eta):
+        s = self.pack(Sigma)
+        out = np.zeros_like(s)
+        check(lib().xca_dyson_solve(self._h, dptr(s), float(eta), dptr(out)))
+        return self.unpack(out)
+
+    def ppsc_solve(self, max_order, tol=1e-10, maxiter=100, mix=1.0, eta0=0.0):
+        """Self-consistent pseudo-particle G; returns (G blocks, eta, iterations, last max|G_new - G|)."""
+        out = np.zeros(sum(self.r * b * b for b in self.dims))
+        eta = C.c_double(0)
+        it = C.c_int(0)
+        diff = C.c_double(0)
+        check(lib().xca_ppsc_solve(self._h, float(eta0), int(max_order), float(tol), int(maxiter), float(mix), dptr(out), C.byref(eta),
+                                   C.byref(it), C.byref(diff)))
+        return self.unpack(out), eta.value, it.value, diff.value
