@@ -32,6 +32,18 @@ def build_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of the dominant chain launch, from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a plain copy of known size); None if not measured."""
+    import re
+    key = f"traffic_bytes_per_launch[{workload}]:"
+    for f in sorted((ROOT / "profiles").glob("*.md")):
+        for line in f.read_text().splitlines():
+            if line.startswith(key):
+                return float(re.findall(r"[0-9.eE+]+", line[len(key):])[0])
+    return None
+
+
 def cpu_baseline(P, G, max_order, budget_s):
     """Time the oracle (CPU restatement of the reference path, complex128 like the reference) on the
     same workload, single-threaded, one Sigma evaluation or a bounded sample of its flat indices."""
@@ -51,6 +63,19 @@ def cpu_baseline(P, G, max_order, budget_s):
             "sample": f"one full Sigma evaluation (orders 1-{max_order}, all f_ix) in complex128 on one host core, {el:.2f} s"}
 
 
+def copy_calibration(mib):
+    """Plain copy kernel of known size: FETCH_SIZE / WRITE_SIZE per dispatch of this copy divided by
+    mib * 1024 KiB gives the scale factor applied to the same counters of the chain kernel."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "dispatches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,7 +84,12 @@ def main():
     ap.add_argument("--workload", default="oca2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (reference dispatches for the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (the Sigma evaluation "
               "would shard by backbone index with one all-reduce of the Sigma blocks)", file=sys.stderr)
@@ -129,7 +159,7 @@ def main():
                    "items_per_eval": int(sum(per_topo[k][0][0] for k in per_topo)),
                    "canonical_macs_per_eval": total_macs},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
-                     "traffic": None, "kernel": "chain_kernel", "kernel_ms": kms,
+                     "traffic": measured_traffic(args.workload), "kernel": "chain_kernel", "kernel_ms": kms,
                      "flops_per_launch": 2.0 * macs, "topology": topos[dom][0].tolist()},
     }
     if not args.no_cpu:

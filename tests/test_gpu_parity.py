@@ -27,6 +27,7 @@ CASES = {
     "one_fermion_pole": lambda: models.one_fermion(beta=1.0, lam=20.0, eps=1e-10, pole=0.8),
     "two_fermion": lambda: models.two_fermion(beta=2.0, lam=40.0, eps=1e-12, U=3.0, mu=0.5, pole=0.7),
     "n3_h0": lambda: models.n_fermion_h0(3, beta=2.0, lam=20.0, eps=1e-12),
+    "two_band_discrete_bath": models.two_band_discrete_bath,
     "anderson_nca": models.anderson_nca,
     "oca_2orb": models.oca_2orb,
 }
@@ -49,6 +50,28 @@ def test_sigma_per_topology(case, order):
         e = rel_err(s_gpu, s_cpu)
         print(f"  topo={topo} rel_err={e:.2e}", flush=True)
         assert e <= TOL
+
+
+def test_oca_two_band_discrete_bath_matches_tpz100():
+    """Device OCA against the reference's trapezoidal-quadrature fixture (tpz100.ref.h5), compared
+    through the per-tau eigenvalues of each block (basis independent), tolerance 4e-4 as
+    test_block_sparse_OCA_manual.cpp:237-282."""
+    import json
+    from pathlib import Path
+    from test_oracle_reference_fixtures import FIXTURE_N, block_particle_numbers, equispaced_rel
+    fx = json.loads((Path(__file__).resolve().parent / "golden" / "oca_tpz100_blocks.json").read_text())
+    P = models.two_band_discrete_bath()
+    ev = _evaluator(P)
+    S = ev.compute_self_energy(P.G0, np.array([[0, 2], [1, 3]], dtype=np.int32))
+    rel = equispaced_rel(fx["ntau"])
+    Nb = block_particle_numbers(P.ad)
+    worst = 0.0
+    for k, fb in enumerate(fx["blocks_real"]):
+        s = S[Nb.index(FIXTURE_N[k])]
+        seq = po.interp(P.grid.it, P.grid.rf, s, rel).real
+        worst = max(worst, np.abs(np.linalg.eigvalsh(seq) - np.linalg.eigvalsh(np.array(fb))).max())
+    print(f"[gpu] tpz100 fixture max eigenvalue error {worst:.2e}", flush=True)
+    assert worst <= 4e-4
 
 
 def test_flat_index_subset_matches_oracle():

@@ -93,6 +93,21 @@ def n_fermion_h0(nflav, beta=2.0, lam=20.0, eps=1e-12):
     return ImpurityProblem(grid=grid, ad=ad, poles=np.array([0.0]), coeffs=np.eye(nflav)[None], G0=G0, eta0=eta0, name=f"n{nflav}_h0")
 
 
+def two_band_discrete_bath(beta=2.0, lam=40.0, eps=1e-6, U=2.0, J=0.2, s=0.5, levels=(-2.3, 2.3)):
+    """Two-band Kanamori atom (U = 2, J = 0.2, mu = (3U - 5J)/2 - 1.5), blocks by N, coupled to a
+    discrete bath with levels +-2.3: Delta_ij(tau) = sum_e K(tau, e) (1 on the diagonal, s between the
+    two orbitals of one spin); G = the normalised atomic propagator
+    (test/c++/block_sparse_utils.cpp:99-141,224-261)."""
+    grid = DLRGrid(beta, lam, eps)
+    fs = FockSpace(4)
+    H = kanamori_hamiltonian(fs, 2, U, J, (3 * U - 5 * J) / 2 - 1.5)
+    ad = AtomDiag(fs, H, qn="N")
+    jt = sum(np.array([kernel(t, beta * e) for t in grid.it]) for e in levels)
+    M = np.eye(4)
+    M[0, 1] = M[1, 0] = M[2, 3] = M[3, 2] = s
+    return _problem(grid, ad, jt[:, None, None] * M[None], "two_band_discrete_bath")
+
+
 def anderson_nca(beta=10.0, lam=100.0, eps=1e-10, U=2.0, mu=1.0, t2=0.25):
     """Single-orbital Anderson impurity, H = U n_up n_do - mu (n_up + n_do), blocks by N, semicircular
     hybridization (half-bandwidth 1, t^2 = 0.25) expanded in DLR."""
