@@ -106,6 +106,14 @@ int xca_dyson_solve(xca_ctx *ctx, const double *sigma, double eta, double *G_out
 int xca_ppsc_solve(xca_ctx *ctx, double eta0, int max_order, double tol, int maxiter, double mix, double *G_out, double *eta_out,
                    int *iters_out, double *diff_out);
 
+/* Tile width of the chain kernel: variant 0, 1, 2 = 128, 64, 32 columns per tile; -1 (the default)
+ * lets the autotuner time every applicable variant on each new plan and keep the fastest. The
+ * environment variable XCA_TILE_VARIANT sets the initial value. Changing it drops cached plans.
+ * No reference counterpart (the reference has no device tiles). */
+int xca_set_tile_variant(xca_ctx *ctx, int variant);
+/* Variant and columns per tile used by the last accumulate call. */
+int xca_last_tile(xca_ctx *ctx, int *variant, int *columns);
+
 /* Statistics of the last accumulate call: number of (path, pole tuple) items evaluated, canonical
  * multiply-add count, and the device time of the chain kernel in milliseconds. */
 int xca_last_stats(xca_ctx *ctx, int64_t *items, double *macs, double *kernel_ms);

@@ -48,8 +48,11 @@ def test_sigma_per_topology(case, order):
         s_gpu = ev.compute_self_energy(G, t)
         s_cpu = [x.real for x in orc.sigma(G, t)]
         e = rel_err(s_gpu, s_cpu)
-        print(f"  topo={topo} rel_err={e:.2e}", flush=True)
-        assert e <= TOL
+        # 1e-12, or 4x the oracle's own rounding floor where that is higher (the two-band model on its
+        # eps = 1e-6 grid reaches 1.9e-12 at third order)
+        floor = oracle_noise_floor(orc, G, t, None)
+        print(f"  topo={topo} rel_err={e:.2e} oracle_floor={floor:.2e}", flush=True)
+        assert e <= max(TOL, 4 * floor)
 
 
 def test_oca_two_band_discrete_bath_matches_tpz100():
@@ -72,6 +75,33 @@ def test_oca_two_band_discrete_bath_matches_tpz100():
         worst = max(worst, np.abs(np.linalg.eigvalsh(seq) - np.linalg.eigvalsh(np.array(fb))).max())
     print(f"[gpu] tpz100 fixture max eigenvalue error {worst:.2e}", flush=True)
     assert worst <= 4e-4
+
+
+@pytest.mark.parametrize("case,order", [("oca_2orb", 2), ("two_band_discrete_bath", 3)])
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_every_tile_variant(variant, case, order):
+    """Each chain-kernel tile width (128/64/32 columns) gives the oracle's Sigma, and so does the
+    autotuner's choice (-1). The two-band third order once produced NaN through stale tile columns
+    (0 * inf in the padding rows); it runs twice here so every tile reuses a dirty LDS slab."""
+    P = CASES[case]()
+    G = P.perturbed_G() if case == "oca_2orb" else P.G0
+    ev = _evaluator(P)
+    orc = po.OracleEvaluator(P, complex_arith=False)
+    topos = [t for o in range(1, order + 1) for _, t in all_connected_pairings(o)]
+    ref = {str(t): [x.real for x in orc.sigma(G, np.array(t, dtype=np.int32))] for t in topos}
+    tol = {str(t): max(TOL, 4 * oracle_noise_floor(orc, G, np.array(t, dtype=np.int32), None)) for t in topos}
+    for v in (variant, -1):
+        ev.set_tile_variant(v)
+        for rep in range(2):
+            for topo in topos:
+                t = np.array(topo, dtype=np.int32)
+                e = rel_err(ev.compute_self_energy(G, t), ref[str(topo)])
+                used = ev.last_tile()
+                print(f"[gpu] {case} variant {v} used {used} topo={topo} rel_err={e:.2e} tol={tol[str(topo)]:.1e}", flush=True)
+                assert np.isfinite(e) and e <= tol[str(topo)]
+                # a forced width that cannot hold the widest block path falls back to 128 columns
+                if v >= 0:
+                    assert used[0] in (v, 0)
 
 
 def test_flat_index_subset_matches_oracle():
@@ -115,7 +145,8 @@ def oracle_noise_floor(orc, G, t, fix, seed=5):
     rng = np.random.default_rng(seed)
     a = orc.sigma(G, t, fix)
     b = orc.sigma([g * (1 + 1e-16 * rng.standard_normal(g.shape)) for g in G], t, fix)
-    return max(np.abs(x - y).max() for x, y in zip(a, b)) / max(np.abs(x).max() for x in a)
+    den = max(np.abs(x).max() for x in a)
+    return max(np.abs(x - y).max() for x, y in zip(a, b)) / den if den > 0 else 0.0
 
 
 @pytest.mark.parametrize("topo_idx", [0, 1, 2, 3])

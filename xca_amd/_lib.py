@@ -37,6 +37,8 @@ EXPORTS = {
     "xca_get_sigma": (C.c_int, [C.c_void_p, d_p]),
     "xca_synchronize": (C.c_int, [C.c_void_p]),
     "xca_last_stats": (C.c_int, [C.c_void_p, i64_p, d_p, d_p]),
+    "xca_set_tile_variant": (C.c_int, [C.c_void_p, C.c_int]),
+    "xca_last_tile": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "xca_dyson_init": (C.c_int, [C.c_void_p, d_p]),
     "xca_dyson_solve": (C.c_int, [C.c_void_p, d_p, C.c_double, d_p]),
     "xca_ppsc_solve": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_int, C.c_double, d_p, d_p, C.POINTER(C.c_int), d_p]),

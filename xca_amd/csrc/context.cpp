@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -19,10 +20,11 @@
 #include "host/model.hpp"
 
 namespace xca::dev {
-int tile_columns();
+int num_tile_variants();
+int tile_columns(int variant);
 hipError_t launch_tab(int RB, const DlrFrag &D, const DTabJob *jobs, int njobs, const double *G, const double *kp, const double *km, int p,
                       double *ftab, hipStream_t s);
-hipError_t launch_chain(int RB, const void *args, int nchunks, hipStream_t s);
+hipError_t launch_chain(int RB, int variant, const void *args, hipStream_t s);
 hipError_t launch_fill(int64_t n, double v, double *p, hipStream_t s);
 hipError_t launch_reduce(const int64_t *slot, const int *red_chunks, const I2 *red_range, const int64_t *sig_off, const int *sig_size,
                          int nblocks, int max_size, const double *partial, double *sigma, double scale, hipStream_t s);
@@ -37,27 +39,39 @@ using namespace xca::dev;
 // tiles per chunk: enough chunks to cover the GPU several times, at most 64 tiles per chunk
 static int chunk_tiles(int64_t total_tiles) { return (int)std::max<int64_t>(1, std::min<int64_t>(64, total_tiles / 2048)); }
 
-struct DevPlan {
-  DevBuf<DOp> ops;
+// Programs, chunks and partial slots of one tile width.
+struct DevVariant {
+  int NT = 0;
   DevBuf<DProg> progs;
-  DevBuf<DTile> tiles;
-  DevBuf<I2> chunks;
+  DevBuf<DChunk> chunks;
   DevBuf<int64_t> slot;
-  DevBuf<DTabJob> tabjobs;
   DevBuf<int> red_chunks;
   DevBuf<I2> red_range;
-  DevBuf<double> ftab, partial;
-  int nchunks = 0, ntabjobs = 0;
+  DevBuf<double> partial;
+  int nchunks = 0;
+  double ms = -1.0;  // autotuner estimate of the launch time
+  std::vector<DProg> h_progs;
+  std::vector<DChunk> h_chunks;
+  std::vector<int64_t> h_slot;
+  std::vector<I2> h_red_range;
+  std::vector<int> h_red;
+};
+
+// Everything one (topology, zero-block pattern, flat-index subset) needs on the device: the op list
+// and edge tables (independent of the tile width) and one DevVariant per candidate tile width.
+struct DevPlan {
+  DevBuf<DOp> ops;
+  DevBuf<DTabJob> tabjobs;
+  DevBuf<double> ftab;
+  DevBuf<int> queue;
+  int ntabjobs = 0;
   int64_t items = 0;
   double macs = 0;
-  // host copies kept for the pinned upload lifetime
+  bool tables_ready = false;
+  int chosen = -1;
+  std::vector<std::unique_ptr<DevVariant>> var;  // index = tile variant (null if not applicable)
   std::vector<DOp> h_ops;
-  std::vector<DProg> h_progs;
-  std::vector<DTile> h_tiles;
-  std::vector<I2> h_chunks, h_red_range;
-  std::vector<int64_t> h_slot;
   std::vector<DTabJob> h_tab;
-  std::vector<int> h_red;
 };
 
 }  // namespace xca
@@ -205,6 +219,17 @@ double canonical_macs(const Program &pg, int r) {
   return s;
 }
 
+// Program template of one block path: ops resolved to device form, its pole-tuple count and (for an
+// explicit flat-index subset) the requested tuples.
+struct ProgSpec {
+  DProg d;
+  std::vector<int> tuples;  // explicit subset: tuple indices in the class enumeration
+  bool explicit_subset = false;
+  int nl = 0;
+  int fb[kMaxLines + 1];
+  int cls = 0;
+};
+
 DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t nfix) {
   std::vector<int> key;
   key.push_back(m);
@@ -223,7 +248,6 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
   Plan plan;
   build_programs(c->M, c->ps, topo, m, 0, c->G_zero, plan);
   auto dp = std::make_unique<DevPlan>();
-  const int NT = tile_columns();
   const int nl = m - 1;
   // explicit tuple lists when a subset of flat indices is requested
   std::map<std::pair<int, int>, std::vector<int>> wanted;  // (fb_ix, o_ix) -> p_ix list
@@ -237,23 +261,17 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
       wanted[{(int)(rest / p_max), o_ix}].push_back((int)(rest % p_max));
     }
   }
-  // total tile estimate for the chunk size
-  int64_t est_tiles = 0;
-  for (auto &pg : plan.progs) {
-    std::vector<int> fb(m);
-    for (int i = 0, f = pg.fb_ix; i < m; ++i, f /= 2) fb[i] = f % 2;
-    int64_t nt = 1;
-    for (int x = 0; x < nl; ++x) nt *= (int64_t)c->ps.get(fb[x + 1], (pg.cls >> x) & 1).size();
-    int per = std::max(1, std::min(NT, NT / std::max(1, pg.max_w)));
-    est_tiles += (nt + per - 1) / per;
-  }
+  // program specs (tile-width independent)
+  std::vector<ProgSpec> specs;
   for (auto &pg : plan.progs) {
     for (auto &op : pg.ops) {
       if (op.bin > 12 || op.bout > 12 || (op.type == OP_Z && op.qs > kMaxSet))
         throw std::invalid_argument("xca: device path supports blocks of dimension <= 12 and symmetry sets of size <= 8");
     }
-    if (pg.max_w > NT) throw std::invalid_argument("xca: block path too wide for one tile");
-    DProg d{};
+    if (pg.max_w > tile_columns(0)) throw std::invalid_argument("xca: block path too wide for one tile");
+    ProgSpec sp;
+    DProg &d = sp.d;
+    d = DProg{};
     d.op0 = (int)dp->h_ops.size();
     d.nops = (int)pg.ops.size();
     d.sigma_block = pg.sigma_block;
@@ -262,31 +280,29 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
     d.g_init = c->M.G_off[pg.init_block];
     d.sign = pg.sign;
     d.nlines = nl;
-    std::vector<int> fb(m);
-    for (int i = 0, f = pg.fb_ix; i < m; ++i, f /= 2) fb[i] = f % 2;
+    d.max_w = pg.max_w;
+    sp.nl = nl;
+    sp.cls = pg.cls;
+    for (int i = 0, f = pg.fb_ix; i < m; ++i, f /= 2) sp.fb[i] = f % 2;
     int64_t ntup = 1;
     for (int x = 0; x < nl; ++x) {
       d.pk_sign[x] = pg.pk_sign[x];
       d.pk_exp[x] = pg.pk_exp[x];
-      int fwd = fb[x + 1], rb = (pg.cls >> x) & 1;
+      int fwd = sp.fb[x + 1], rb = (pg.cls >> x) & 1;
       d.set_off[x] = c->plist_off[(fwd ? 0 : 2) + rb];
       d.set_len[x] = (int)c->ps.get(fwd, rb).size();
       ntup *= d.set_len[x];
     }
-    // explicit subset: keep the requested tuples of this program's class, as a plist of tuple
-    // indices in the class enumeration
-    std::vector<int> tuples;
     if (fix) {
+      sp.explicit_subset = true;
       auto w = wanted.find({pg.fb_ix, pg.o_ix});
       if (w == wanted.end()) continue;
       for (int pix : w->second) {
-        int u = 0, stride = 1, cls = 0;
-        int pp = pix;
-        bool ok = true;
+        int u = 0, stride = 1, cls = 0, pp = pix;
         for (int x = 0; x < nl; ++x) {
           int l = pp % p;
           pp /= p;
-          int fwd = fb[x + 1];
+          int fwd = sp.fb[x + 1];
           double wl = P.poles[l] * P.beta;
           int rb = fwd ? (wl <= 0 ? 0 : 1) : (wl >= 0 ? 0 : 1);
           cls |= rb << x;
@@ -295,13 +311,12 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
           u += pos * stride;
           stride *= (int)lst.size();
         }
-        if (cls != pg.cls) ok = false;
-        if (ok) tuples.push_back(u);
+        if (cls == pg.cls) sp.tuples.push_back(u);
       }
-      if (tuples.empty()) continue;
-      ntup = (int64_t)tuples.size();
+      if (sp.tuples.empty()) continue;
+      ntup = (int64_t)sp.tuples.size();
     }
-    // ops
+    d.ntup = (int)ntup;
     for (auto &op : pg.ops) {
       DOp o{};
       o.type = op.type; o.bin = op.bin; o.bout = op.bout; o.cols = op.cols;
@@ -319,75 +334,93 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
         for (int k = 0; k < op.qs; ++k) o.orb[k] = c->M.S.to_orb[op.set][k];
       dp->h_ops.push_back(o);
     }
-    d.max_w = pg.max_w;
-    d.items_per_tile = std::max(1, std::min(NT, NT / pg.max_w));
-    int pidx = (int)dp->h_progs.size();
-    dp->h_progs.push_back(d);
-    double macs1 = canonical_macs(pg, r);
-    // tiles and chunks
-    int64_t per = d.items_per_tile;
-    if (!fix) {
-      int64_t ntiles = (ntup + per - 1) / per;
-      const int ct = chunk_tiles(est_tiles);
-      for (int64_t t0 = 0; t0 < ntiles; t0 += ct) {
-        I2 ch;
-        ch.x = (int)dp->h_tiles.size();
-        for (int64_t t = t0; t < std::min(ntiles, t0 + ct); ++t) {
-          DTile tl{pidx, (int)(t * per), (int)std::min<int64_t>(per, ntup - t * per), (int)dp->h_chunks.size()};
-          dp->h_tiles.push_back(tl);
+    dp->items += ntup;
+    dp->macs += canonical_macs(pg, r) * (double)ntup;
+    specs.push_back(std::move(sp));
+  }
+  hipStream_t s = c->stream;
+  // one variant per tile width
+  // candidates: every tile width, or the forced one with the widest as fallback
+  std::vector<int> cand;
+  if (c->tile_variant >= 0) {
+    cand.push_back(c->tile_variant);
+    if (c->tile_variant != 0) cand.push_back(0);
+  } else {
+    for (int v = 0; v < num_tile_variants(); ++v) cand.push_back(v);
+  }
+  dp->var.resize(num_tile_variants());
+  for (int v : cand) {
+    const int NT = tile_columns(v);
+    bool ok = true;
+    int64_t est_tiles = 0;
+    for (auto &sp : specs) {
+      if (sp.d.max_w > NT) { ok = false; break; }
+      int per = std::max(1, NT / sp.d.max_w);
+      est_tiles += (sp.d.ntup + per - 1) / per;
+    }
+    if (!ok) continue;
+    auto V = std::make_unique<DevVariant>();
+    V->NT = NT;
+    const int ct = chunk_tiles(est_tiles);
+    for (auto &sp : specs) {
+      DProg d = sp.d;
+      d.items_per_tile = std::max(1, NT / d.max_w);
+      if (!sp.explicit_subset) {
+        int pidx = (int)V->h_progs.size();
+        V->h_progs.push_back(d);
+        int ntiles = (d.ntup + d.items_per_tile - 1) / d.items_per_tile;
+        for (int t0 = 0; t0 < ntiles; t0 += ct) V->h_chunks.push_back(DChunk{pidx, t0, std::min(ntiles, t0 + ct), 0, 0});
+      } else {
+        // each requested tuple becomes a one-tuple program whose pole lists have length 1
+        for (int u : sp.tuples) {
+          DProg d1 = d;
+          int decode = u;
+          for (int x = 0; x < sp.nl; ++x) {
+            int len = d.set_len[x];
+            int pos = decode % len;
+            decode /= len;
+            d1.set_off[x] = c->plist_off[(sp.fb[x + 1] ? 0 : 2) + ((sp.cls >> x) & 1)] + pos;
+            d1.set_len[x] = 1;
+          }
+          d1.ntup = 1;
+          int pidx = (int)V->h_progs.size();
+          V->h_progs.push_back(d1);
+          V->h_chunks.push_back(DChunk{pidx, 0, 1, 0, 0});
         }
-        ch.y = (int)dp->h_tiles.size();
-        dp->h_chunks.push_back(ch);
-        dp->h_slot.push_back(0);
-      }
-    } else {
-      // each requested tuple becomes a one-item tile of a one-tuple program copy
-      dp->h_progs.pop_back();
-      for (int u : tuples) {
-        DProg d1 = d;
-        d1.set_off[0] = 0;  // patched below
-        int decode = u;
-        // a program whose pole lists have length 1 selects exactly this tuple
-        for (int x = 0; x < nl; ++x) {
-          int len = d.set_len[x];
-          int pos = decode % len;
-          decode /= len;
-          d1.set_off[x] = (c->plist_off[(fb[x + 1] ? 0 : 2) + ((pg.cls >> x) & 1)]) + pos;
-          d1.set_len[x] = 1;
-        }
-        int pi = (int)dp->h_progs.size();
-        dp->h_progs.push_back(d1);
-        I2 ch;
-        ch.x = (int)dp->h_tiles.size();
-        dp->h_tiles.push_back(DTile{pi, 0, 1, (int)dp->h_chunks.size()});
-        ch.y = ch.x + 1;
-        dp->h_chunks.push_back(ch);
-        dp->h_slot.push_back(0);
       }
     }
-    dp->items += ntup;
-    dp->macs += macs1 * (double)ntup;
-  }
-  // partial slots
-  int64_t off = 0;
-  std::vector<std::vector<int>> per_block(P.nb);
-  for (size_t ci = 0; ci < dp->h_chunks.size(); ++ci) {
-    const DProg &d = dp->h_progs[dp->h_tiles[dp->h_chunks[ci].x].prog];
-    dp->h_slot[ci] = off;
-    off += (int64_t)r * d.b0 * d.b0;
-    per_block[d.sigma_block].push_back((int)ci);
-  }
-  for (int b = 0; b < P.nb; ++b) {
-    I2 rg;
-    rg.x = (int)dp->h_red.size();
-    dp->h_red.insert(dp->h_red.end(), per_block[b].begin(), per_block[b].end());
-    rg.y = (int)dp->h_red.size();
-    dp->h_red_range.push_back(rg);
+    // partial slots and the per-block reduction lists
+    int64_t off = 0;
+    std::vector<std::vector<int>> per_block(P.nb);
+    V->h_slot.resize(V->h_chunks.size());
+    for (size_t ci = 0; ci < V->h_chunks.size(); ++ci) {
+      const DProg &d = V->h_progs[V->h_chunks[ci].prog];
+      V->h_chunks[ci].slot = off;
+      V->h_slot[ci] = off;
+      off += (int64_t)r * d.b0 * d.b0;
+      per_block[d.sigma_block].push_back((int)ci);
+    }
+    for (int b = 0; b < P.nb; ++b) {
+      I2 rg;
+      rg.x = (int)V->h_red.size();
+      V->h_red.insert(V->h_red.end(), per_block[b].begin(), per_block[b].end());
+      rg.y = (int)V->h_red.size();
+      V->h_red_range.push_back(rg);
+    }
+    V->nchunks = (int)V->h_chunks.size();
+    V->progs.upload(V->h_progs, s);
+    V->chunks.upload(V->h_chunks, s);
+    V->slot.upload(V->h_slot, s);
+    V->red_chunks.upload(V->h_red, s);
+    V->red_range.upload(V->h_red_range, s);
+    V->partial.alloc(std::max<int64_t>(off, 1));
+    dp->var[v] = std::move(V);
+    if (c->tile_variant >= 0 && dp->chosen < 0) dp->chosen = v;
   }
   // edge-table jobs
   for (const auto &tb : plan.tables) {
     int b = P.dims[tb.block], bb = b * b;
-    int per = std::max(1, NT / bb);
+    int per = std::max(1, tile_columns(0) / bb);
     for (int64_t e0 = 0; e0 < tb.nentries; e0 += per) {
       DTabJob j{};
       j.block = tb.block; j.b = b; j.nlines = tb.nlines; j.e0 = (int)e0;
@@ -398,23 +431,76 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
       dp->h_tab.push_back(j);
     }
   }
-  hipStream_t s = c->stream;
   dp->ops.upload(dp->h_ops, s);
-  dp->progs.upload(dp->h_progs, s);
-  dp->tiles.upload(dp->h_tiles, s);
-  dp->chunks.upload(dp->h_chunks, s);
-  dp->slot.upload(dp->h_slot, s);
   dp->tabjobs.upload(dp->h_tab, s);
-  dp->red_chunks.upload(dp->h_red, s);
-  dp->red_range.upload(dp->h_red_range, s);
   dp->ftab.alloc(std::max<int64_t>(plan.ftab_size, 1));
-  dp->partial.alloc(std::max<int64_t>(off, 1));
-  dp->nchunks = (int)dp->h_chunks.size();
+  dp->queue.alloc(1);
   dp->ntabjobs = (int)dp->h_tab.size();
   HIP_CHECK(hipStreamSynchronize(s));
   DevPlan *raw = dp.get();
   c->plans[key] = std::move(dp);
   return raw;
+}
+
+ChainArgs chain_args(xca_ctx *c, DevPlan *dp, DevVariant *V) {
+  ChainArgs A{};
+  A.D = c->D;
+  A.ops = dp->ops.ptr; A.progs = V->progs.ptr; A.chunks = V->chunks.ptr;
+  A.plist = c->plist.ptr; A.G = c->G.ptr; A.opmats = c->opmats.ptr; A.ftab = dp->ftab.ptr; A.kp = c->kp.ptr; A.km = c->km.ptr;
+  A.delta = c->delta.ptr; A.delta_refl = c->delta_refl.ptr; A.kpow = c->kpow.ptr; A.partial = V->partial.ptr;
+  A.queue = dp->queue.ptr; A.nchunks = V->nchunks;
+  A.r = c->M.P.r; A.p = c->M.P.p; A.n = c->M.P.n; A.maxexp = c->maxexp;
+  return A;
+}
+
+// Tile-width autotuner: time every applicable variant on this plan (all chunks when the plan is
+// small, else an evenly strided sample of about 2048 chunks, scaled by the items it covers) and
+// keep the fastest. Runs once per plan; the partial slots written here are rewritten by the real
+// launch (the first tile of a chunk overwrites its slot).
+void autotune(xca_ctx *c, DevPlan *dp) {
+  hipStream_t s = c->stream;
+  int best = -1;
+  for (int v = 0; v < (int)dp->var.size(); ++v) {
+    DevVariant *V = dp->var[v].get();
+    if (!V || V->nchunks == 0) continue;
+    const int stride = std::max(1, V->nchunks / 2048);
+    DevBuf<DChunk> sample;
+    int64_t items_all = 0, items_sample = 0;
+    std::vector<DChunk> hs;
+    for (int ci = 0; ci < V->nchunks; ++ci) {
+      const DChunk &ch = V->h_chunks[ci];
+      const DProg &d = V->h_progs[ch.prog];
+      int64_t it = std::min<int64_t>((int64_t)ch.t1 * d.items_per_tile, d.ntup) - (int64_t)ch.t0 * d.items_per_tile;
+      items_all += it;
+      if (ci % stride == 0) {
+        hs.push_back(ch);
+        items_sample += it;
+      }
+    }
+    ChainArgs A = chain_args(c, dp, V);
+    if (stride > 1) {
+      sample.upload(hs, s);
+      A.chunks = sample.ptr;
+      A.nchunks = (int)hs.size();
+    }
+    float best_ms = 1e30f;
+    for (int rep = 0; rep < 2; ++rep) {
+      HIP_CHECK(hipEventRecord(c->ev0, s));
+      HIP_CHECK(launch_chain(c->RB, v, &A, s));
+      HIP_CHECK(hipEventRecord(c->ev1, s));
+      HIP_CHECK(hipEventSynchronize(c->ev1));
+      float ms = 0.f;
+      HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      best_ms = std::min(best_ms, ms);
+    }
+    V->ms = best_ms * (double)items_all / (double)std::max<int64_t>(items_sample, 1);
+    if (best < 0 || V->ms < dp->var[best]->ms) best = v;
+  }
+  if (best < 0) throw std::runtime_error("xca: no tile variant applies to this plan");
+  dp->chosen = best;
+  // release the other variants' buffers
+  for (int v = 0; v < (int)dp->var.size(); ++v)
+    if (v != best) dp->var[v].reset();
 }
 
 void accumulate(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t nfix, double scale) {
@@ -424,22 +510,20 @@ void accumulate(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t 
   c->last_items = dp->items;
   c->last_macs = dp->macs;
   c->last_ms = 0;
-  if (dp->nchunks == 0) return;
+  if (dp->items == 0) return;
   if (dp->ntabjobs > 0)
     HIP_CHECK(launch_tab(c->RB, c->D, dp->tabjobs.ptr, dp->ntabjobs, c->G.ptr, c->kp.ptr, c->km.ptr, c->M.P.p, dp->ftab.ptr, s));
-  ChainArgs A{};
-  A.D = c->D;
-  A.ops = dp->ops.ptr; A.progs = dp->progs.ptr; A.tiles = dp->tiles.ptr; A.chunks = dp->chunks.ptr; A.chunk_slot = dp->slot.ptr;
-  A.plist = c->plist.ptr; A.G = c->G.ptr; A.opmats = c->opmats.ptr; A.ftab = dp->ftab.ptr; A.kp = c->kp.ptr; A.km = c->km.ptr;
-  A.delta = c->delta.ptr; A.delta_refl = c->delta_refl.ptr; A.kpow = c->kpow.ptr; A.partial = dp->partial.ptr;
-  A.r = c->M.P.r; A.p = c->M.P.p; A.n = c->M.P.n; A.maxexp = c->maxexp;
+  if (dp->chosen < 0) autotune(c, dp);
+  DevVariant *V = dp->var[dp->chosen].get();
+  c->last_variant = dp->chosen;
+  ChainArgs A = chain_args(c, dp, V);
   HIP_CHECK(hipEventRecord(c->ev0, s));
-  HIP_CHECK(launch_chain(c->RB, &A, dp->nchunks, s));
+  HIP_CHECK(launch_chain(c->RB, dp->chosen, &A, s));
   HIP_CHECK(hipEventRecord(c->ev1, s));
   int maxsz = 0;
   for (int b = 0; b < c->M.P.nb; ++b) maxsz = std::max(maxsz, c->M.P.r * c->M.P.dims[b] * c->M.P.dims[b]);
-  HIP_CHECK(launch_reduce(dp->slot.ptr, dp->red_chunks.ptr, dp->red_range.ptr, c->sig_off.ptr, c->sig_size.ptr, c->M.P.nb, maxsz,
-                          dp->partial.ptr, c->sigma.ptr, scale, s));
+  HIP_CHECK(launch_reduce(V->slot.ptr, V->red_chunks.ptr, V->red_range.ptr, c->sig_off.ptr, c->sig_size.ptr, c->M.P.nb, maxsz,
+                          V->partial.ptr, c->sigma.ptr, scale, s));
 }
 
 }  // namespace
@@ -538,6 +622,8 @@ extern "C" xca_ctx *xca_ctx_create(const xca_problem *desc, int device) {
     c->RB = (P.r <= 32) ? 2 : 4;
     if (P.r > 64) throw std::invalid_argument("xca: DLR rank above 64 is not supported by the device path");
     c->RP = 16 * c->RB;
+    if (const char *tv = std::getenv("XCA_TILE_VARIANT")) c->tile_variant = std::atoi(tv);
+    if (c->tile_variant >= num_tile_variants()) c->tile_variant = -1;
     HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIP_CHECK(hipEventCreate(&c->ev0));
     HIP_CHECK(hipEventCreate(&c->ev1));
@@ -652,6 +738,28 @@ extern "C" int xca_sigma(xca_ctx *c, const double *G, int m, const int *topo, co
     accumulate(c, m, topo, fix, fix ? nfix : 0, 1.0);
     return xca_get_sigma(c, out);
   })
+}
+
+extern "C" int xca_set_tile_variant(xca_ctx *c, int variant) {
+  if (!c) return -1;
+  if (variant < -1 || variant >= num_tile_variants()) {
+    xca::set_error("xca_set_tile_variant: variant out of range");
+    return -1;
+  }
+  if (hipSetDevice(c->device) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
+    xca::set_error("xca_set_tile_variant: device synchronization failed");
+    return -1;
+  }
+  c->tile_variant = variant;
+  c->plans.clear();
+  return 0;
+}
+
+extern "C" int xca_last_tile(xca_ctx *c, int *variant, int *columns) {
+  if (!c) return -1;
+  if (variant) *variant = c->last_variant;
+  if (columns) *columns = c->last_variant >= 0 ? tile_columns(c->last_variant) : 0;
+  return 0;
 }
 
 extern "C" int xca_last_stats(xca_ctx *c, int64_t *items, double *macs, double *kernel_ms) {

@@ -59,6 +59,8 @@ struct xca_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int RB = 0, RP = 0, maxexp = 8;
+  int tile_variant = -1;  // forced tile variant, -1: autotuned per plan
+  int last_variant = -1;
   std::vector<int> plist_h;
   int plist_off[4];  // fwd_a, fwd_b, bwd_a, bwd_b
   xca::DevBuf<double> opmats, kp, km, delta, delta_refl, kpow, G, sigma, frag;

@@ -17,6 +17,7 @@
 // f = vals2coefs(GK) and W f come from edge tables computed by tab_kernel with the same routines.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "engine.hpp"
@@ -72,14 +73,19 @@ __device__ __forceinline__ void zero_acc(v4d (&acc)[RB]) {
 // block sits in lane j + 16 (i % 4), register i / 4. Row of register v in this lane:
 __device__ __forceinline__ int acc_row(int rb, int v, int lane) { return 16 * rb + 4 * v + (lane >> 4); }
 
-// store accumulators to LDS, column c0 + (lane & 15)
+// store accumulators to LDS, column c0 + (lane & 15). Rows at or beyond the DLR rank r are stored
+// as exact zeros: the padding rows must stay zero for every later GEMM that contracts over them, and
+// columns a tile does not use hold stale values that may have overflowed (0 * inf would be NaN).
 template <int RB>
-__device__ __forceinline__ void store_acc(const v4d (&acc)[RB], double *S, int ldn, int c0, int lane) {
+__device__ __forceinline__ void store_acc(const v4d (&acc)[RB], double *S, int ldn, int c0, int lane, int r) {
   const int col = c0 + (lane & 15);
 #pragma unroll
   for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) S[(size_t)acc_row(rb, v, lane) * ldn + col] = acc[rb][v];
+    for (int v = 0; v < 4; ++v) {
+      const int row = acc_row(rb, v, lane);
+      S[(size_t)row * ldn + col] = (row < r) ? acc[rb][v] : 0.0;
+    }
 }
 
 // Row-partitioned GEMM: the wave computes row block rb of the product for NCB column blocks
@@ -105,12 +111,15 @@ __device__ __forceinline__ void gemm_rows(const double *__restrict__ Afrag, int 
 }
 
 template <int NCB>
-__device__ __forceinline__ void store_rows(const v4d (&acc)[NCB], double *S, int ldn, int rb, int cb0, int lane) {
+__device__ __forceinline__ void store_rows(const v4d (&acc)[NCB], double *S, int ldn, int rb, int cb0, int lane, int r) {
   const int col = 16 * cb0 + (lane & 15);
 #pragma unroll
   for (int c = 0; c < NCB; ++c)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) S[(size_t)acc_row(rb, v, lane) * ldn + col + 16 * c] = acc[c][v];
+    for (int v = 0; v < 4; ++v) {
+      const int row = acc_row(rb, v, lane);
+      S[(size_t)row * ldn + col + 16 * c] = (row < r) ? acc[c][v] : 0.0;
+    }
 }
 
 template <int RB>
@@ -223,7 +232,7 @@ __global__ void __launch_bounds__(64 * NW) tab_kernel(DlrFrag D, const DTabJob *
   v4d acc[RB];
   const int c0 = 16 * wv;
   vals2coefs_block<RB>(D, SA, LDN, c0, acc, lane);
-  store_acc<RB>(acc, SB, LDN, c0, lane);
+  store_acc<RB>(acc, SB, LDN, c0, lane, r);
   // f to global
   {
     const int col = c0 + (lane & 15);
@@ -260,15 +269,17 @@ __global__ void __launch_bounds__(64 * NW) tab_kernel(DlrFrag D, const DTabJob *
 }
 
 // -------------------------------------------------------------------------------------------
-// Chain kernel: one workgroup per chunk of tiles of one program.
+// Chain kernel: persistent workgroups drain a queue of chunks (runs of tiles of one program); each
+// chunk writes its own partial slot, so the result does not depend on which workgroup took it.
 
-
-template <int RB, int NW>
+template <int RB, int NW, int NTILE>
 __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainArgs A) {
-  constexpr int RP = 16 * RB, NT = 64 * NW, NTILE = 16 * NW, LDN = NTILE + 4;
-  constexpr int NCB = NTILE / 16 * RB / NW;  // column blocks per wave in the row-partitioned GEMMs
-  static_assert(NW % RB == 0, "waves must tile the row blocks");
+  constexpr int RP = 16 * RB, NT = 64 * NW, LDN = NTILE + 4;
+  constexpr int NCBT = NTILE / 16;           // 16-column blocks in a tile
+  constexpr int NCB = NCBT * RB / NW;        // column blocks per wave in the row-partitioned GEMMs
+  static_assert(NW % RB == 0 && (NCBT * RB) % NW == 0 && NCB >= 1, "waves must tile the (row, column) blocks");
   extern __shared__ double lds[];
+  __shared__ int qnext;
   double *S0 = lds, *S1 = lds + RP * LDN;
   double *Ul = lds + 2 * RP * LDN, *Dl = Ul + RP * (RP - 1) / 2;
   double *ipref = Dl + RP * 4;                                 // per item prefactor (NTILE)
@@ -277,22 +288,28 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainArgs A) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int r = A.r;
   const DlrFrag &D = A.D;
-  const I2 ch = A.chunks[blockIdx.x];
-  const int64_t slot = A.chunk_slot[blockIdx.x];
 
   for (int x = tid; x < RP * LDN; x += NT) { S0[x] = 0.0; S1[x] = 0.0; }
   load_u<RB>(D, Ul, Dl, tid, NT);
   const int rbw = wv % RB, cbw = (wv / RB) * NCB;  // this wave's row block and first column block
   const int nk = (r + 3) >> 2;
+  for (;;) {
+  __syncthreads();
+  if (tid == 0) qnext = atomicAdd(A.queue, 1);
+  __syncthreads();
+  const int chunk = qnext;
+  if (chunk >= A.nchunks) break;
+  const DChunk ch = A.chunks[chunk];
+  const DProg P = A.progs[ch.prog];
+  const int64_t slot = ch.slot;
   bool first_tile = true;
-  for (int ti = ch.x; ti < ch.y; ++ti) {
-    const DTile tile = A.tiles[ti];
-    const DProg P = A.progs[tile.prog];
-    const int P_items = tile.count;
+  for (int ti = ch.t0; ti < ch.t1; ++ti) {
+    const int tstart = ti * P.items_per_tile;
+    const int P_items = min(P.items_per_tile, P.ntup - tstart);
     __syncthreads();
     // pole tuple of each item, prefactor
     for (int it = tid; it < P_items; it += NT) {
-      int u = tile.start + it;
+      int u = tstart + it;
       double pref = 1.0;
       int ls[kMaxLines];
 #pragma unroll
@@ -347,26 +364,25 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainArgs A) {
         __syncthreads();
       } else if (op.type == 1) {  // edge integration
         const int b = op.bin, w = b * cols;
-        const int c0 = 16 * wv;
         {  // y = (L^{-1} P) T -> Soth
           v4d acc[NCB];
           zero_acc(acc);
           gemm_rows<NCB>(D.LiP, 4 * RB, rbw, 0, nk, Scur, LDN, cbw, acc, lane);
-          store_rows<NCB>(acc, Soth, LDN, rbw, cbw, lane);
+          store_rows<NCB>(acc, Soth, LDN, rbw, cbw, lane, r);
         }
         __syncthreads();
-        {  // g = U^{-1} y in place (one column block per wave)
+        for (int cb = wv; cb < NCBT; cb += NW) {  // g = U^{-1} y in place, one column block per wave at a time
           v4d acc[RB];
-          load_acc<RB>(acc, Soth, LDN, c0, lane);
+          load_acc<RB>(acc, Soth, LDN, 16 * cb, lane);
           usolve_lds<RB>(acc, Ul, Dl, r, lane);
-          store_acc<RB>(acc, Soth, LDN, c0, lane);
+          store_acc<RB>(acc, Soth, LDN, 16 * cb, lane, r);
         }
         __syncthreads();
         {  // W g -> Scur (T is dead)
           v4d acc[NCB];
           zero_acc(acc);
           gemm_rows<NCB>(D.W, 4 * RB, rbw, 0, nk, Soth, LDN, cbw, acc, lane);
-          store_rows<NCB>(acc, Scur, LDN, rbw, cbw, lane);
+          store_rows<NCB>(acc, Scur, LDN, rbw, cbw, lane, r);
         }
         __syncthreads();
         // c = -(Wf o g + f o Wg) -> Soth, d = f o g -> Scur
@@ -405,7 +421,7 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainArgs A) {
           gemm_rows<NCB>(D.M2, 8 * RB, rbw, 0, nk, Soth, LDN, cbw, acc, lane);
           gemm_rows<NCB>(D.M2, 8 * RB, rbw, 4 * RB, nk, Scur, LDN, cbw, acc, lane);
           __syncthreads();
-          store_rows<NCB>(acc, Soth, LDN, rbw, cbw, lane);
+          store_rows<NCB>(acc, Soth, LDN, rbw, cbw, lane, r);
         }
         // keep the padding rows of the slab that held d zero for the next GEMM that reads it
         double *tmp = Scur; Scur = Soth; Soth = tmp;
@@ -476,27 +492,44 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainArgs A) {
     first_tile = false;
     (void)rows;
   }
+  }
 }
 
-// Sigma[block] += scale * sum over the block's chunks (fixed order) of the partial slots.
-__global__ void reduce_kernel(const int64_t *__restrict__ slot, const int *__restrict__ red_chunks, const I2 *__restrict__ red_range,
-                              const int64_t *__restrict__ sig_off, const int *__restrict__ sig_size, const double *__restrict__ partial,
-                              double *__restrict__ sigma, double scale) {
-  const int blk = blockIdx.y;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= sig_size[blk]) return;
+// Sigma[block] += scale * sum over the block's chunks of the partial slots. Thread (e, g) of a
+// 16 x 16 workgroup sums chunks g, g + 16, ... of element e; the 16 group sums are then added in a
+// fixed order, so the result is deterministic.
+__global__ void __launch_bounds__(256) reduce_kernel(const int64_t *__restrict__ slot, const int *__restrict__ red_chunks,
+                                                     const I2 *__restrict__ red_range, const int64_t *__restrict__ sig_off,
+                                                     const int *__restrict__ sig_size, const double *__restrict__ partial,
+                                                     double *__restrict__ sigma, double scale) {
+  __shared__ double part[16][17];
+  const int blk = blockIdx.y, e = threadIdx.x & 15, g = threadIdx.x >> 4;
+  const int x = blockIdx.x * 16 + e;
   const I2 rg = red_range[blk];
+  const bool live = x < sig_size[blk];
   double s = 0.0;
-  for (int c = rg.x; c < rg.y; ++c) s += partial[slot[red_chunks[c]] + x];
-  if (rg.y > rg.x) sigma[sig_off[blk] + x] += scale * s;
+  if (live)
+    for (int c = rg.x + g; c < rg.y; c += 16) s += partial[slot[red_chunks[c]] + x];
+  part[g][e] = s;
+  __syncthreads();
+  if (g == 0 && live && rg.y > rg.x) {
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) t += part[q][e];
+    sigma[sig_off[blk] + x] += scale * t;
+  }
+}
+
+__global__ void reset_queue_kernel(int *q) {
+  if (threadIdx.x == 0) q[0] = 0;
 }
 
 // -------------------------------------------------------------------------------------------
 // host-callable launchers
 
-template <int RB, int NW>
+template <int RB, int NW, int NTILE>
 static size_t lds_bytes() {
-  constexpr int RP = 16 * RB, NTILE = 16 * NW, LDN = NTILE + 4;
+  constexpr int RP = 16 * RB, LDN = NTILE + 4;
   return sizeof(double) * ((size_t)2 * RP * LDN + RP * (RP - 1) / 2 + RP * 4 + NTILE + NTILE) + sizeof(int) * NTILE * kMaxLines;
 }
 
@@ -511,12 +544,25 @@ static hipError_t launch_tab_t(const DlrFrag &D, const DTabJob *jobs, int njobs,
   return hipGetLastError();
 }
 
-template <int RB, int NW>
-static hipError_t launch_chain_t(const ChainArgs &A, int nchunks, hipStream_t s) {
-  size_t sh = lds_bytes<RB, NW>();
-  hipError_t e = hipFuncSetAttribute((const void *)chain_kernel<RB, NW>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+// persistent grid: as many workgroups as fit on the device at once, at most one per chunk
+template <int RB, int NW, int NTILE>
+static hipError_t launch_chain_t(const ChainArgs &A, hipStream_t s) {
+  static int resident = 0;  // workgroups resident on the device for this instance
+  const size_t sh = lds_bytes<RB, NW, NTILE>();
+  const void *fn = (const void *)chain_kernel<RB, NW, NTILE>;
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((chain_kernel<RB, NW>), dim3(nchunks), dim3(64 * NW), sh, s, A);
+  if (resident == 0) {
+    int dev = 0, ncu = 0, per = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 64 * NW, sh)) != hipSuccess) return e;
+    resident = std::max(1, ncu * std::max(per, 1));
+  }
+  hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, s, A.queue);
+  const int grid = std::min(A.nchunks, resident);
+  if (grid <= 0) return hipGetLastError();
+  hipLaunchKernelGGL((chain_kernel<RB, NW, NTILE>), dim3(grid), dim3(64 * NW), sh, s, A);
   return hipGetLastError();
 }
 
@@ -525,7 +571,12 @@ static hipError_t launch_chain_t(const ChainArgs &A, int nchunks, hipStream_t s)
 // entry points used by the host orchestration (context.cpp)
 namespace xca::dev {
 
-int tile_columns() { return 128; }
+// Tile variants (waves, columns) selectable per plan: 128 columns on 8 waves, 64 on 4, 32 on 4.
+int num_tile_variants() { return 3; }
+int tile_columns(int variant) {
+  static const int cols[] = {128, 64, 32};
+  return cols[variant];
+}
 
 hipError_t launch_tab(int RB, const DlrFrag &D, const DTabJob *jobs, int njobs, const double *G, const double *kp, const double *km, int p,
                       double *ftab, hipStream_t s) {
@@ -536,18 +587,27 @@ hipError_t launch_tab(int RB, const DlrFrag &D, const DTabJob *jobs, int njobs, 
   }
 }
 
-hipError_t launch_chain(int RB, const void *args, int nchunks, hipStream_t s) {
+hipError_t launch_chain(int RB, int variant, const void *args, hipStream_t s) {
   const ChainArgs &A = *static_cast<const ChainArgs *>(args);
-  switch (RB) {
-    case 2: return launch_chain_t<2, 8>(A, nchunks, s);
-    case 4: return launch_chain_t<4, 8>(A, nchunks, s);
-    default: return hipErrorInvalidValue;
+  if (RB == 2) {
+    switch (variant) {
+      case 0: return launch_chain_t<2, 8, 128>(A, s);
+      case 1: return launch_chain_t<2, 4, 64>(A, s);
+      case 2: return launch_chain_t<2, 4, 32>(A, s);
+    }
+  } else if (RB == 4) {
+    switch (variant) {
+      case 0: return launch_chain_t<4, 8, 128>(A, s);
+      case 1: return launch_chain_t<4, 4, 64>(A, s);
+      case 2: return launch_chain_t<4, 4, 32>(A, s);
+    }
   }
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_reduce(const int64_t *slot, const int *red_chunks, const I2 *red_range, const int64_t *sig_off, const int *sig_size,
                          int nblocks, int max_size, const double *partial, double *sigma, double scale, hipStream_t s) {
-  dim3 grid((max_size + 255) / 256, nblocks);
+  dim3 grid((max_size + 15) / 16, nblocks);
   hipLaunchKernelGGL(reduce_kernel, grid, dim3(256), 0, s, slot, red_chunks, red_range, sig_off, sig_size, partial, sigma, scale);
   return hipGetLastError();
 }

@@ -26,12 +26,16 @@ struct DProg {
   int b1, g_init, sign, nlines;
   int pk_sign[kMaxLines], pk_exp[kMaxLines];
   int set_off[kMaxLines], set_len[kMaxLines];  // pole lists per line (into the pole-list array)
-  int64_t slot_base;                            // not used by the chain kernel
-  int items_per_tile, max_w;
+  int ntup;                                     // pole tuples (items) of the program
+  int items_per_tile, max_w, pad0;
 };
 
-struct DTile {
-  int prog, start, count, chunk;
+// A run of consecutive tiles [t0, t1) of one program; tile t holds items
+// [t * items_per_tile, min((t + 1) * items_per_tile, ntup)). The chunk's results go to its own
+// partial slot (offset in doubles).
+struct DChunk {
+  int prog, t0, t1, pad0;
+  int64_t slot;
 };
 
 // Edge-function table job: entries [e0, e0 + ne) of one table
@@ -61,12 +65,12 @@ struct ChainArgs {
   DlrFrag D;
   const DOp *ops;
   const DProg *progs;
-  const DTile *tiles;
-  const I2 *chunks;            // (tile begin, tile end)
-  const int64_t *chunk_slot;   // partial-slot offset (doubles) per chunk
+  const DChunk *chunks;
   const int *plist;            // pole lists
   const double *G, *opmats, *ftab, *kp, *km, *delta, *delta_refl, *kpow;
   double *partial;
+  int *queue;                  // chunk counter of the persistent launch (reset before each launch)
+  int nchunks;
   int r, p, n, maxexp;
 };
 

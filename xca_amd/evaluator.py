@@ -129,6 +129,16 @@ class DiagramEvaluator:
         check(lib().xca_last_stats(self._h, C.byref(items), C.byref(macs), C.byref(ms)))
         return items.value, macs.value, ms.value
 
+    def set_tile_variant(self, variant):
+        """Force the chain kernel's tile width (0, 1, 2 = 128, 64, 32 columns) or -1 for the autotuner."""
+        check(lib().xca_set_tile_variant(self._h, int(variant)))
+
+    def last_tile(self):
+        v = C.c_int(0)
+        cols = C.c_int(0)
+        check(lib().xca_last_tile(self._h, C.byref(v), C.byref(cols)))
+        return v.value, cols.value
+
     def pseudo_particle_self_energy(self, G, max_order):
         """Sigma summed over orders 1..max_order and connected topologies with the solver's signs
         (block_sparse_solver.py:597-638: net factor -s_topo per topology)."""
