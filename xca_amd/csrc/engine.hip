@@ -93,7 +93,8 @@ __device__ __forceinline__ void store_acc(const v4d (&acc)[RB], double *S, int l
 // One A fragment (global, prefetched two steps ahead) feeds NCB MFMAs; B fragments come from LDS.
 template <int NCB>
 __device__ __forceinline__ void gemm_rows(const double *__restrict__ Afrag, int ldk, int rb, int koff, int NK, const double *S, int ldn,
-                                          int cb0, v4d (&acc)[NCB], int lane) {
+                                          int cb0, int ncnt, v4d (&acc)[NCB], int lane) {
+  if (ncnt <= 0) return;
   const int krow = lane >> 4, col = 16 * cb0 + (lane & 15);
   const double *Ap = Afrag + ((size_t)rb * ldk + koff) * 64 + lane;
   double a0 = Ap[0], a1 = (NK > 1) ? Ap[64] : 0.0;
@@ -102,24 +103,26 @@ __device__ __forceinline__ void gemm_rows(const double *__restrict__ Afrag, int 
     const double an = (ks + 2 < NK) ? Ap[(size_t)(ks + 2) * 64] : 0.0;
     double b[NCB];
 #pragma unroll
-    for (int c = 0; c < NCB; ++c) b[c] = S[(size_t)(4 * ks + krow) * ldn + col + 16 * c];
+    for (int c = 0; c < NCB; ++c) b[c] = (c < ncnt) ? S[(size_t)(4 * ks + krow) * ldn + col + 16 * c] : 0.0;
 #pragma unroll
-    for (int c = 0; c < NCB; ++c) acc[c] = mfma_f64(a0, b[c], acc[c]);
+    for (int c = 0; c < NCB; ++c)
+      if (c < ncnt) acc[c] = mfma_f64(a0, b[c], acc[c]);
     a0 = a1;
     a1 = an;
   }
 }
 
 template <int NCB>
-__device__ __forceinline__ void store_rows(const v4d (&acc)[NCB], double *S, int ldn, int rb, int cb0, int lane, int r) {
+__device__ __forceinline__ void store_rows(const v4d (&acc)[NCB], double *S, int ldn, int rb, int cb0, int ncnt, int lane, int r) {
   const int col = 16 * cb0 + (lane & 15);
 #pragma unroll
   for (int c = 0; c < NCB; ++c)
+    if (c < ncnt)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int row = acc_row(rb, v, lane);
-      S[(size_t)row * ldn + col + 16 * c] = (row < r) ? acc[c][v] : 0.0;
-    }
+      for (int v = 0; v < 4; ++v) {
+        const int row = acc_row(rb, v, lane);
+        S[(size_t)row * ldn + col + 16 * c] = (row < r) ? acc[c][v] : 0.0;
+      }
 }
 
 template <int RB>
@@ -291,7 +294,8 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainArgs A) {
 
   for (int x = tid; x < RP * LDN; x += NT) { S0[x] = 0.0; S1[x] = 0.0; }
   load_u<RB>(D, Ul, Dl, tid, NT);
-  const int rbw = wv % RB, cbw = (wv / RB) * NCB;  // this wave's row block and first column block
+  const int rbw = wv % RB, grp = wv / RB;  // this wave's row block and column group
+  constexpr int NGRP = NW / RB;              // column groups
   const int nk = (r + 3) >> 2;
   for (;;) {
   __syncthreads();
@@ -364,14 +368,19 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainArgs A) {
         __syncthreads();
       } else if (op.type == 1) {  // edge integration
         const int b = op.bin, w = b * cols;
+        // only the column blocks this tile's items occupy: group g of the waves takes blocks
+        // [g cpg, min((g + 1) cpg, ncb)) of every row block
+        const int ncb = (P_items * w + 15) >> 4;
+        const int cpg = (ncb + NGRP - 1) / NGRP;
+        const int cbw = grp * cpg, ncw = min(cpg, ncb - cbw);
         {  // y = (L^{-1} P) T -> Soth
           v4d acc[NCB];
           zero_acc(acc);
-          gemm_rows<NCB>(D.LiP, 4 * RB, rbw, 0, nk, Scur, LDN, cbw, acc, lane);
-          store_rows<NCB>(acc, Soth, LDN, rbw, cbw, lane, r);
+          gemm_rows<NCB>(D.LiP, 4 * RB, rbw, 0, nk, Scur, LDN, cbw, ncw, acc, lane);
+          store_rows<NCB>(acc, Soth, LDN, rbw, cbw, ncw, lane, r);
         }
         __syncthreads();
-        for (int cb = wv; cb < NCBT; cb += NW) {  // g = U^{-1} y in place, one column block per wave at a time
+        for (int cb = wv; cb < ncb; cb += NW) {  // g = U^{-1} y in place, one column block per wave at a time
           v4d acc[RB];
           load_acc<RB>(acc, Soth, LDN, 16 * cb, lane);
           usolve_lds<RB>(acc, Ul, Dl, r, lane);
@@ -381,8 +390,8 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainArgs A) {
         {  // W g -> Scur (T is dead)
           v4d acc[NCB];
           zero_acc(acc);
-          gemm_rows<NCB>(D.W, 4 * RB, rbw, 0, nk, Soth, LDN, cbw, acc, lane);
-          store_rows<NCB>(acc, Scur, LDN, rbw, cbw, lane, r);
+          gemm_rows<NCB>(D.W, 4 * RB, rbw, 0, nk, Soth, LDN, cbw, ncw, acc, lane);
+          store_rows<NCB>(acc, Scur, LDN, rbw, cbw, ncw, lane, r);
         }
         __syncthreads();
         // c = -(Wf o g + f o Wg) -> Soth, d = f o g -> Scur
@@ -418,10 +427,10 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainArgs A) {
         {
           v4d acc[NCB];
           zero_acc(acc);
-          gemm_rows<NCB>(D.M2, 8 * RB, rbw, 0, nk, Soth, LDN, cbw, acc, lane);
-          gemm_rows<NCB>(D.M2, 8 * RB, rbw, 4 * RB, nk, Scur, LDN, cbw, acc, lane);
+          gemm_rows<NCB>(D.M2, 8 * RB, rbw, 0, nk, Soth, LDN, cbw, ncw, acc, lane);
+          gemm_rows<NCB>(D.M2, 8 * RB, rbw, 4 * RB, nk, Scur, LDN, cbw, ncw, acc, lane);
           __syncthreads();
-          store_rows<NCB>(acc, Soth, LDN, rbw, cbw, lane, r);
+          store_rows<NCB>(acc, Soth, LDN, rbw, cbw, ncw, lane, r);
         }
         // keep the padding rows of the slab that held d zero for the next GEMM that reads it
         double *tmp = Scur; Scur = Soth; Soth = tmp;
