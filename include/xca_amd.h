@@ -102,9 +102,43 @@ int xca_dyson_solve(xca_ctx *ctx, const double *sigma, double eta, double *G_out
  * (python/triqs_xca/block_sparse_solver.py:240-355, 427-449): Sigma over orders 1..max_order -> Dyson
  * -> Z = -Tr G(beta), eta += ln|Z|/beta, G_new *= exp(-tau deta) -> stop when max|G_new - G| < tol,
  * else G = mix G_new + (1 - mix) G. Starts from the G0 given to xca_dyson_init and pseudo-particle
- * chemical potential eta0. Outputs the solver's G, eta, the iteration count and the last difference. */
+ * chemical potential eta0. Outputs the solver's G, eta, the iteration count and the last difference.
+ * Each iteration runs as one captured HIP graph (set XCA_NO_GRAPH to launch eagerly). */
 int xca_ppsc_solve(xca_ctx *ctx, double eta0, int max_order, double tol, int maxiter, double mix, double *G_out, double *eta_out,
                    int *iters_out, double *diff_out);
+
+/* Serving: many impurity problems solved to self-consistency on one GPU with continuous batching.
+ * Every admitted problem keeps its operators, G, Sigma, plans and Dyson state resident; one
+ * xca_server_step runs one iteration of every admitted problem in shared batched launches (captured
+ * as a HIP graph while the set of problems is unchanged), retires the converged ones and admits
+ * queued problems in arrival order as batch slots and the device-memory budget allow. A problem's
+ * arithmetic does not depend on the other problems in the batch, so its result is the one
+ * xca_ppsc_solve gives with the same tile variant.
+ * No reference counterpart: the reference runs one BlockSparseSolver per process
+ * (python/triqs_xca/block_sparse_solver.py:240-355). */
+typedef struct xca_server xca_server;
+/* mem_budget_bytes <= 0: 80 % of the free device memory. tile_variant: 0, 1, 2 (fixed, so results do
+ * not depend on timing). */
+xca_server *xca_server_create(int device, int64_t mem_budget_bytes, int max_active, int tile_variant);
+void xca_server_destroy(xca_server *sv);
+/* Queue a problem (G0: its atomic propagator blocks, eta0 the initial pseudo-particle chemical
+ * potential); returns the request id, or < 0 on error (also when its estimated footprint exceeds the
+ * budget). */
+int64_t xca_server_submit(xca_server *sv, const xca_problem *desc, const double *G0, double eta0, int max_order, double tol, int maxiter,
+                          double mix);
+/* Admit, iterate once, retire. Outputs the number of active and queued problems and of problems that
+ * finished in this step. */
+int xca_server_step(xca_server *sv, int *n_active, int *n_queued, int *n_finished);
+/* Ids of requests finished since the last poll (at most max_ids); returns how many. */
+int xca_server_poll(xca_server *sv, int64_t *ids, int max_ids);
+/* Result of a finished request (G blocks, eta, iterations, last difference, converged flag and,
+ * optionally, 3 times in seconds: queued, admitted-to-finished, submitted-to-finished); releases it. */
+int xca_server_result(xca_server *sv, int64_t id, double *G_out, double *eta_out, int *iters_out, double *diff_out, int *converged,
+                      double *times_out);
+/* Estimated device bytes of a submitted request. */
+int64_t xca_server_request_bytes(xca_server *sv, int64_t id);
+/* Iterations run, device bytes held by admitted requests, and the budget. */
+int xca_server_stats(xca_server *sv, int64_t *ticks, int64_t *bytes_in_use, int64_t *budget);
 
 /* Tile width of the chain kernel: variant 0, 1, 2 = 128, 64, 32 columns per tile; -1 (the default)
  * lets the autotuner time every applicable variant on each new plan and keep the fastest. The

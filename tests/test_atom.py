@@ -57,3 +57,17 @@ def test_block_operators_consistent_with_dense():
                 continue
             np.testing.assert_array_equal(ad.c_blocks[(o, j)], dense[np.ix_(ad.fock_states[t], ad.fock_states[j])])
     np.testing.assert_allclose(ad.dense_H(), H.toarray(), atol=0)
+
+
+def test_serving_request_stream_shape():
+    """The serving stream: 1-3 orbitals, Lambda = 20 beta, beta in [10, 100], about 70 % NCA, and the
+    same stream for the same seed."""
+    from xca_amd import models
+    a = models.serving_requests(40, seed=7)
+    b = models.serving_requests(40, seed=7)
+    assert [P.name for P, _ in a] == [P.name for P, _ in b]
+    for P, order in a:
+        assert P.n in (2, 4, 6) and order in (1, 2)
+        assert 10.0 <= P.beta <= 100.0 and P.grid.lam == 20.0 * P.beta
+    frac_nca = np.mean([o == 1 for _, o in a])
+    assert 0.5 <= frac_nca <= 0.9

@@ -37,6 +37,14 @@ EXPORTS = {
     "xca_get_sigma": (C.c_int, [C.c_void_p, d_p]),
     "xca_synchronize": (C.c_int, [C.c_void_p]),
     "xca_last_stats": (C.c_int, [C.c_void_p, i64_p, d_p, d_p]),
+    "xca_server_create": (C.c_void_p, [C.c_int, C.c_int64, C.c_int, C.c_int]),
+    "xca_server_destroy": (None, [C.c_void_p]),
+    "xca_server_submit": (C.c_int64, [C.c_void_p, C.POINTER(XcaProblem), d_p, C.c_double, C.c_int, C.c_double, C.c_int, C.c_double]),
+    "xca_server_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "xca_server_poll": (C.c_int, [C.c_void_p, i64_p, C.c_int]),
+    "xca_server_result": (C.c_int, [C.c_void_p, C.c_int64, d_p, d_p, C.POINTER(C.c_int), d_p, C.POINTER(C.c_int), d_p]),
+    "xca_server_request_bytes": (C.c_int64, [C.c_void_p, C.c_int64]),
+    "xca_server_stats": (C.c_int, [C.c_void_p, i64_p, i64_p, i64_p]),
     "xca_set_tile_variant": (C.c_int, [C.c_void_p, C.c_int]),
     "xca_last_tile": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "xca_dyson_init": (C.c_int, [C.c_void_p, d_p]),

@@ -18,61 +18,12 @@
 #include "host/errors.hpp"
 #include "host/linalg.hpp"
 #include "host/model.hpp"
-
-namespace xca::dev {
-int num_tile_variants();
-int tile_columns(int variant);
-hipError_t launch_tab(int RB, const DlrFrag &D, const DTabJob *jobs, int njobs, const double *G, const double *kp, const double *km, int p,
-                      double *ftab, hipStream_t s);
-hipError_t launch_chain(int RB, int variant, const void *args, hipStream_t s);
-hipError_t launch_fill(int64_t n, double v, double *p, hipStream_t s);
-hipError_t launch_reduce(const int64_t *slot, const int *red_chunks, const I2 *red_range, const int64_t *sig_off, const int *sig_size,
-                         int nblocks, int max_size, const double *partial, double *sigma, double scale, hipStream_t s);
-size_t chain_args_size();
-
-}  // namespace xca::dev
+#include "plan.hpp"
 
 namespace xca {
 
-using namespace xca::dev;
-
 // tiles per chunk: enough chunks to cover the GPU several times, at most 64 tiles per chunk
 static int chunk_tiles(int64_t total_tiles) { return (int)std::max<int64_t>(1, std::min<int64_t>(64, total_tiles / 2048)); }
-
-// Programs, chunks and partial slots of one tile width.
-struct DevVariant {
-  int NT = 0;
-  DevBuf<DProg> progs;
-  DevBuf<DChunk> chunks;
-  DevBuf<int64_t> slot;
-  DevBuf<int> red_chunks;
-  DevBuf<I2> red_range;
-  DevBuf<double> partial;
-  int nchunks = 0;
-  double ms = -1.0;  // autotuner estimate of the launch time
-  std::vector<DProg> h_progs;
-  std::vector<DChunk> h_chunks;
-  std::vector<int64_t> h_slot;
-  std::vector<I2> h_red_range;
-  std::vector<int> h_red;
-};
-
-// Everything one (topology, zero-block pattern, flat-index subset) needs on the device: the op list
-// and edge tables (independent of the tile width) and one DevVariant per candidate tile width.
-struct DevPlan {
-  DevBuf<DOp> ops;
-  DevBuf<DTabJob> tabjobs;
-  DevBuf<double> ftab;
-  DevBuf<int> queue;
-  int ntabjobs = 0;
-  int64_t items = 0;
-  double macs = 0;
-  bool tables_ready = false;
-  int chosen = -1;
-  std::vector<std::unique_ptr<DevVariant>> var;  // index = tile variant (null if not applicable)
-  std::vector<DOp> h_ops;
-  std::vector<DTabJob> h_tab;
-};
 
 }  // namespace xca
 
@@ -229,6 +180,15 @@ struct ProgSpec {
   int fb[kMaxLines + 1];
   int cls = 0;
 };
+
+}  // namespace
+
+namespace xca {
+
+static int64_t variant_bytes(const DevVariant &V) {
+  return (int64_t)(V.progs.n * sizeof(DProg) + V.chunks.n * sizeof(DChunk) + V.slot.n * sizeof(int64_t) + V.red_chunks.n * sizeof(int) +
+                   V.red_range.n * sizeof(I2) + V.partial.n * sizeof(double));
+}
 
 DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t nfix) {
   std::vector<int> key;
@@ -436,6 +396,9 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
   dp->ftab.alloc(std::max<int64_t>(plan.ftab_size, 1));
   dp->queue.alloc(1);
   dp->ntabjobs = (int)dp->h_tab.size();
+  dp->device_bytes = (int64_t)(dp->ops.n * sizeof(DOp) + dp->tabjobs.n * sizeof(DTabJob) + dp->ftab.n * sizeof(double));
+  for (auto &V : dp->var)
+    if (V) dp->device_bytes += variant_bytes(*V);
   HIP_CHECK(hipStreamSynchronize(s));
   DevPlan *raw = dp.get();
   c->plans[key] = std::move(dp);
@@ -448,7 +411,7 @@ ChainArgs chain_args(xca_ctx *c, DevPlan *dp, DevVariant *V) {
   A.ops = dp->ops.ptr; A.progs = V->progs.ptr; A.chunks = V->chunks.ptr;
   A.plist = c->plist.ptr; A.G = c->G.ptr; A.opmats = c->opmats.ptr; A.ftab = dp->ftab.ptr; A.kp = c->kp.ptr; A.km = c->km.ptr;
   A.delta = c->delta.ptr; A.delta_refl = c->delta_refl.ptr; A.kpow = c->kpow.ptr; A.partial = V->partial.ptr;
-  A.queue = dp->queue.ptr; A.nchunks = V->nchunks;
+  A.nchunks = V->nchunks;
   A.r = c->M.P.r; A.p = c->M.P.p; A.n = c->M.P.n; A.maxexp = c->maxexp;
   return A;
 }
@@ -477,16 +440,22 @@ void autotune(xca_ctx *c, DevPlan *dp) {
         items_sample += it;
       }
     }
-    ChainArgs A = chain_args(c, dp, V);
+    std::vector<ChainArgs> A(1, chain_args(c, dp, V));
     if (stride > 1) {
       sample.upload(hs, s);
-      A.chunks = sample.ptr;
-      A.nchunks = (int)hs.size();
+      A[0].chunks = sample.ptr;
+      A[0].nchunks = (int)hs.size();
     }
+    std::vector<int> pre{0, A[0].nchunks};
+    DevBuf<ChainArgs> dA;
+    DevBuf<int> dpre;
+    dA.upload(A, s);
+    dpre.upload(pre, s);
+    const ChainBatch B{dA.ptr, dpre.ptr, 1, A[0].nchunks, dp->queue.ptr};
     float best_ms = 1e30f;
     for (int rep = 0; rep < 2; ++rep) {
       HIP_CHECK(hipEventRecord(c->ev0, s));
-      HIP_CHECK(launch_chain(c->RB, v, &A, s));
+      HIP_CHECK(launch_chain(c->RB, v, B, s));
       HIP_CHECK(hipEventRecord(c->ev1, s));
       HIP_CHECK(hipEventSynchronize(c->ev1));
       float ms = 0.f;
@@ -497,10 +466,26 @@ void autotune(xca_ctx *c, DevPlan *dp) {
     if (best < 0 || V->ms < dp->var[best]->ms) best = v;
   }
   if (best < 0) throw std::runtime_error("xca: no tile variant applies to this plan");
+  HIP_CHECK(hipStreamSynchronize(s));
   dp->chosen = best;
   // release the other variants' buffers
   for (int v = 0; v < (int)dp->var.size(); ++v)
-    if (v != best) dp->var[v].reset();
+    if (v != best && dp->var[v]) {
+      dp->device_bytes -= variant_bytes(*dp->var[v]);
+      dp->var[v].reset();
+    }
+}
+
+TabSeg tab_seg(xca_ctx *c, DevPlan *dp) {
+  TabSeg T{};
+  T.D = c->D;
+  T.jobs = dp->tabjobs.ptr;
+  T.G = c->G.ptr;
+  T.kp = c->kp.ptr;
+  T.km = c->km.ptr;
+  T.ftab = dp->ftab.ptr;
+  T.p = c->M.P.p;
+  return T;
 }
 
 void accumulate(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t nfix, double scale) {
@@ -511,24 +496,32 @@ void accumulate(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t 
   c->last_macs = dp->macs;
   c->last_ms = 0;
   if (dp->items == 0) return;
-  if (dp->ntabjobs > 0)
-    HIP_CHECK(launch_tab(c->RB, c->D, dp->tabjobs.ptr, dp->ntabjobs, c->G.ptr, c->kp.ptr, c->km.ptr, c->M.P.p, dp->ftab.ptr, s));
+  if (dp->ntabjobs > 0) {
+    if (dp->h_tseg.empty()) {
+      dp->h_tseg.assign(1, tab_seg(c, dp));
+      dp->h_tseg_prefix = {0, dp->ntabjobs};
+      dp->tseg.upload(dp->h_tseg, s);
+      dp->tseg_prefix.upload(dp->h_tseg_prefix, s);
+    }
+    HIP_CHECK(launch_tab(c->RB, TabBatch{dp->tseg.ptr, dp->tseg_prefix.ptr, 1, dp->ntabjobs}, s));
+  }
   if (dp->chosen < 0) autotune(c, dp);
   DevVariant *V = dp->var[dp->chosen].get();
   c->last_variant = dp->chosen;
-  ChainArgs A = chain_args(c, dp, V);
+  if (V->h_seg.empty()) {
+    V->h_seg.assign(1, chain_args(c, dp, V));
+    V->h_seg_prefix = {0, V->nchunks};
+    V->seg.upload(V->h_seg, s);
+    V->seg_prefix.upload(V->h_seg_prefix, s);
+  }
   HIP_CHECK(hipEventRecord(c->ev0, s));
-  HIP_CHECK(launch_chain(c->RB, dp->chosen, &A, s));
+  HIP_CHECK(launch_chain(c->RB, dp->chosen, ChainBatch{V->seg.ptr, V->seg_prefix.ptr, 1, V->nchunks, dp->queue.ptr}, s));
   HIP_CHECK(hipEventRecord(c->ev1, s));
   int maxsz = 0;
   for (int b = 0; b < c->M.P.nb; ++b) maxsz = std::max(maxsz, c->M.P.r * c->M.P.dims[b] * c->M.P.dims[b]);
   HIP_CHECK(launch_reduce(V->slot.ptr, V->red_chunks.ptr, V->red_range.ptr, c->sig_off.ptr, c->sig_size.ptr, c->M.P.nb, maxsz,
                           V->partial.ptr, c->sigma.ptr, scale, s));
 }
-
-}  // namespace
-
-namespace xca {
 
 // Connected pairings of 2*order vertices with their parity sign (python/triqs_xca/diag.py:48-127):
 // vertex 0 is paired first with each later vertex, the sign flips for an even distance, and a
@@ -603,65 +596,118 @@ void sigma_full_order(xca_ctx *c, int max_order) {
     return -1;                         \
   }
 
+namespace xca {
+
+// Host part of a context: the model, symmetry sets, operators and pole classes. No device memory.
+xca_ctx *ctx_init_host(const xca_problem *desc) {
+  auto c = std::make_unique<xca_ctx>();
+  c->M.init(problem_from(desc));
+  const Problem &P = c->M.P;
+  std::vector<double> w(P.p);
+  for (int l = 0; l < P.p; ++l) w[l] = P.poles[l] * P.beta;
+  c->ps.init(w);
+  // padded DLR rank: 32 or 64 rows (16-row MFMA blocks; the row-partitioned GEMMs need the row-block
+  // count to divide the waves of a workgroup)
+  if (P.r > 64) throw std::invalid_argument("xca: DLR rank above 64 is not supported by the device path");
+  c->RB = (P.r <= 32) ? 2 : 4;
+  c->RP = 16 * c->RB;
+  if (const char *tv = std::getenv("XCA_TILE_VARIANT")) c->tile_variant = std::atoi(tv);
+  if (c->tile_variant >= num_tile_variants()) c->tile_variant = -1;
+  return c.release();
+}
+
+// Device part: stream, events, resident operator and kernel tables, DLR fragments.
+void ctx_init_device(xca_ctx *c, int device) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw std::runtime_error("xca_ctx_create: no GPU device available");
+  if (device < 0 || device >= ndev) throw std::runtime_error("xca_ctx_create: device index out of range");
+  c->device = device;
+  HIP_CHECK(hipSetDevice(device));
+  const Problem &P = c->M.P;
+  HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIP_CHECK(hipEventCreate(&c->ev0));
+  HIP_CHECK(hipEventCreate(&c->ev1));
+  hipStream_t s = c->stream;
+  c->opmats.upload(c->M.opmats, s);
+  c->kp.upload(c->M.kp, s);
+  c->km.upload(c->M.km, s);
+  c->delta.upload(c->M.delta, s);
+  c->delta_refl.upload(c->M.delta_refl, s);
+  std::vector<double> kpow((size_t)2 * c->maxexp * P.p);
+  for (int sg = 0; sg < 2; ++sg)
+    for (int e = 1; e <= c->maxexp; ++e)
+      for (int l = 0; l < P.p; ++l) kpow[((size_t)sg * c->maxexp + (e - 1)) * P.p + l] = std::pow(sg == 0 ? c->M.k0p[l] : c->M.k0m[l], -e);
+  c->kpow.upload(kpow, s);
+  // pole lists
+  const std::vector<int> *lists[4] = {&c->ps.fwd_a, &c->ps.fwd_b, &c->ps.bwd_a, &c->ps.bwd_b};
+  c->plist_h.clear();
+  for (int i = 0; i < 4; ++i) {
+    c->plist_off[i] = (int)c->plist_h.size();
+    c->plist_h.insert(c->plist_h.end(), lists[i]->begin(), lists[i]->end());
+  }
+  c->plist.upload(c->plist_h, s);
+  std::vector<int64_t> so(P.nb);
+  std::vector<int> ss(P.nb);
+  for (int b = 0; b < P.nb; ++b) {
+    so[b] = c->M.G_off[b];
+    ss[b] = P.r * P.dims[b] * P.dims[b];
+  }
+  c->sig_off.upload(so, s);
+  c->sig_size.upload(ss, s);
+  c->G.alloc(c->M.G_size);
+  c->sigma.alloc(c->M.G_size);
+  HIP_CHECK(launch_fill(c->M.G_size, 0.0, c->sigma.ptr, s));
+  build_frags(c);
+  HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// Device bytes of the resident context tables (without plans and Dyson state).
+int64_t ctx_bytes(const xca_ctx *c) {
+  const Problem &P = c->M.P;
+  const int64_t RP = 16 * c->RB;
+  return 8 * ((int64_t)c->M.opmats.size() + c->M.kp.size() + c->M.km.size() + c->M.delta.size() + c->M.delta_refl.size() +
+              2 * c->maxexp * P.p + 2 * c->M.G_size + 12 * RP * RP) +
+         4 * (4 * P.p + 2 * P.nb);
+}
+
+// Device bytes a plan of this topology would take with the widest tile, from the host program list.
+int64_t plan_bytes_estimate(xca_ctx *c, int m, const int *topo) {
+  Plan plan;
+  build_programs(c->M, c->ps, topo, m, 0, c->G_zero, plan);
+  const Problem &P = c->M.P;
+  const int NT = tile_columns(0);
+  int64_t est_tiles = 0;
+  std::vector<std::pair<int64_t, int>> per_prog;  // (tiles, b0)
+  for (auto &pg : plan.progs) {
+    int fb[kMaxLines + 1];
+    for (int i = 0, f = pg.fb_ix; i < m; ++i, f /= 2) fb[i] = f % 2;
+    int64_t ntup = 1;
+    for (int x = 0; x < m - 1; ++x) ntup *= (int64_t)c->ps.get(fb[x + 1], (pg.cls >> x) & 1).size();
+    const int per = std::max(1, NT / std::max(1, pg.max_w));
+    const int64_t nt = (ntup + per - 1) / per;
+    est_tiles += nt;
+    per_prog.push_back({nt, pg.b0});
+  }
+  const int ct = chunk_tiles(est_tiles);
+  int64_t bytes = 8 * plan.ftab_size;
+  for (auto &pp : per_prog) {
+    const int64_t nch = (pp.first + ct - 1) / ct;
+    bytes += nch * (8 * (int64_t)P.r * pp.second * pp.second + (int64_t)sizeof(DChunk) + 12);
+  }
+  return bytes + (int64_t)plan.progs.size() * (sizeof(DProg) + 4 * sizeof(DOp));
+}
+
+}  // namespace xca
+
 extern "C" xca_ctx *xca_ctx_create(const xca_problem *desc, int device) {
   xca_ctx *c = nullptr;
   try {
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw std::runtime_error("xca_ctx_create: no GPU device available");
-    if (device < 0 || device >= ndev) throw std::runtime_error("xca_ctx_create: device index out of range");
-    c = new xca_ctx();
-    c->device = device;
-    HIP_CHECK(hipSetDevice(device));
-    c->M.init(problem_from(desc));
-    const Problem &P = c->M.P;
-    std::vector<double> w(P.p);
-    for (int l = 0; l < P.p; ++l) w[l] = P.poles[l] * P.beta;
-    c->ps.init(w);
-    // padded DLR rank: 32 or 64 rows (16-row MFMA blocks; the row-partitioned GEMMs need the row-block
-    // count to divide the 8 waves of a workgroup)
-    c->RB = (P.r <= 32) ? 2 : 4;
-    if (P.r > 64) throw std::invalid_argument("xca: DLR rank above 64 is not supported by the device path");
-    c->RP = 16 * c->RB;
-    if (const char *tv = std::getenv("XCA_TILE_VARIANT")) c->tile_variant = std::atoi(tv);
-    if (c->tile_variant >= num_tile_variants()) c->tile_variant = -1;
-    HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    HIP_CHECK(hipEventCreate(&c->ev0));
-    HIP_CHECK(hipEventCreate(&c->ev1));
-    hipStream_t s = c->stream;
-    c->opmats.upload(c->M.opmats, s);
-    c->kp.upload(c->M.kp, s);
-    c->km.upload(c->M.km, s);
-    c->delta.upload(c->M.delta, s);
-    c->delta_refl.upload(c->M.delta_refl, s);
-    std::vector<double> kpow((size_t)2 * c->maxexp * P.p);
-    for (int sg = 0; sg < 2; ++sg)
-      for (int e = 1; e <= c->maxexp; ++e)
-        for (int l = 0; l < P.p; ++l) kpow[((size_t)sg * c->maxexp + (e - 1)) * P.p + l] = std::pow(sg == 0 ? c->M.k0p[l] : c->M.k0m[l], -e);
-    c->kpow.upload(kpow, s);
-    // pole lists
-    const std::vector<int> *lists[4] = {&c->ps.fwd_a, &c->ps.fwd_b, &c->ps.bwd_a, &c->ps.bwd_b};
-    for (int i = 0; i < 4; ++i) {
-      c->plist_off[i] = (int)c->plist_h.size();
-      c->plist_h.insert(c->plist_h.end(), lists[i]->begin(), lists[i]->end());
-    }
-    c->plist.upload(c->plist_h, s);
-    std::vector<int64_t> so(P.nb);
-    std::vector<int> ss(P.nb);
-    for (int b = 0; b < P.nb; ++b) {
-      so[b] = c->M.G_off[b];
-      ss[b] = P.r * P.dims[b] * P.dims[b];
-    }
-    c->sig_off.upload(so, s);
-    c->sig_size.upload(ss, s);
-    c->G.alloc(c->M.G_size);
-    c->sigma.alloc(c->M.G_size);
-    HIP_CHECK(launch_fill(c->M.G_size, 0.0, c->sigma.ptr, s));
-    build_frags(c);
-    HIP_CHECK(hipStreamSynchronize(s));
+    c = ctx_init_host(desc);
+    ctx_init_device(c, device);
     return c;
   } catch (std::exception &e) {
     xca::set_error(e.what());
-    delete c;
+    xca_ctx_destroy(c);
     return nullptr;
   }
 }

@@ -16,22 +16,20 @@
 
 #include <cstdint>
 
+#include "dyson.hpp"
+
 namespace xca::dev {
 
-struct DysonArgs {
-  int r, nb;
-  double beta, eta;
-  const int *dims;          // (nb)
-  const int64_t *goff;      // (nb) offsets of (r, n, n) blocks
-  const int64_t *soff;      // (nb) offsets of (r n)^2 system matrices
-  const double *lu;         // (r, r) LU factors of cf2it
-  const int *piv;           // (r)
-  const double *cf2it, *tcf2it, *W;          // (r, r)
-  const double *G0, *g0c, *Wg0c;              // packed blocks
-  const double *sigma;                        // packed blocks
-  double *sigc, *ggs, *wf, *sys, *gout;       // scratch / output
-  int *ipiv;                                  // (sum r n) pivots
-};
+// segment s of a batch holding global index k: the last s with prefix[s] <= k
+__device__ __forceinline__ static int find_seg(const int *prefix, int nsegs, int k) {
+  int lo = 0, hi = nsegs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= k) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
 
 // in-place LU solve of one column (stride between nodes) with the DLR interpolation factors
 __device__ void lu_solve_col(const double *__restrict__ lu, const int *__restrict__ piv, int r, double *x, int stride) {
@@ -75,8 +73,11 @@ __device__ void lu_solve_t_col(const double *__restrict__ lu, const int *__restr
 }
 
 // Phase 1 (one workgroup per block): sigc, ggs, vals2coefs(ggs), W f, and the system matrix.
-__global__ void __launch_bounds__(256) dyson_build_kernel(DysonArgs A) {
-  const int b = blockIdx.x, n = A.dims[b], r = A.r, nn = n * n, N = r * n;
+__global__ void __launch_bounds__(256) dyson_build_kernel(DysonBatch Bt) {
+  const int pidx = find_seg(Bt.prefix, Bt.nprobs, blockIdx.x);
+  const DysonArgs &A = Bt.probs[pidx];
+  const int b = blockIdx.x - Bt.prefix[pidx], n = A.dims[b], r = A.r, nn = n * n, N = r * n;
+  const double eta = *A.eta;
   const int tid = threadIdx.x, NT = blockDim.x;
   const size_t go = A.goff[b];
   const double *G0 = A.G0 + go, *g0c = A.g0c + go, *Wg0c = A.Wg0c + go;
@@ -116,7 +117,7 @@ __global__ void __launch_bounds__(256) dyson_build_kernel(DysonArgs A) {
       h1 += A.cf2it[t * r + k] * ggs[(size_t)k * nn + c];
       h2 += A.tcf2it[t * r + k] * dtmp[(size_t)k * nn + c];
     }
-    wf[x] = -(A.beta * (h1 + h2) + A.eta * G0[x]);
+    wf[x] = -(A.beta * (h1 + h2) + eta * G0[x]);
   }
   __syncthreads();
   for (int x = tid; x < r * nn; x += NT) ggs[x] = wf[x];
@@ -149,16 +150,13 @@ __global__ void __launch_bounds__(256) dyson_build_kernel(DysonArgs A) {
 }
 
 // Phase 2 (one workgroup per block): pivoted LU of the system matrix and the solve with G0 columns.
-__global__ void __launch_bounds__(256) dyson_solve_kernel(DysonArgs A) {
-  const int b = blockIdx.x, n = A.dims[b], r = A.r, N = r * n;
+__global__ void __launch_bounds__(256) dyson_solve_kernel(DysonBatch Bt) {
+  const int pidx = find_seg(Bt.prefix, Bt.nprobs, blockIdx.x);
+  const DysonArgs &A = Bt.probs[pidx];
+  const int b = blockIdx.x - Bt.prefix[pidx], n = A.dims[b], r = A.r, N = r * n;
   const int tid = threadIdx.x, NT = blockDim.x;
   double *S = A.sys + A.soff[b];
-  int *ip = A.ipiv;
-  {
-    int base = 0;
-    for (int q = 0; q < b; ++q) base += A.r * A.dims[q];
-    ip += base;
-  }
+  int *ip = A.ipiv + A.poff[b];
   __shared__ double red_v[256];
   __shared__ int red_i[256];
   for (int k = 0; k < N; ++k) {
@@ -235,63 +233,77 @@ __global__ void __launch_bounds__(256) dyson_solve_kernel(DysonArgs A) {
   }
 }
 
-// Z = -Tr G(beta): coefficients of each diagonal entry (LU solve) evaluated at t = 1, summed in a
-// fixed order by one thread.
-__global__ void trace_beta_kernel(int r, int nb, int ndiag, const int *dims, const int64_t *goff, const double *lu, const int *piv,
-                                  const double *kbeta, const double *G, double *scratch, double *out) {
-  const int tid = threadIdx.x;
-  for (int d = tid; d < ndiag; d += blockDim.x) {
+// Per problem (one workgroup each): Z = -Tr G_new(beta) from the DLR coefficients of the diagonal
+// entries (LU solve, summed in a fixed order by one thread), deta = ln|Z| / beta, eta += deta; the
+// difference accumulator is cleared for the update kernel that follows.
+__global__ void __launch_bounds__(256) trace_kernel(IterBatch Bt) {
+  const IterArgs &A = Bt.probs[blockIdx.x];
+  const int tid = threadIdx.x, r = A.r;
+  for (int d = tid; d < A.ndiag; d += blockDim.x) {
     int b = 0, i = d;
-    while (i >= dims[b]) { i -= dims[b]; ++b; }
-    const int n = dims[b];
-    double *x = scratch + (size_t)d * r;
-    for (int t = 0; t < r; ++t) x[t] = G[goff[b] + (size_t)t * n * n + i * n + i];
-    lu_solve_col(lu, piv, r, x, 1);
+    while (i >= A.dims[b]) { i -= A.dims[b]; ++b; }
+    const int n = A.dims[b];
+    double *x = A.trs + (size_t)d * r;
+    for (int t = 0; t < r; ++t) x[t] = A.Gnew[A.goff[b] + (size_t)t * n * n + i * n + i];
+    lu_solve_col(A.lu, A.piv, r, x, 1);
     double acc = 0.0;
-    for (int k = 0; k < r; ++k) acc += kbeta[k] * x[k];
+    for (int k = 0; k < r; ++k) acc += A.kbeta[k] * x[k];
     x[0] = acc;
   }
   __syncthreads();
   if (tid == 0) {
-    double s = 0.0;
-    for (int d = 0; d < ndiag; ++d) s += scratch[(size_t)d * r];
-    out[0] = s;
+    double tr = 0.0;
+    for (int d = 0; d < A.ndiag; ++d) tr += A.trs[(size_t)d * r];
+    const double Z = -tr;
+    const double deta = log(fabs(Z)) / A.beta;
+    A.status[kStZ] = Z;
+    A.status[kStDeta] = deta;
+    A.status[kStEta] += deta;
+    A.status[kStDiff] = 0.0;
   }
 }
 
-// G_new *= exp(-tau deta); diff = max |G_new - G|; optionally G = mix G_new + (1 - mix) G
-__global__ void scale_diff_kernel(int64_t total, int r, int nb, const int *dims, const int64_t *goff, const double *tau, double deta,
-                                  double *Gnew, const double *G, unsigned long long *dmax) {
+// G_new *= exp(-tau deta); status[diff] = max |G_new - G| (non-negative doubles order like their bits)
+__global__ void __launch_bounds__(256) scale_diff_kernel(IterBatch Bt) {
+  const IterArgs &A = Bt.probs[blockIdx.y];
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double d = 0.0;
-  if (x < total) {
+  if (x < A.total) {
     int b = 0;
-    while (b + 1 < nb && goff[b + 1] <= x) ++b;
-    const int n = dims[b];
-    const int t = (int)((x - goff[b]) / ((int64_t)n * n));
-    const double v = Gnew[x] * exp(-tau[t] * deta);
-    Gnew[x] = v;
-    d = fabs(v - G[x]);
+    while (b + 1 < A.nb && A.goff[b + 1] <= x) ++b;
+    const int n = A.dims[b];
+    const int t = (int)((x - A.goff[b]) / ((int64_t)n * n));
+    const double v = A.Gnew[x] * exp(-A.tau[t] * A.status[kStDeta]);
+    A.Gnew[x] = v;
+    d = fabs(v - A.G[x]);
   }
-  atomicMax(dmax, (unsigned long long)__double_as_longlong(d));
+  if (blockIdx.x * blockDim.x < A.total)
+    atomicMax(reinterpret_cast<unsigned long long *>(A.status + kStDiff), (unsigned long long)__double_as_longlong(d));
 }
 
-__global__ void mix_kernel(int64_t total, double mix, const double *Gnew, double *G) {
+// Converged (diff < tol): G stays (the solver stops without mixing). Otherwise G = mix G_new + (1 - mix) G.
+__global__ void __launch_bounds__(256) mix_kernel(IterBatch Bt) {
+  const IterArgs &A = Bt.probs[blockIdx.y];
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x < total) G[x] = mix * Gnew[x] + (1.0 - mix) * G[x];
+  const bool conv = A.status[kStDiff] < A.tol;
+  if (x == 0) A.status[kStConv] = conv ? 1.0 : 0.0;
+  if (!conv && x < A.total) A.G[x] = A.mix * A.Gnew[x] + (1.0 - A.mix) * A.G[x];
 }
 
-__global__ void block_max_kernel(int nb, const int64_t *goff, const int64_t *gend, const double *G, double *out) {
+// max |G| per (problem, block), in a fixed order
+__global__ void __launch_bounds__(256) block_max_kernel(IterBatch Bt) {
   __shared__ double part[256];
-  const int b = blockIdx.x;
+  const int pidx = find_seg(Bt.prefix, Bt.nprobs, blockIdx.x);
+  const IterArgs &A = Bt.probs[pidx];
+  const int b = blockIdx.x - Bt.prefix[pidx];
   double m = 0.0;
-  for (int64_t x = goff[b] + threadIdx.x; x < gend[b]; x += blockDim.x) m = fmax(m, fabs(G[x]));
+  for (int64_t x = A.goff[b] + threadIdx.x; x < A.gend[b]; x += blockDim.x) m = fmax(m, fabs(A.G[x]));
   part[threadIdx.x] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int q = 0; q < blockDim.x; ++q) s = fmax(s, part[q]);
-    out[b] = s;
+    double t = 0.0;
+    for (int q = 0; q < (int)blockDim.x; ++q) t = fmax(t, part[q]);
+    A.status[kStBmax + b] = t;
   }
 }
 
@@ -301,29 +313,28 @@ __global__ void fill_kernel(int64_t n, double v, double *p) {
 }
 
 // host-callable launchers
-hipError_t launch_dyson(const DysonArgs &A, hipStream_t s) {
-  hipLaunchKernelGGL(dyson_build_kernel, dim3(A.nb), dim3(256), 0, s, A);
-  hipLaunchKernelGGL(dyson_solve_kernel, dim3(A.nb), dim3(256), 0, s, A);
+hipError_t launch_dyson(const DysonBatch &B, hipStream_t s) {
+  if (B.total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(dyson_build_kernel, dim3(B.total), dim3(256), 0, s, B);
+  hipLaunchKernelGGL(dyson_solve_kernel, dim3(B.total), dim3(256), 0, s, B);
   return hipGetLastError();
 }
-hipError_t launch_trace_beta(int r, int nb, int ndiag, const int *dims, const int64_t *goff, const double *lu, const int *piv, const double *kbeta,
-                             const double *G, double *scratch, double *out, hipStream_t s) {
-  hipLaunchKernelGGL(trace_beta_kernel, dim3(1), dim3(256), 0, s, r, nb, ndiag, dims, goff, lu, piv, kbeta, G, scratch, out);
+
+hipError_t launch_iterate(const IterBatch &B, hipStream_t s) {
+  if (B.nprobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(trace_kernel, dim3(B.nprobs), dim3(256), 0, s, B);
+  const dim3 grid((unsigned)((B.max_total + 255) / 256), B.nprobs);
+  hipLaunchKernelGGL(scale_diff_kernel, grid, dim3(256), 0, s, B);
+  hipLaunchKernelGGL(mix_kernel, grid, dim3(256), 0, s, B);
   return hipGetLastError();
 }
-hipError_t launch_scale_diff(int64_t total, int r, int nb, const int *dims, const int64_t *goff, const double *tau, double deta, double *Gnew,
-                             const double *G, unsigned long long *dmax, hipStream_t s) {
-  hipLaunchKernelGGL(scale_diff_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, total, r, nb, dims, goff, tau, deta, Gnew, G, dmax);
+
+hipError_t launch_block_max(const IterBatch &B, hipStream_t s) {
+  if (B.total_blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(block_max_kernel, dim3(B.total_blocks), dim3(256), 0, s, B);
   return hipGetLastError();
 }
-hipError_t launch_mix(int64_t total, double mix, const double *Gnew, double *G, hipStream_t s) {
-  hipLaunchKernelGGL(mix_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, total, mix, Gnew, G);
-  return hipGetLastError();
-}
-hipError_t launch_block_max(int nb, const int64_t *goff, const int64_t *gend, const double *G, double *out, hipStream_t s) {
-  hipLaunchKernelGGL(block_max_kernel, dim3(nb), dim3(256), 0, s, nb, goff, gend, G, out);
-  return hipGetLastError();
-}
+
 hipError_t launch_fill(int64_t n, double v, double *p, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, v, p);

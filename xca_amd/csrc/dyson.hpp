@@ -1,0 +1,73 @@
+// Device descriptors of the Dyson solve and of the self-consistency update, shared by the kernels
+// (dyson.hip) and the host orchestration (ppsc.cpp, batch.cpp).
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+namespace xca::dev {
+
+// Dyson solve of every block of one problem.
+struct DysonArgs {
+  int r, nb;
+  double beta;
+  const double *eta;        // device scalar: pseudo-particle chemical potential
+  const int *dims;          // (nb)
+  const int64_t *goff;      // (nb) offsets of (r, n, n) blocks
+  const int64_t *soff;      // (nb) offsets of (r n)^2 system matrices
+  const int64_t *poff;      // (nb) offsets of the r n pivots of each block
+  const double *lu;         // (r, r) LU factors of cf2it
+  const int *piv;           // (r)
+  const double *cf2it, *tcf2it, *W;           // (r, r)
+  const double *G0, *g0c, *Wg0c;              // packed blocks
+  const double *sigma;                        // packed blocks
+  double *sigc, *ggs, *wf, *sys, *gout;       // scratch / output
+  int *ipiv;                                  // pivots
+};
+
+// Many problems: global block k belongs to problem s with prefix[s] <= k < prefix[s + 1].
+struct DysonBatch {
+  const DysonArgs *probs;
+  const int *prefix;
+  int nprobs, total;
+};
+
+// Layout of the per-problem iteration status (doubles) read back after every iteration.
+constexpr int kStDiff = 0;   // max |G_new - G| (bits of a non-negative double, atomic max)
+constexpr int kStEta = 1;    // pseudo-particle chemical potential
+constexpr int kStZ = 2;      // Z = -Tr G_new(beta) before the normalisation
+constexpr int kStDeta = 3;   // ln|Z| / beta of this iteration
+constexpr int kStConv = 4;   // 1 when this iteration met the tolerance (G is then left unchanged)
+constexpr int kStBmax = 5;   // (nb) max |G| per block after the update
+constexpr int status_stride(int nb) { return ((kStBmax + nb + 7) / 8) * 8; }
+
+// Normalisation, convergence test and mixing of one problem (block_sparse_solver.py:295-346,427-449).
+struct IterArgs {
+  int r, nb, ndiag, pad0;
+  double beta, tol, mix;
+  int64_t total;            // size of G
+  const int *dims;
+  const int64_t *goff, *gend;
+  const double *lu;
+  const int *piv;
+  const double *kbeta;      // K(1, w_k): evaluation at tau = beta
+  const double *tau;        // (r) absolute times
+  double *G;                // current iterate
+  double *Gnew;             // Dyson solution, normalised in place
+  double *trs;              // (ndiag, r) scratch
+  double *status;           // status_stride(nb) doubles
+};
+
+struct IterBatch {
+  const IterArgs *probs;
+  const int *prefix;        // (problem, block) prefix for the per-block kernels
+  int nprobs, total_blocks;
+  int64_t max_total;        // largest G among the problems
+};
+
+hipError_t launch_dyson(const DysonBatch &B, hipStream_t s);
+hipError_t launch_iterate(const IterBatch &B, hipStream_t s);
+hipError_t launch_block_max(const IterBatch &B, hipStream_t s);
+hipError_t launch_fill(int64_t n, double v, double *p, hipStream_t s);
+
+}  // namespace xca::dev

@@ -207,15 +207,30 @@ __device__ __forceinline__ void vals2coefs_block(const DlrFrag &D, const double 
 // -------------------------------------------------------------------------------------------
 // Edge-function tables: f = vals2coefs(G_b K...) and W f for every entry of a table job.
 
+// segment s of a batch holding global index k: the last s with prefix[s] <= k
+__device__ __forceinline__ int find_segment(const int *prefix, int nsegs, int k) {
+  int lo = 0, hi = nsegs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= k) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 template <int RB, int NW>
-__global__ void __launch_bounds__(64 * NW) tab_kernel(DlrFrag D, const DTabJob *__restrict__ jobs, const double *__restrict__ G,
-                                                      const double *__restrict__ kp, const double *__restrict__ km, int p,
-                                                      double *__restrict__ ftab) {
+__global__ void __launch_bounds__(64 * NW) tab_kernel(TabBatch B) {
   constexpr int RP = 16 * RB, NT = 64 * NW, LDN = 16 * NW + 8;
   extern __shared__ double lds[];
-  double *SA = lds, *SB = lds + RP * LDN, *xch = lds + 2 * RP * LDN;
+  double *SA = lds, *SB = lds + RP * LDN;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const DTabJob J = jobs[blockIdx.x];
+  const int sidx = find_segment(B.prefix, B.nsegs, blockIdx.x);
+  const TabSeg &T = B.segs[sidx];
+  const DlrFrag D = T.D;
+  const double *__restrict__ G = T.G, *__restrict__ kp = T.kp, *__restrict__ km = T.km;
+  double *__restrict__ ftab = T.ftab;
+  const int p = T.p;
+  const DTabJob J = T.jobs[blockIdx.x - B.prefix[sidx]];
   const int r = D.r, b = J.b, bb = b * b, ncol = J.ne * bb;
   for (int x = tid; x < RP * LDN; x += NT) { SA[x] = 0.0; SB[x] = 0.0; }
   __syncthreads();
@@ -276,7 +291,7 @@ __global__ void __launch_bounds__(64 * NW) tab_kernel(DlrFrag D, const DTabJob *
 // chunk writes its own partial slot, so the result does not depend on which workgroup took it.
 
 template <int RB, int NW, int NTILE>
-__global__ void __launch_bounds__(64 * NW) chain_kernel(ChainArgs A) {
+__global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
   constexpr int RP = 16 * RB, NT = 64 * NW, LDN = NTILE + 4;
   constexpr int NCBT = NTILE / 16;           // 16-column blocks in a tile
   constexpr int NCB = NCBT * RB / NW;        // column blocks per wave in the row-partitioned GEMMs
@@ -289,21 +304,28 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainArgs A) {
   unsigned long long *imax = (unsigned long long *)(ipref + NTILE);  // per item max |T|
   int *ipole = (int *)(imax + NTILE);                          // (NTILE, kMaxLines)
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int r = A.r;
-  const DlrFrag &D = A.D;
-
-  for (int x = tid; x < RP * LDN; x += NT) { S0[x] = 0.0; S1[x] = 0.0; }
-  load_u<RB>(D, Ul, Dl, tid, NT);
   const int rbw = wv % RB, grp = wv / RB;  // this wave's row block and column group
   constexpr int NGRP = NW / RB;              // column groups
-  const int nk = (r + 3) >> 2;
+  const double *cur = nullptr;               // DLR fragments of the problem whose U sits in LDS
   for (;;) {
   __syncthreads();
-  if (tid == 0) qnext = atomicAdd(A.queue, 1);
+  if (tid == 0) qnext = atomicAdd(B.queue, 1);
   __syncthreads();
-  const int chunk = qnext;
-  if (chunk >= A.nchunks) break;
-  const DChunk ch = A.chunks[chunk];
+  const int gchunk = qnext;
+  if (gchunk >= B.total) break;
+  const int sidx = find_segment(B.prefix, B.nsegs, gchunk);
+  const ChainArgs A = B.segs[sidx];
+  const DlrFrag &D = A.D;
+  const int r = A.r;
+  const int nk = (r + 3) >> 2;
+  if (D.LiP != cur) {
+    // another problem: clean slabs (the padding rows start at its own rank) and its U factor
+    for (int x = tid; x < RP * LDN; x += NT) { S0[x] = 0.0; S1[x] = 0.0; }
+    load_u<RB>(D, Ul, Dl, tid, NT);
+    cur = D.LiP;
+    __syncthreads();
+  }
+  const DChunk ch = A.chunks[gchunk - B.prefix[sidx]];
   const DProg P = A.progs[ch.prog];
   const int64_t slot = ch.slot;
   bool first_tile = true;
@@ -529,6 +551,36 @@ __global__ void __launch_bounds__(256) reduce_kernel(const int64_t *__restrict__
   }
 }
 
+// Sigma of each problem, written: sum over its plans in order of scale * (chunk sums of that plan,
+// split over 16 groups and combined in a fixed order). blockIdx.y runs over (problem, block) pairs.
+__global__ void __launch_bounds__(256) reduce_assign_kernel(RedBatch B) {
+  __shared__ double part[16][17];
+  const int pidx = find_segment(B.prefix, B.nprobs, blockIdx.y);
+  const RedProb &Q = B.probs[pidx];
+  const int blk = blockIdx.y - B.prefix[pidx];
+  const int e = threadIdx.x & 15, g = threadIdx.x >> 4;
+  const int x = blockIdx.x * 16 + e;
+  const bool live = x < Q.sig_size[blk];
+  double acc = 0.0;
+  for (int q = 0; q < Q.nplans; ++q) {
+    const RedPlan &R = Q.plans[q];
+    const I2 rg = R.red_range[blk];
+    double s = 0.0;
+    if (live)
+      for (int c = rg.x + g; c < rg.y; c += 16) s += R.partial[R.slot[R.red_chunks[c]] + x];
+    __syncthreads();
+    part[g][e] = s;
+    __syncthreads();
+    if (g == 0) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) t += part[k][e];
+      acc += R.scale * t;
+    }
+  }
+  if (g == 0 && live) Q.sigma[Q.sig_off[blk] + x] = acc;
+}
+
 __global__ void reset_queue_kernel(int *q) {
   if (threadIdx.x == 0) q[0] = 0;
 }
@@ -543,35 +595,39 @@ static size_t lds_bytes() {
 }
 
 template <int RB, int NW>
-static hipError_t launch_tab_t(const DlrFrag &D, const DTabJob *jobs, int njobs, const double *G, const double *kp, const double *km, int p,
-                               double *ftab, hipStream_t s) {
+static hipError_t launch_tab_t(const TabBatch &B, hipStream_t s) {
   constexpr int RP = 16 * RB, LDN = 16 * NW + 8;
-  size_t sh = sizeof(double) * ((size_t)2 * RP * LDN + 64 * NW);
-  hipError_t e = hipFuncSetAttribute((const void *)tab_kernel<RB, NW>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((tab_kernel<RB, NW>), dim3(njobs), dim3(64 * NW), sh, s, D, jobs, G, kp, km, p, ftab);
+  const size_t sh = sizeof(double) * ((size_t)2 * RP * LDN + 64 * NW);
+  static bool ready = false;  // attribute set once, outside any stream capture after the first launch
+  if (!ready) {
+    hipError_t e = hipFuncSetAttribute((const void *)tab_kernel<RB, NW>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e != hipSuccess) return e;
+    ready = true;
+  }
+  if (B.total <= 0) return hipSuccess;
+  hipLaunchKernelGGL((tab_kernel<RB, NW>), dim3(B.total), dim3(64 * NW), sh, s, B);
   return hipGetLastError();
 }
 
 // persistent grid: as many workgroups as fit on the device at once, at most one per chunk
 template <int RB, int NW, int NTILE>
-static hipError_t launch_chain_t(const ChainArgs &A, hipStream_t s) {
-  static int resident = 0;  // workgroups resident on the device for this instance
+static hipError_t launch_chain_t(const ChainBatch &B, hipStream_t s) {
+  static int resident = 0;  // workgroups resident on the device for this instance (set on first use)
   const size_t sh = lds_bytes<RB, NW, NTILE>();
   const void *fn = (const void *)chain_kernel<RB, NW, NTILE>;
-  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
-  if (e != hipSuccess) return e;
   if (resident == 0) {
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e != hipSuccess) return e;
     int dev = 0, ncu = 0, per = 0;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
     if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
     if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 64 * NW, sh)) != hipSuccess) return e;
     resident = std::max(1, ncu * std::max(per, 1));
   }
-  hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, s, A.queue);
-  const int grid = std::min(A.nchunks, resident);
-  if (grid <= 0) return hipGetLastError();
-  hipLaunchKernelGGL((chain_kernel<RB, NW, NTILE>), dim3(grid), dim3(64 * NW), sh, s, A);
+  if (B.total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, s, B.queue);
+  const int grid = std::min(B.total, resident);
+  hipLaunchKernelGGL((chain_kernel<RB, NW, NTILE>), dim3(grid), dim3(64 * NW), sh, s, B);
   return hipGetLastError();
 }
 
@@ -587,28 +643,26 @@ int tile_columns(int variant) {
   return cols[variant];
 }
 
-hipError_t launch_tab(int RB, const DlrFrag &D, const DTabJob *jobs, int njobs, const double *G, const double *kp, const double *km, int p,
-                      double *ftab, hipStream_t s) {
+hipError_t launch_tab(int RB, const TabBatch &B, hipStream_t s) {
   switch (RB) {
-    case 2: return launch_tab_t<2, 8>(D, jobs, njobs, G, kp, km, p, ftab, s);
-    case 4: return launch_tab_t<4, 8>(D, jobs, njobs, G, kp, km, p, ftab, s);
+    case 2: return launch_tab_t<2, 8>(B, s);
+    case 4: return launch_tab_t<4, 8>(B, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t launch_chain(int RB, int variant, const void *args, hipStream_t s) {
-  const ChainArgs &A = *static_cast<const ChainArgs *>(args);
+hipError_t launch_chain(int RB, int variant, const ChainBatch &B, hipStream_t s) {
   if (RB == 2) {
     switch (variant) {
-      case 0: return launch_chain_t<2, 8, 128>(A, s);
-      case 1: return launch_chain_t<2, 4, 64>(A, s);
-      case 2: return launch_chain_t<2, 4, 32>(A, s);
+      case 0: return launch_chain_t<2, 8, 128>(B, s);
+      case 1: return launch_chain_t<2, 4, 64>(B, s);
+      case 2: return launch_chain_t<2, 4, 32>(B, s);
     }
   } else if (RB == 4) {
     switch (variant) {
-      case 0: return launch_chain_t<4, 8, 128>(A, s);
-      case 1: return launch_chain_t<4, 4, 64>(A, s);
-      case 2: return launch_chain_t<4, 4, 32>(A, s);
+      case 0: return launch_chain_t<4, 8, 128>(B, s);
+      case 1: return launch_chain_t<4, 4, 64>(B, s);
+      case 2: return launch_chain_t<4, 4, 32>(B, s);
     }
   }
   return hipErrorInvalidValue;
@@ -618,6 +672,13 @@ hipError_t launch_reduce(const int64_t *slot, const int *red_chunks, const I2 *r
                          int nblocks, int max_size, const double *partial, double *sigma, double scale, hipStream_t s) {
   dim3 grid((max_size + 15) / 16, nblocks);
   hipLaunchKernelGGL(reduce_kernel, grid, dim3(256), 0, s, slot, red_chunks, red_range, sig_off, sig_size, partial, sigma, scale);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_assign(const RedBatch &B, int max_size, hipStream_t s) {
+  if (B.total <= 0) return hipSuccess;
+  dim3 grid((max_size + 15) / 16, B.total);
+  hipLaunchKernelGGL(reduce_assign_kernel, grid, dim3(256), 0, s, B);
   return hipGetLastError();
 }
 

@@ -69,9 +69,57 @@ struct ChainArgs {
   const int *plist;            // pole lists
   const double *G, *opmats, *ftab, *kp, *km, *delta, *delta_refl, *kpow;
   double *partial;
-  int *queue;                  // chunk counter of the persistent launch (reset before each launch)
   int nchunks;
   int r, p, n, maxexp;
+};
+
+// One chain launch drains the chunks of several segments (plans, possibly of different problems
+// with the same padded DLR rank): global chunk k belongs to the segment s with
+// prefix[s] <= k < prefix[s + 1].
+struct ChainBatch {
+  const ChainArgs *segs;
+  const int *prefix;  // (nsegs + 1)
+  int nsegs, total;
+  int *queue;         // chunk counter (reset before each launch)
+};
+
+// Edge-table work of one plan.
+struct TabSeg {
+  DlrFrag D;
+  const DTabJob *jobs;
+  const double *G, *kp, *km;
+  double *ftab;
+  int p, pad0;
+};
+
+struct TabBatch {
+  const TabSeg *segs;
+  const int *prefix;  // job prefix (nsegs + 1)
+  int nsegs, total;
+};
+
+// Sigma of one problem = sum over its plans, in order, of scale * (sum over the plan's chunks of
+// the partial slots), written (not accumulated) into the problem's Sigma buffer.
+struct RedPlan {
+  const int64_t *slot;
+  const int *red_chunks;
+  const I2 *red_range;
+  const double *partial;
+  double scale;
+};
+
+struct RedProb {
+  const RedPlan *plans;
+  const int64_t *sig_off;
+  const int *sig_size;
+  double *sigma;
+  int nplans, nb;
+};
+
+struct RedBatch {
+  const RedProb *probs;
+  const int *prefix;  // (problem, block) prefix over problems (nprobs + 1)
+  int nprobs, total;
 };
 
 }  // namespace xca::dev

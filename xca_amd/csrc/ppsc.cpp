@@ -1,6 +1,8 @@
-// Dyson solve and pseudo-particle self-consistency loop on the device; C ABI.
-// Follows dyson_it_ppsc (c++/triqs_xca/dlr_dyson_ppsc.hpp:92-168) and BlockSparseSolver.solve with
-// classic normalization (python/triqs_xca/block_sparse_solver.py:295-346, 427-449, 572-590).
+// Dyson solve on the device; C ABI of the per-problem Dyson state.
+// Follows dyson_it_ppsc (c++/triqs_xca/dlr_dyson_ppsc.hpp:92-168) with the hermitisation of
+// BlockSparseSolver.solve_dyson (python/triqs_xca/block_sparse_solver.py:572-590).
+#include "ppsc.hpp"
+
 #include <hip/hip_runtime_api.h>
 
 #include <cmath>
@@ -8,72 +10,56 @@
 #include <vector>
 
 #include "../../include/xca_amd.h"
-#include "ctx.hpp"
 #include "host/dlr.hpp"
 #include "host/errors.hpp"
 #include "host/linalg.hpp"
 
-namespace xca::dev {
-struct DysonArgs {
-  int r, nb;
-  double beta, eta;
-  const int *dims;
-  const int64_t *goff;
-  const int64_t *soff;
-  const double *lu;
-  const int *piv;
-  const double *cf2it, *tcf2it, *W;
-  const double *G0, *g0c, *Wg0c;
-  const double *sigma;
-  double *sigc, *ggs, *wf, *sys, *gout;
-  int *ipiv;
-};
-hipError_t launch_dyson(const DysonArgs &A, hipStream_t s);
-hipError_t launch_trace_beta(int r, int nb, int ndiag, const int *dims, const int64_t *goff, const double *lu, const int *piv, const double *kbeta,
-                             const double *G, double *scratch, double *out, hipStream_t s);
-hipError_t launch_scale_diff(int64_t total, int r, int nb, const int *dims, const int64_t *goff, const double *tau, double deta, double *Gnew,
-                             const double *G, unsigned long long *dmax, hipStream_t s);
-hipError_t launch_mix(int64_t total, double mix, const double *Gnew, double *G, hipStream_t s);
-hipError_t launch_block_max(int nb, const int64_t *goff, const int64_t *gend, const double *G, double *out, hipStream_t s);
-hipError_t launch_fill(int64_t n, double v, double *p, hipStream_t s);
-}  // namespace xca::dev
-
 using namespace xca;
 using namespace xca::dev;
-
-struct XcaDyson {
-  DevBuf<int> dims, piv, ipiv;
-  DevBuf<int64_t> goff, gend, soff;
-  DevBuf<double> lu, cf2it, tcf2it, W, tau, kbeta;
-  DevBuf<double> G0, g0c, Wg0c, sigc, ggs, wf, sys, gnew, trs, scal, sig_in;
-  DevBuf<unsigned long long> dmax;
-  std::vector<double> G0_h;
-  int ndiag = 0;
-};
 
 void xca_dyson_release(xca_ctx *c) {
   delete c->dyson;
   c->dyson = nullptr;
 }
 
-namespace {
+namespace xca {
 
-DysonArgs dyson_args(xca_ctx *c, double eta, const double *sigma_dev) {
+DysonArgs dyson_args(xca_ctx *c, const double *eta_dev, const double *sigma_dev) {
   XcaDyson *d = c->dyson;
   DysonArgs A{};
-  A.r = c->M.P.r; A.nb = c->M.P.nb; A.beta = c->M.P.beta; A.eta = eta;
-  A.dims = d->dims.ptr; A.goff = d->goff.ptr; A.soff = d->soff.ptr; A.lu = d->lu.ptr; A.piv = d->piv.ptr;
+  A.r = c->M.P.r; A.nb = c->M.P.nb; A.beta = c->M.P.beta; A.eta = eta_dev;
+  A.dims = d->dims.ptr; A.goff = d->goff.ptr; A.soff = d->soff.ptr; A.poff = d->poff.ptr; A.lu = d->lu.ptr; A.piv = d->piv.ptr;
   A.cf2it = d->cf2it.ptr; A.tcf2it = d->tcf2it.ptr; A.W = d->W.ptr;
   A.G0 = d->G0.ptr; A.g0c = d->g0c.ptr; A.Wg0c = d->Wg0c.ptr; A.sigma = sigma_dev;
   A.sigc = d->sigc.ptr; A.ggs = d->ggs.ptr; A.wf = d->wf.ptr; A.sys = d->sys.ptr; A.gout = d->gnew.ptr; A.ipiv = d->ipiv.ptr;
   return A;
 }
 
+IterArgs iter_args(xca_ctx *c, double *status, double tol, double mix) {
+  XcaDyson *d = c->dyson;
+  IterArgs A{};
+  A.r = c->M.P.r; A.nb = c->M.P.nb; A.ndiag = d->ndiag;
+  A.beta = c->M.P.beta; A.tol = tol; A.mix = mix; A.total = c->M.G_size;
+  A.dims = d->dims.ptr; A.goff = d->goff.ptr; A.gend = d->gend.ptr; A.lu = d->lu.ptr; A.piv = d->piv.ptr;
+  A.kbeta = d->kbeta.ptr; A.tau = d->tau.ptr; A.G = c->G.ptr; A.Gnew = d->gnew.ptr; A.trs = d->trs.ptr; A.status = status;
+  return A;
+}
+
+int64_t dyson_bytes(const xca_ctx *c) {
+  const Problem &P = c->M.P;
+  int64_t sys = 0, ndiag = 0;
+  for (int b = 0; b < P.nb; ++b) {
+    sys += (int64_t)P.r * P.dims[b] * P.r * P.dims[b];
+    ndiag += P.dims[b];
+  }
+  return 8 * (7 * c->M.G_size + sys + 2 * P.r * ndiag + 6 * (int64_t)P.r * P.r) + 4 * P.r * ndiag;
+}
+
 void require_dyson(xca_ctx *c) {
   if (!c->dyson) throw std::runtime_error("xca: xca_dyson_init has not been called");
 }
 
-}  // namespace
+}  // namespace xca
 
 extern "C" int xca_dyson_init(xca_ctx *c, const double *G0) {
   try {
@@ -86,19 +72,22 @@ extern "C" int xca_dyson_init(xca_ctx *c, const double *G0) {
     const int r = P.r, nb = P.nb;
     hipStream_t s = c->stream;
     std::vector<int> dims(P.dims);
-    std::vector<int64_t> goff(nb), gend(nb), soff(nb);
-    int64_t so = 0;
+    std::vector<int64_t> goff(nb), gend(nb), soff(nb), poff(nb);
+    int64_t so = 0, po = 0;
     for (int b = 0; b < nb; ++b) {
       goff[b] = c->M.G_off[b];
       gend[b] = goff[b] + (int64_t)r * P.dims[b] * P.dims[b];
       soff[b] = so;
       so += (int64_t)r * P.dims[b] * r * P.dims[b];
+      poff[b] = po;
+      po += (int64_t)r * P.dims[b];
       d->ndiag += P.dims[b];
     }
     d->dims.upload(dims, s);
     d->goff.upload(goff, s);
     d->gend.upload(gend, s);
     d->soff.upload(soff, s);
+    d->poff.upload(poff, s);
     d->lu.upload(o.lu, s);
     d->piv.upload(o.piv, s);
     d->cf2it.upload(o.cf2it, s);
@@ -125,10 +114,10 @@ extern "C" int xca_dyson_init(xca_ctx *c, const double *G0) {
     d->wf.alloc(c->M.G_size);
     d->gnew.alloc(c->M.G_size);
     d->sys.alloc(so);
-    d->ipiv.alloc((size_t)r * d->ndiag);
+    d->ipiv.alloc(po);
     d->trs.alloc((size_t)r * d->ndiag);
-    d->scal.alloc(std::max(8, nb));
-    d->dmax.alloc(1);
+    d->status.alloc(status_stride(nb));
+    d->device_bytes = dyson_bytes(c);
     HIP_CHECK(hipStreamSynchronize(s));
     return 0;
   } catch (std::exception &e) {
@@ -143,77 +132,24 @@ extern "C" int xca_dyson_solve(xca_ctx *c, const double *sigma, double eta, doub
     require_dyson(c);
     HIP_CHECK(hipSetDevice(c->device));
     hipStream_t s = c->stream;
+    XcaDyson *d = c->dyson;
     const double *sig = c->sigma.ptr;
     if (sigma) {
-      c->dyson->sig_in.alloc(c->M.G_size);
-      HIP_CHECK(hipMemcpyAsync(c->dyson->sig_in.ptr, sigma, c->M.G_size * sizeof(double), hipMemcpyHostToDevice, s));
-      sig = c->dyson->sig_in.ptr;
+      d->sig_in.alloc(c->M.G_size);
+      HIP_CHECK(hipMemcpyAsync(d->sig_in.ptr, sigma, c->M.G_size * sizeof(double), hipMemcpyHostToDevice, s));
+      sig = d->sig_in.ptr;
     }
-    DysonArgs A = dyson_args(c, eta, sig);
-    HIP_CHECK(launch_dyson(A, s));
-    HIP_CHECK(hipMemcpyAsync(G_out, c->dyson->gnew.ptr, c->M.G_size * sizeof(double), hipMemcpyDeviceToHost, s));
+    double *eta_dev = d->status.ptr + kStEta;
+    HIP_CHECK(launch_fill(1, eta, eta_dev, s));
+    std::vector<DysonArgs> A(1, dyson_args(c, eta_dev, sig));
+    std::vector<int> pre{0, c->M.P.nb};
+    DevBuf<DysonArgs> dA;
+    DevBuf<int> dpre;
+    dA.upload(A, s);
+    dpre.upload(pre, s);
+    HIP_CHECK(launch_dyson(DysonBatch{dA.ptr, dpre.ptr, 1, c->M.P.nb}, s));
+    HIP_CHECK(hipMemcpyAsync(G_out, d->gnew.ptr, c->M.G_size * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
-    return 0;
-  } catch (std::exception &e) {
-    xca::set_error(e.what());
-    return -1;
-  }
-}
-
-namespace xca {
-void sigma_full_order(xca_ctx *c, int max_order);  // context.cpp
-}
-
-// Pseudo-particle self-consistency: Sigma (orders 1..max_order) -> Dyson -> normalize -> mix, until
-// max |G_new - G| < tol or maxiter iterations. On return G_out holds the solver's G (the last mixed
-// iterate, block_sparse_solver.py:338-346), eta_out the pseudo-particle chemical potential.
-extern "C" int xca_ppsc_solve(xca_ctx *c, double eta0, int max_order, double tol, int maxiter, double mix, double *G_out, double *eta_out,
-                              int *iters_out, double *diff_out) {
-  try {
-    require_dyson(c);
-    HIP_CHECK(hipSetDevice(c->device));
-    XcaDyson *d = c->dyson;
-    const Problem &P = c->M.P;
-    hipStream_t s = c->stream;
-    const int64_t total = c->M.G_size;
-    HIP_CHECK(hipMemcpyAsync(c->G.ptr, d->G0.ptr, total * sizeof(double), hipMemcpyDeviceToDevice, s));
-    double eta = eta0, diff = 0.0;
-    int iter = 0;
-    std::vector<double> bmax(P.nb);
-    for (iter = 1; iter <= maxiter; ++iter) {
-      // zero-block flags of G (block_sparse.cpp:45-49)
-      HIP_CHECK(launch_block_max(P.nb, d->goff.ptr, d->gend.ptr, c->G.ptr, d->scal.ptr, s));
-      HIP_CHECK(hipMemcpyAsync(bmax.data(), d->scal.ptr, P.nb * sizeof(double), hipMemcpyDeviceToHost, s));
-      HIP_CHECK(hipStreamSynchronize(s));
-      c->G_zero.assign(P.nb, 0);
-      for (int b = 0; b < P.nb; ++b) c->G_zero[b] = !(bmax[b] >= 1e-16);
-      c->have_G = true;
-      sigma_full_order(c, max_order);
-      DysonArgs A = dyson_args(c, eta, c->sigma.ptr);
-      HIP_CHECK(launch_dyson(A, s));
-      double trz = 0.0;
-      HIP_CHECK(launch_trace_beta(P.r, P.nb, d->ndiag, d->dims.ptr, d->goff.ptr, d->lu.ptr, d->piv.ptr, d->kbeta.ptr, d->gnew.ptr, d->trs.ptr,
-                                  d->scal.ptr, s));
-      HIP_CHECK(hipMemcpyAsync(&trz, d->scal.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
-      HIP_CHECK(hipStreamSynchronize(s));
-      const double Z = -trz;
-      const double deta = std::log(std::fabs(Z)) / P.beta;
-      HIP_CHECK(launch_fill(1, 0.0, reinterpret_cast<double *>(d->dmax.ptr), s));
-      HIP_CHECK(launch_scale_diff(total, P.r, P.nb, d->dims.ptr, d->goff.ptr, d->tau.ptr, deta, d->gnew.ptr, c->G.ptr, d->dmax.ptr, s));
-      unsigned long long dm = 0;
-      HIP_CHECK(hipMemcpyAsync(&dm, d->dmax.ptr, sizeof(dm), hipMemcpyDeviceToHost, s));
-      HIP_CHECK(hipStreamSynchronize(s));
-      std::memcpy(&diff, &dm, sizeof(double));
-      eta += deta;
-      if (diff < tol) break;
-      HIP_CHECK(launch_mix(total, mix, d->gnew.ptr, c->G.ptr, s));
-    }
-    if (iter > maxiter) iter = maxiter;
-    HIP_CHECK(hipMemcpyAsync(G_out, c->G.ptr, total * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    if (eta_out) *eta_out = eta;
-    if (iters_out) *iters_out = iter;
-    if (diff_out) *diff_out = diff;
     return 0;
   } catch (std::exception &e) {
     xca::set_error(e.what());

@@ -1,0 +1,27 @@
+// Per-problem Dyson state and the descriptors the batched iteration engine launches with.
+#pragma once
+#include <vector>
+
+#include "ctx.hpp"
+#include "dyson.hpp"
+
+struct XcaDyson {
+  xca::DevBuf<int> dims, piv, ipiv;
+  xca::DevBuf<int64_t> goff, gend, soff, poff;
+  xca::DevBuf<double> lu, cf2it, tcf2it, W, tau, kbeta;
+  xca::DevBuf<double> G0, g0c, Wg0c, sigc, ggs, wf, sys, gnew, trs, sig_in;
+  xca::DevBuf<double> status;  // iteration status of the single-problem loop and the eta scalar of xca_dyson_solve
+  std::vector<double> G0_h;
+  int ndiag = 0;
+  int64_t device_bytes = 0;
+};
+
+namespace xca {
+// Dyson descriptor of one problem; eta is a device scalar, sigma a device buffer.
+dev::DysonArgs dyson_args(xca_ctx *c, const double *eta_dev, const double *sigma_dev);
+// Normalisation / convergence / mixing descriptor of one problem writing its status at `status`.
+dev::IterArgs iter_args(xca_ctx *c, double *status, double tol, double mix);
+// Device bytes xca_dyson_init allocates for this problem (without allocating).
+int64_t dyson_bytes(const xca_ctx *c);
+void require_dyson(xca_ctx *c);
+}  // namespace xca

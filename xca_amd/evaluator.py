@@ -23,6 +23,20 @@ def _real(a, what):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def problem_struct(hyb_poles, hyb_coeffs, grid, ad, keep):
+    """xca_problem for the ABI; every array it points to is appended to `keep` (poles first)."""
+    n = ad.fs.n
+    poles = keep(as_f64(hyb_poles).ravel())
+    coeffs = keep(_real(hyb_coeffs, "hyb_coeffs"))
+    if coeffs.ndim != 3 or coeffs.shape[0] != len(poles) or coeffs.shape[1] != n or coeffs.shape[2] != n:
+        raise XcaError(f"DiagramEvaluator: hyb_coeffs must have shape (p, n, n) with p = hyb_poles.size() = {len(poles)} "
+                       f"and n = {n}, got {coeffs.shape}")
+    cm, dm = ad.packed_ops()
+    return XcaProblem(grid.beta, grid.r, dptr(keep(as_f64(grid.it))), dptr(keep(as_f64(grid.rf))), len(poles), dptr(poles), dptr(coeffs),
+                      n, ad.nblocks, iptr(keep(as_i32(ad.dims))), iptr(keep(as_i32(ad.c_conn))), iptr(keep(as_i32(ad.cdag_conn))),
+                      dptr(keep(_real(cm, "c"))), dptr(keep(_real(dm, "cdag"))))
+
+
 class DiagramEvaluator:
     """Pseudo-particle self-energy evaluator on one GPU.
 
@@ -44,15 +58,8 @@ class DiagramEvaluator:
             self._keep.append(a)
             return a
 
-        poles = keep(as_f64(hyb_poles).ravel())
-        coeffs = keep(_real(hyb_coeffs, "hyb_coeffs"))
-        if coeffs.ndim != 3 or coeffs.shape[0] != len(poles) or coeffs.shape[1] != self.n or coeffs.shape[2] != self.n:
-            raise XcaError(f"DiagramEvaluator: hyb_coeffs must have shape (p, n, n) with p = hyb_poles.size() = {len(poles)} "
-                           f"and n = {self.n}, got {coeffs.shape}")
-        cm, dm = ad.packed_ops()
-        d = XcaProblem(self.beta, self.r, dptr(keep(as_f64(grid.it))), dptr(keep(as_f64(grid.rf))), len(poles), dptr(poles), dptr(coeffs),
-                       self.n, ad.nblocks, iptr(keep(as_i32(self.dims))), iptr(keep(as_i32(ad.c_conn))), iptr(keep(as_i32(ad.cdag_conn))),
-                       dptr(keep(_real(cm, "c"))), dptr(keep(_real(dm, "cdag"))))
+        d = problem_struct(hyb_poles, hyb_coeffs, grid, ad, keep)
+        poles = self._keep[0]
         h = lib().xca_ctx_create(C.byref(d), device)
         if not h:
             raise XcaError(last_error())

@@ -150,3 +150,24 @@ def tca_3orb():
     """3-orbital Kanamori, U = 5, J = 0.7, filling 2 (mu = 4.35), beta = 100, Lambda = 2000,
     off-diagonal inter-orbital hybridization 0.05."""
     return kanamori(3, 5.0, 0.7, 4.35, beta=100.0, lam=2000.0, offdiag=0.05)
+
+
+def serving_request(rng, eps=1e-10):
+    """One request of the serving stream: 1-3 orbitals, Kanamori U ~ U[1, 6], J = U/8, half filling
+    mu = [U + (M - 1)(2U' - J)]/2 with U' = U - 2J, beta ~ U[10, 100], Lambda = 20 beta, semicircular
+    hybridization; order NCA with probability 0.7, else OCA. Returns (problem, order)."""
+    norb = int(rng.integers(1, 4))
+    U = float(rng.uniform(1.0, 6.0))
+    J = U / 8.0
+    Up = U - 2.0 * J
+    mu = (U + (norb - 1) * (2.0 * Up - J)) / 2.0
+    beta = float(rng.uniform(10.0, 100.0))
+    order = 1 if rng.uniform() < 0.7 else 2
+    P = kanamori(norb, U, J, mu, beta=beta, lam=20.0 * beta, eps=eps)
+    P.name = f"serve_n{norb}_U{U:.2f}_b{beta:.1f}_o{order}"
+    return P, order
+
+
+def serving_requests(n, seed=7):
+    rng = np.random.default_rng(seed)
+    return [serving_request(rng) for _ in range(n)]
