@@ -76,6 +76,60 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "dispatches": 5}))
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def solve_latency(n):
+    """Self-consistent NCA solve of the single-orbital Anderson impurity (U = 2, mu = 1, beta = 10,
+    Lambda = 100, eps = 1e-10, semicircular hybridization, tol 1e-10, mix 1) timed n times, each from
+    the atomic propagator: device solve including its host driver and result download."""
+    from xca_amd import models
+    from xca_amd.evaluator import DiagramEvaluator
+    P = models.anderson_nca()
+    ev = DiagramEvaluator(P.poles, P.coeffs, P.grid, P.ad)
+    ev.dyson_init(P.G0)
+    ev.ppsc_solve(1, tol=1e-10, maxiter=100, mix=1.0)  # plans and autotuning
+    times, iters = [], 0
+    for _ in range(n):
+        t0 = time.perf_counter()
+        _, _, iters, _ = ev.ppsc_solve(1, tol=1e-10, maxiter=100, mix=1.0)
+        times.append(time.perf_counter() - t0)
+    t = np.array(times) * 1e3
+    return {"workload": "NCA 1-orbital Anderson U=2 mu=1 beta=10 Lambda=100 eps=1e-10, full self-consistency to 1e-10",
+            "solves": n, "iterations": int(iters), "p50_ms": float(np.percentile(t, 50)), "p99_ms": float(np.percentile(t, 99)),
+            "mean_ms": float(t.mean())}
+
+
+def serving(n, seed=7):
+    """Poisson stream of n requests (1-3 orbital Kanamori, U ~ U[1,6], J = U/8, half filling,
+    beta ~ U[10,100], Lambda = 20 beta, eps = 1e-10, NCA 70 % / OCA 30 %, full self-consistency to
+    1e-10). The arrival rate is 1.5x the single-stream solve rate measured on the first 16 requests
+    solved one at a time."""
+    from xca_amd import models
+    from xca_amd.serving import Server, run_stream
+    t0 = time.perf_counter()
+    reqs = models.serving_requests(n, seed=seed)
+    log(f"serving: built {n} requests in {time.perf_counter() - t0:.1f} s")
+    ncal = min(16, n)
+    one = Server(max_active=1, tile_variant=0)
+    cal = run_stream(one, reqs[:ncal], rate=1e6, seed=seed)
+    single_rate = ncal / cal["span_s"]
+    del one
+    rate = 1.5 * single_rate
+    log(f"serving: single-stream {single_rate:.2f} solves/s, arrival rate {rate:.2f}/s")
+    sv = Server(max_active=64, tile_variant=0)
+    out = run_stream(sv, reqs, rate=rate, seed=seed, log=log)
+    iters = [r["iters"] for r in out["results"]]
+    conv = sum(r["converged"] for r in out["results"])
+    return {"workload": "serving stream: 1-3 orbital Kanamori, U~U[1,6], J=U/8, half filling, beta~U[10,100], Lambda=20beta, "
+                        "eps=1e-10, NCA 70%/OCA 30%, self-consistency to 1e-10, Poisson arrivals",
+            "requests": n, "seed": seed, "arrival_rate_per_s": rate, "single_stream_solves_per_s": single_rate,
+            "solves_per_s": out["solves_per_s"], "p50_s": out["p50_s"], "p99_s": out["p99_s"], "span_s": out["span_s"],
+            "converged": int(conv), "mean_iterations": float(np.mean(iters)), "max_active": out["max_active"],
+            "iterations_run": out["stats"]["ticks"], "max_active_slots": 64}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +138,8 @@ def main():
     ap.add_argument("--workload", default="oca2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--solves", type=int, default=50, help="NCA self-consistent solves timed for the latency figures (0: skip)")
+    ap.add_argument("--serving", type=int, default=512, help="requests in the served stream (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (reference dispatches for the HBM byte counters)")
     args = ap.parse_args()
@@ -162,6 +218,12 @@ def main():
                      "traffic": measured_traffic(args.workload), "kernel": "chain_kernel", "kernel_ms": kms,
                      "flops_per_launch": 2.0 * macs, "topology": topos[dom][0].tolist()},
     }
+    if args.solves > 0:
+        out["solve"] = solve_latency(args.solves)
+        log(f"solve latency p50 {out['solve']['p50_ms']:.2f} ms p99 {out['solve']['p99_ms']:.2f} ms")
+    if args.serving > 0:
+        out["serving"] = serving(args.serving)
+        log(f"serving {out['serving']['solves_per_s']:.2f} solves/s p50 {out['serving']['p50_s']:.3f} s p99 {out['serving']['p99_s']:.3f} s")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(P, G, max_order, args.cpu_budget)
     print(json.dumps(out))

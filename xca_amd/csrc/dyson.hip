@@ -1,17 +1,6 @@
-// Pseudo-particle Dyson equation and the iteration helpers on the device (FP64).
-//
-// Per Green's-function block b of dimension n (dyson_it_ppsc::solve, c++/triqs_xca/dlr_dyson_ppsc.hpp:135-168):
-//   sigc = vals2coefs(Sigma_b)
-//   ggs  = -(convolve_TO(beta, g0c, sigc) + eta G0_b)
-//   S    = I + convmat(vals2coefs(ggs))          (r n x r n, acting on one column of G)
-//   G_b  = S^{-1} G0_b  (pivoted LU, n right-hand sides), then hermitized (block_sparse_solver.py:579-587).
-// vals2coefs is the LU solve with the DLR interpolation factors (substitution per column, backward
-// stable). The convolution matrix is first assembled in coefficient space from the same time-ordered
-// convolution formula as the self-energy kernels,
-//   Mc[(t,i),(k',i')] = beta ( -cf2it(t,k') (W f)_k'[i,i'] + tcf2it(t,k') f_k'[i,i'] - sum_k cf2it(t,k) f_k[i,i'] W(k,k') ),
-// and then composed with vals2coefs from the right by transposed LU solves, C^T = cf2it^{-T} Mc^T.
-// Multiplying by an explicit inverse of cf2it instead amplifies rounding by its condition number and
-// leaves the self-consistency loop stalled at ~1e-7.
+// Pseudo-particle Dyson equation and the self-consistency update on the device (FP64), batched over
+// the blocks of many problems (dyson_it_ppsc::solve, c++/triqs_xca/dlr_dyson_ppsc.hpp:135-168, and
+// BlockSparseSolver.solve, python/triqs_xca/block_sparse_solver.py:295-346,427-449,572-590).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -51,43 +40,46 @@ __device__ void lu_solve_col(const double *__restrict__ lu, const int *__restric
   }
 }
 
-// in-place solve of cf2it^T y = x for one column: U^T z = x, L^T w = z, then the row interchanges
-// undone in reverse order
-__device__ void lu_solve_t_col(const double *__restrict__ lu, const int *__restrict__ piv, int r, double *x, int stride) {
-  for (int i = 0; i < r; ++i) {
-    double s = x[(size_t)i * stride];
-    for (int k = 0; k < i; ++k) s -= lu[k * r + i] * x[(size_t)k * stride];
-    x[(size_t)i * stride] = s / lu[i * r + i];
-  }
-  for (int i = r - 2; i >= 0; --i) {
-    double s = x[(size_t)i * stride];
-    for (int k = i + 1; k < r; ++k) s -= lu[k * r + i] * x[(size_t)k * stride];
-    x[(size_t)i * stride] = s;
-  }
-  for (int k = r - 1; k >= 0; --k)
-    if (piv[k] != k) {
-      double a = x[(size_t)k * stride], b = x[(size_t)piv[k] * stride];
-      x[(size_t)k * stride] = b;
-      x[(size_t)piv[k] * stride] = a;
-    }
-}
+typedef double v4d __attribute__((ext_vector_type(4)));
 
-// Phase 1 (one workgroup per block): sigc, ggs, vals2coefs(ggs), W f, and the system matrix.
-__global__ void __launch_bounds__(256) dyson_build_kernel(DysonBatch Bt) {
+__device__ __forceinline__ v4d mfma16x16x4(double a, double b, v4d c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+
+constexpr int kDysonThreads = 1024;
+
+// Dyson solve of one block per workgroup (1024 threads), in the coefficient form of
+// dyson_it_ppsc::solve (dlr_dyson_ppsc.hpp:135-168):
+//   sigc = vals2coefs(Sigma_b), ggs = -(convolve_TO(beta, g0c, sigc) + eta G0_b), f = vals2coefs(ggs)
+//   (I + convmat(f)) G = G0  with  convmat(f) = Mc cf2it^{-1}  is solved as
+//   (cf2it x I + Mc) g = G0,  G = (cf2it x I) g,
+// where Mc[(t,i),(k',i')] = beta ( -cf2it(t,k') (W f)_k'[i,i'] + tcf2it(t,k') f_k'[i,i']
+//                                  - sum_k cf2it(t,k) f_k[i,i'] W(k,k') )
+// maps DLR coefficients to values. The same equation without the inverse of cf2it: its rounding
+// sensitivity is 2e-16 against 6e-16 for the transposed-solve form and 4e-9 for an explicit inverse.
+// The N x N system (N = r n, padded to a multiple of 16 with an identity block) is factorised by a
+// right-looking blocked LU with partial pivoting: 16-column panels, then the trailing update
+// A22 -= L21 U12 on FP64 MFMA tiles. The n right-hand sides (columns of G0) are solved by blocked
+// substitution, G = (cf2it x I) g is formed and hermitised (block_sparse_solver.py:579-587).
+__global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
+  __shared__ double red_v[kDysonThreads];
+  __shared__ int red_i[kDysonThreads];
   const int pidx = find_seg(Bt.prefix, Bt.nprobs, blockIdx.x);
   const DysonArgs &A = Bt.probs[pidx];
   const int b = blockIdx.x - Bt.prefix[pidx], n = A.dims[b], r = A.r, nn = n * n, N = r * n;
+  const int Np = (N + 15) & ~15;
   const double eta = *A.eta;
-  const int tid = threadIdx.x, NT = blockDim.x;
+  const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wv = tid >> 6, NWV = NT >> 6;
   const size_t go = A.goff[b];
   const double *G0 = A.G0 + go, *g0c = A.g0c + go, *Wg0c = A.Wg0c + go;
   double *sigc = A.sigc + go, *ggs = A.ggs + go, *wf = A.wf + go;
-  // sigc = vals2coefs(Sigma_b)
+  double *S = A.sys + A.soff[b];
+  double *B = A.rhs + A.roff[b];
+  int *ip = A.ipiv + A.poff[b];
+  // ---- sigc = vals2coefs(Sigma_b)
   for (int x = tid; x < r * nn; x += NT) sigc[x] = A.sigma[go + x];
   __syncthreads();
   for (int c = tid; c < nn; c += NT) lu_solve_col(A.lu, A.piv, r, sigc + c, nn);
   __syncthreads();
-  // W sigc -> wf (scratch)
+  // ---- W sigc -> wf (scratch)
   for (int x = tid; x < r * nn; x += NT) {
     const int t = x / nn, c = x % nn;
     double s = 0.0;
@@ -95,8 +87,7 @@ __global__ void __launch_bounds__(256) dyson_build_kernel(DysonBatch Bt) {
     wf[x] = s;
   }
   __syncthreads();
-  // node products: c = -(Wg0c sigc + g0c Wsig) -> ggs (temporarily), d = g0c sigc -> sys scratch
-  double *dtmp = A.sys + A.soff[b];
+  // ---- node products: c = -(Wg0c sigc + g0c Wsig) -> ggs (temporarily), d = g0c sigc -> B (scratch)
   for (int x = tid; x < r * nn; x += NT) {
     const int t = x / nn, i = (x % nn) / n, j = x % n;
     const double *f = g0c + (size_t)t * nn, *wfk = Wg0c + (size_t)t * nn, *g = sigc + (size_t)t * nn, *wg = wf + (size_t)t * nn;
@@ -106,23 +97,23 @@ __global__ void __launch_bounds__(256) dyson_build_kernel(DysonBatch Bt) {
       s2 += f[i * n + l] * g[l * n + j];
     }
     ggs[x] = -s1;
-    dtmp[x] = s2;
+    B[x] = s2;
   }
   __syncthreads();
-  // h = beta (cf2it c + tcf2it d); ggs = -(h + eta G0) -> wf (scratch), then copy to ggs
+  // ---- ggs = -(beta (cf2it c + tcf2it d) + eta G0) -> wf, then copied to ggs
   for (int x = tid; x < r * nn; x += NT) {
     const int t = x / nn, c = x % nn;
     double h1 = 0.0, h2 = 0.0;
     for (int k = 0; k < r; ++k) {
       h1 += A.cf2it[t * r + k] * ggs[(size_t)k * nn + c];
-      h2 += A.tcf2it[t * r + k] * dtmp[(size_t)k * nn + c];
+      h2 += A.tcf2it[t * r + k] * B[(size_t)k * nn + c];
     }
     wf[x] = -(A.beta * (h1 + h2) + eta * G0[x]);
   }
   __syncthreads();
   for (int x = tid; x < r * nn; x += NT) ggs[x] = wf[x];
   __syncthreads();
-  // f = vals2coefs(ggs) in place, W f -> wf
+  // ---- f = vals2coefs(ggs) in place, W f -> wf
   for (int c = tid; c < nn; c += NT) lu_solve_col(A.lu, A.piv, r, ggs + c, nn);
   __syncthreads();
   for (int x = tid; x < r * nn; x += NT) {
@@ -132,103 +123,160 @@ __global__ void __launch_bounds__(256) dyson_build_kernel(DysonBatch Bt) {
     wf[x] = s;
   }
   __syncthreads();
-  // coefficient-space convolution matrix Mc
-  double *S = A.sys + A.soff[b];
-  for (int x = tid; x < N * N; x += NT) {
-    const int row = x / N, col = x % N;
-    const int t = row / n, i = row % n, kp = col / n, ip = col % n;
-    const int c = i * n + ip;
-    double s = -A.cf2it[t * r + kp] * wf[(size_t)kp * nn + c] + A.tcf2it[t * r + kp] * ggs[(size_t)kp * nn + c];
-    for (int k = 0; k < r; ++k) s -= A.cf2it[t * r + k] * ggs[(size_t)k * nn + c] * A.W[k * r + kp];
-    S[x] = A.beta * s;
+  // ---- system cf2it x I + Mc, padded with identity to Np x Np
+  for (int x = tid; x < Np * Np; x += NT) {
+    const int row = x / Np, col = x % Np;
+    double v;
+    if (row < N && col < N) {
+      const int t = row / n, i = row % n, kp = col / n, ipp = col % n;
+      const int c = i * n + ipp;
+      double s = -A.cf2it[t * r + kp] * wf[(size_t)kp * nn + c] + A.tcf2it[t * r + kp] * ggs[(size_t)kp * nn + c];
+      for (int k = 0; k < r; ++k) s -= A.cf2it[t * r + k] * ggs[(size_t)k * nn + c] * A.W[k * r + kp];
+      v = A.beta * s + (i == ipp ? A.cf2it[t * r + kp] : 0.0);
+    } else {
+      v = (row == col) ? 1.0 : 0.0;
+    }
+    S[x] = v;
   }
   __syncthreads();
-  // C = Mc cf2it^{-1}: transposed solve per (row, column orbital), then the identity
-  for (int x = tid; x < N * n; x += NT) lu_solve_t_col(A.lu, A.piv, r, S + (size_t)(x / n) * N + (x % n), n);
-  __syncthreads();
-  for (int x = tid; x < N; x += NT) S[(size_t)x * N + x] += 1.0;
-}
-
-// Phase 2 (one workgroup per block): pivoted LU of the system matrix and the solve with G0 columns.
-__global__ void __launch_bounds__(256) dyson_solve_kernel(DysonBatch Bt) {
-  const int pidx = find_seg(Bt.prefix, Bt.nprobs, blockIdx.x);
-  const DysonArgs &A = Bt.probs[pidx];
-  const int b = blockIdx.x - Bt.prefix[pidx], n = A.dims[b], r = A.r, N = r * n;
-  const int tid = threadIdx.x, NT = blockDim.x;
-  double *S = A.sys + A.soff[b];
-  int *ip = A.ipiv + A.poff[b];
-  __shared__ double red_v[256];
-  __shared__ int red_i[256];
-  for (int k = 0; k < N; ++k) {
-    // pivot: first row of maximal |S[i][k]|, i >= k
-    double best = -1.0;
-    int bi = N;
-    for (int i = k + tid; i < N; i += NT) {
-      const double v = fabs(S[(size_t)i * N + k]);
-      if (v > best || (v == best && i < bi)) { best = v; bi = i; }
-    }
-    red_v[tid] = best;
-    red_i[tid] = bi;
-    __syncthreads();
-    for (int h = NT / 2; h > 0; h >>= 1) {
-      if (tid < h) {
-        const double v2 = red_v[tid + h];
-        const int i2 = red_i[tid + h];
-        if (v2 > red_v[tid] || (v2 == red_v[tid] && i2 < red_i[tid])) { red_v[tid] = v2; red_i[tid] = i2; }
+  // ---- blocked LU with partial pivoting
+  for (int k0 = 0; k0 < Np; k0 += 16) {
+    for (int j = k0; j < k0 + 16; ++j) {
+      // pivot: first row of maximal |S[i][j]|, i >= j
+      double best = -1.0;
+      int bi = Np;
+      for (int i = j + tid; i < Np; i += NT) {
+        const double v = fabs(S[(size_t)i * Np + j]);
+        if (v > best) { best = v; bi = i; }
+      }
+      red_v[tid] = best;
+      red_i[tid] = bi;
+      __syncthreads();
+      for (int h = NT / 2; h > 0; h >>= 1) {
+        if (tid < h) {
+          const double v2 = red_v[tid + h];
+          const int i2 = red_i[tid + h];
+          if (v2 > red_v[tid] || (v2 == red_v[tid] && i2 < red_i[tid])) { red_v[tid] = v2; red_i[tid] = i2; }
+        }
+        __syncthreads();
+      }
+      const int p = red_i[0];
+      if (tid == 0) ip[j] = p;
+      if (p != j)
+        for (int c = tid; c < Np; c += NT) {
+          const double a = S[(size_t)j * Np + c];
+          S[(size_t)j * Np + c] = S[(size_t)p * Np + c];
+          S[(size_t)p * Np + c] = a;
+        }
+      __syncthreads();
+      const double pv = S[(size_t)j * Np + j];
+      // scale the column and update the rest of the panel (columns j+1 .. k0+15)
+      const int pw = k0 + 15 - j;  // panel columns right of j
+      for (int i = j + 1 + tid; i < Np; i += NT) {
+        const double l = S[(size_t)i * Np + j] / pv;
+        S[(size_t)i * Np + j] = l;
+        for (int c = 1; c <= pw; ++c) S[(size_t)i * Np + j + c] -= l * S[(size_t)j * Np + j + c];
       }
       __syncthreads();
     }
-    const int p = red_i[0];
-    if (tid == 0) ip[k] = p;
-    if (p != k)
-      for (int j = tid; j < N; j += NT) {
-        const double a = S[(size_t)k * N + j];
-        S[(size_t)k * N + j] = S[(size_t)p * N + j];
-        S[(size_t)p * N + j] = a;
+    const int c1 = k0 + 16;
+    if (c1 >= Np) break;
+    // U12 = L11^{-1} A12 (unit lower 16 x 16), one column per thread
+    for (int c = c1 + tid; c < Np; c += NT) {
+      double x[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) x[q] = S[(size_t)(k0 + q) * Np + c];
+#pragma unroll
+      for (int q = 1; q < 16; ++q) {
+        double s = x[q];
+#pragma unroll
+        for (int q0 = 0; q0 < q; ++q0)
+          if (q0 < q) s -= S[(size_t)(k0 + q) * Np + k0 + q0] * x[q0];
+        x[q] = s;
       }
+#pragma unroll
+      for (int q = 1; q < 16; ++q) S[(size_t)(k0 + q) * Np + c] = x[q];
+    }
     __syncthreads();
-    const double pv = S[(size_t)k * N + k];
-    for (int i = k + 1 + tid; i < N; i += NT) S[(size_t)i * N + k] /= pv;
-    __syncthreads();
-    const int m = N - k - 1;
-    for (int x = tid; x < m * m; x += NT) {
-      const int i = k + 1 + x / m, j = k + 1 + x % m;
-      S[(size_t)i * N + j] -= S[(size_t)i * N + k] * S[(size_t)k * N + j];
+    // A22 -= L21 U12 on 16 x 16 MFMA tiles, one tile per wave at a time
+    const int nt = (Np - c1) >> 4;
+    for (int q = wv; q < nt * nt; q += NWV) {
+      const int ti = c1 + 16 * (q / nt), tj = c1 + 16 * (q % nt);
+      v4d acc;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[v] = S[(size_t)(ti + 4 * v + (lane >> 4)) * Np + tj + (lane & 15)];
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const double a = -S[(size_t)(ti + (lane & 15)) * Np + k0 + 4 * ks + (lane >> 4)];
+        const double bb = S[(size_t)(k0 + 4 * ks + (lane >> 4)) * Np + tj + (lane & 15)];
+        acc = mfma16x16x4(a, bb, acc);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) S[(size_t)(ti + 4 * v + (lane >> 4)) * Np + tj + (lane & 15)] = acc[v];
     }
     __syncthreads();
   }
-  // right-hand sides: B[(t,i)][j] = G0[t][i][j]; one thread per column j
-  double *B = A.gout + A.goff[b];
-  const double *G0 = A.G0 + A.goff[b];
-  for (int x = tid; x < N * n; x += NT) B[x] = G0[x];
+  // ---- right-hand sides: B[(t,i)][j] = G0[t][i][j], padding rows zero; row interchanges in order
+  for (int x = tid; x < Np * n; x += NT) B[x] = (x < N * n) ? G0[x] : 0.0;
   __syncthreads();
-  for (int j = tid; j < n; j += NT) {
-    for (int k = 0; k < N; ++k)
+  for (int col = tid; col < n; col += NT)
+    for (int k = 0; k < Np; ++k)
       if (ip[k] != k) {
-        const double a = B[(size_t)k * n + j];
-        B[(size_t)k * n + j] = B[(size_t)ip[k] * n + j];
-        B[(size_t)ip[k] * n + j] = a;
+        const double a = B[(size_t)k * n + col];
+        B[(size_t)k * n + col] = B[(size_t)ip[k] * n + col];
+        B[(size_t)ip[k] * n + col] = a;
       }
-    for (int i = 1; i < N; ++i) {
-      double s = B[(size_t)i * n + j];
-      for (int k = 0; k < i; ++k) s -= S[(size_t)i * N + k] * B[(size_t)k * n + j];
-      B[(size_t)i * n + j] = s;
+  __syncthreads();
+  // forward substitution with the unit lower factor, 16 rows at a time
+  for (int k0 = 0; k0 < Np; k0 += 16) {
+    for (int col = tid; col < n; col += NT)
+      for (int q = 1; q < 16; ++q) {
+        double s = B[(size_t)(k0 + q) * n + col];
+        for (int q0 = 0; q0 < q; ++q0) s -= S[(size_t)(k0 + q) * Np + k0 + q0] * B[(size_t)(k0 + q0) * n + col];
+        B[(size_t)(k0 + q) * n + col] = s;
+      }
+    __syncthreads();
+    for (int x = tid; x < (Np - k0 - 16) * n; x += NT) {
+      const int i = k0 + 16 + x / n, col = x % n;
+      double s = B[(size_t)i * n + col];
+      for (int q = 0; q < 16; ++q) s -= S[(size_t)i * Np + k0 + q] * B[(size_t)(k0 + q) * n + col];
+      B[(size_t)i * n + col] = s;
     }
-    for (int i = N - 1; i >= 0; --i) {
-      double s = B[(size_t)i * n + j];
-      for (int k = i + 1; k < N; ++k) s -= S[(size_t)i * N + k] * B[(size_t)k * n + j];
-      B[(size_t)i * n + j] = s / S[(size_t)i * N + i];
+    __syncthreads();
+  }
+  // backward substitution with the upper factor
+  for (int k0 = Np - 16; k0 >= 0; k0 -= 16) {
+    for (int col = tid; col < n; col += NT)
+      for (int q = 15; q >= 0; --q) {
+        double s = B[(size_t)(k0 + q) * n + col];
+        for (int q1 = q + 1; q1 < 16; ++q1) s -= S[(size_t)(k0 + q) * Np + k0 + q1] * B[(size_t)(k0 + q1) * n + col];
+        B[(size_t)(k0 + q) * n + col] = s / S[(size_t)(k0 + q) * Np + k0 + q];
+      }
+    __syncthreads();
+    for (int x = tid; x < k0 * n; x += NT) {
+      const int i = x / n, col = x % n;
+      double s = B[(size_t)i * n + col];
+      for (int q = 0; q < 16; ++q) s -= S[(size_t)i * Np + k0 + q] * B[(size_t)(k0 + q) * n + col];
+      B[(size_t)i * n + col] = s;
     }
+    __syncthreads();
+  }
+  // ---- G[t][i][j] = sum_k cf2it(t,k) g[(k,i)][j], hermitised
+  double *Gout = A.gout + go;
+  for (int x = tid; x < r * nn; x += NT) {
+    const int t = x / nn, ij = x % nn;
+    double s = 0.0;
+    for (int k = 0; k < r; ++k) s += A.cf2it[t * r + k] * B[(size_t)k * nn + ij];
+    Gout[x] = s;
   }
   __syncthreads();
-  // hermitize (real symmetric part) per node
-  for (int x = tid; x < r * n * n; x += NT) {
-    const int t = x / (n * n), i = (x / n) % n, j = x % n;
+  for (int x = tid; x < r * nn; x += NT) {
+    const int t = x / nn, i = (x / n) % n, j = x % n;
     if (i < j) {
-      const double a = B[(size_t)t * n * n + i * n + j], c = B[(size_t)t * n * n + j * n + i];
+      const double a = Gout[(size_t)t * nn + i * n + j], c = Gout[(size_t)t * nn + j * n + i];
       const double h = 0.5 * (a + c);
-      B[(size_t)t * n * n + i * n + j] = h;
-      B[(size_t)t * n * n + j * n + i] = h;
+      Gout[(size_t)t * nn + i * n + j] = h;
+      Gout[(size_t)t * nn + j * n + i] = h;
     }
   }
 }
@@ -315,8 +363,7 @@ __global__ void fill_kernel(int64_t n, double v, double *p) {
 // host-callable launchers
 hipError_t launch_dyson(const DysonBatch &B, hipStream_t s) {
   if (B.total <= 0) return hipSuccess;
-  hipLaunchKernelGGL(dyson_build_kernel, dim3(B.total), dim3(256), 0, s, B);
-  hipLaunchKernelGGL(dyson_solve_kernel, dim3(B.total), dim3(256), 0, s, B);
+  hipLaunchKernelGGL(dyson_kernel, dim3(B.total), dim3(kDysonThreads), 0, s, B);
   return hipGetLastError();
 }
 

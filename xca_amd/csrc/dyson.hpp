@@ -14,14 +14,15 @@ struct DysonArgs {
   const double *eta;        // device scalar: pseudo-particle chemical potential
   const int *dims;          // (nb)
   const int64_t *goff;      // (nb) offsets of (r, n, n) blocks
-  const int64_t *soff;      // (nb) offsets of (r n)^2 system matrices
-  const int64_t *poff;      // (nb) offsets of the r n pivots of each block
+  const int64_t *soff;      // (nb) offsets of the Np x Np system matrices (Np = r n rounded up to 16)
+  const int64_t *poff;      // (nb) offsets of the Np pivots of each block
+  const int64_t *roff;      // (nb) offsets of the Np x n right-hand sides
   const double *lu;         // (r, r) LU factors of cf2it
   const int *piv;           // (r)
   const double *cf2it, *tcf2it, *W;           // (r, r)
   const double *G0, *g0c, *Wg0c;              // packed blocks
   const double *sigma;                        // packed blocks
-  double *sigc, *ggs, *wf, *sys, *gout;       // scratch / output
+  double *sigc, *ggs, *wf, *sys, *rhs, *gout; // scratch / output
   int *ipiv;                                  // pivots
 };
 

@@ -28,10 +28,10 @@ DysonArgs dyson_args(xca_ctx *c, const double *eta_dev, const double *sigma_dev)
   XcaDyson *d = c->dyson;
   DysonArgs A{};
   A.r = c->M.P.r; A.nb = c->M.P.nb; A.beta = c->M.P.beta; A.eta = eta_dev;
-  A.dims = d->dims.ptr; A.goff = d->goff.ptr; A.soff = d->soff.ptr; A.poff = d->poff.ptr; A.lu = d->lu.ptr; A.piv = d->piv.ptr;
+  A.dims = d->dims.ptr; A.goff = d->goff.ptr; A.soff = d->soff.ptr; A.poff = d->poff.ptr; A.roff = d->roff.ptr; A.lu = d->lu.ptr; A.piv = d->piv.ptr;
   A.cf2it = d->cf2it.ptr; A.tcf2it = d->tcf2it.ptr; A.W = d->W.ptr;
   A.G0 = d->G0.ptr; A.g0c = d->g0c.ptr; A.Wg0c = d->Wg0c.ptr; A.sigma = sigma_dev;
-  A.sigc = d->sigc.ptr; A.ggs = d->ggs.ptr; A.wf = d->wf.ptr; A.sys = d->sys.ptr; A.gout = d->gnew.ptr; A.ipiv = d->ipiv.ptr;
+  A.sigc = d->sigc.ptr; A.ggs = d->ggs.ptr; A.wf = d->wf.ptr; A.sys = d->sys.ptr; A.rhs = d->rhs.ptr; A.gout = d->gnew.ptr; A.ipiv = d->ipiv.ptr;
   return A;
 }
 
@@ -47,12 +47,15 @@ IterArgs iter_args(xca_ctx *c, double *status, double tol, double mix) {
 
 int64_t dyson_bytes(const xca_ctx *c) {
   const Problem &P = c->M.P;
-  int64_t sys = 0, ndiag = 0;
+  int64_t sys = 0, rhs = 0, piv = 0, ndiag = 0;
   for (int b = 0; b < P.nb; ++b) {
-    sys += (int64_t)P.r * P.dims[b] * P.r * P.dims[b];
+    const int64_t Np = ((int64_t)P.r * P.dims[b] + 15) / 16 * 16;
+    sys += Np * Np;
+    rhs += Np * P.dims[b];
+    piv += Np;
     ndiag += P.dims[b];
   }
-  return 8 * (7 * c->M.G_size + sys + 2 * P.r * ndiag + 6 * (int64_t)P.r * P.r) + 4 * P.r * ndiag;
+  return 8 * (7 * c->M.G_size + sys + rhs + P.r * ndiag + 6 * (int64_t)P.r * P.r) + 4 * piv;
 }
 
 void require_dyson(xca_ctx *c) {
@@ -72,15 +75,18 @@ extern "C" int xca_dyson_init(xca_ctx *c, const double *G0) {
     const int r = P.r, nb = P.nb;
     hipStream_t s = c->stream;
     std::vector<int> dims(P.dims);
-    std::vector<int64_t> goff(nb), gend(nb), soff(nb), poff(nb);
-    int64_t so = 0, po = 0;
+    std::vector<int64_t> goff(nb), gend(nb), soff(nb), poff(nb), roff(nb);
+    int64_t so = 0, po = 0, ro = 0;
     for (int b = 0; b < nb; ++b) {
+      const int64_t Np = ((int64_t)r * P.dims[b] + 15) / 16 * 16;
       goff[b] = c->M.G_off[b];
       gend[b] = goff[b] + (int64_t)r * P.dims[b] * P.dims[b];
       soff[b] = so;
-      so += (int64_t)r * P.dims[b] * r * P.dims[b];
+      so += Np * Np;
       poff[b] = po;
-      po += (int64_t)r * P.dims[b];
+      po += Np;
+      roff[b] = ro;
+      ro += Np * P.dims[b];
       d->ndiag += P.dims[b];
     }
     d->dims.upload(dims, s);
@@ -88,6 +94,7 @@ extern "C" int xca_dyson_init(xca_ctx *c, const double *G0) {
     d->gend.upload(gend, s);
     d->soff.upload(soff, s);
     d->poff.upload(poff, s);
+    d->roff.upload(roff, s);
     d->lu.upload(o.lu, s);
     d->piv.upload(o.piv, s);
     d->cf2it.upload(o.cf2it, s);
@@ -114,6 +121,7 @@ extern "C" int xca_dyson_init(xca_ctx *c, const double *G0) {
     d->wf.alloc(c->M.G_size);
     d->gnew.alloc(c->M.G_size);
     d->sys.alloc(so);
+    d->rhs.alloc(ro);
     d->ipiv.alloc(po);
     d->trs.alloc((size_t)r * d->ndiag);
     d->status.alloc(status_stride(nb));
