@@ -60,8 +60,8 @@ constexpr int kDysonThreads = 1024;
 // A22 -= L21 U12 on FP64 MFMA tiles. The n right-hand sides (columns of G0) are solved by blocked
 // substitution, G = (cf2it x I) g is formed and hermitised (block_sparse_solver.py:579-587).
 __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
-  __shared__ double red_v[kDysonThreads];
-  __shared__ int red_i[kDysonThreads];
+  __shared__ double red_v[kDysonThreads / 64];
+  __shared__ int red_i[kDysonThreads / 64];
   const int pidx = find_seg(Bt.prefix, Bt.nprobs, blockIdx.x);
   const DysonArgs &A = Bt.probs[pidx];
   const int b = blockIdx.x - Bt.prefix[pidx], n = A.dims[b], r = A.r, nn = n * n, N = r * n;
@@ -123,44 +123,60 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
     wf[x] = s;
   }
   __syncthreads();
-  // ---- system cf2it x I + Mc, padded with identity to Np x Np
+  // ---- system cf2it x I + Mc, padded with identity to Np x Np. The sum over k of Mc is, for each
+  // orbital pair (i, i'), the product (cf2it diag(f_{i i'})) W: 16 x 16 MFMA tiles over (t, k').
   for (int x = tid; x < Np * Np; x += NT) {
     const int row = x / Np, col = x % Np;
-    double v;
-    if (row < N && col < N) {
-      const int t = row / n, i = row % n, kp = col / n, ipp = col % n;
-      const int c = i * n + ipp;
-      double s = -A.cf2it[t * r + kp] * wf[(size_t)kp * nn + c] + A.tcf2it[t * r + kp] * ggs[(size_t)kp * nn + c];
-      for (int k = 0; k < r; ++k) s -= A.cf2it[t * r + k] * ggs[(size_t)k * nn + c] * A.W[k * r + kp];
-      v = A.beta * s + (i == ipp ? A.cf2it[t * r + kp] : 0.0);
-    } else {
-      v = (row == col) ? 1.0 : 0.0;
+    if (row >= N || col >= N) S[x] = (row == col) ? 1.0 : 0.0;
+  }
+  {
+    const int RBd = (r + 15) >> 4, nks = (r + 3) >> 2;
+    for (int q = wv; q < nn * RBd * RBd; q += NWV) {
+      const int c = q / (RBd * RBd), tile = q % (RBd * RBd), i = c / n, ipp = c % n;
+      const int t0 = 16 * (tile / RBd), kp0 = 16 * (tile % RBd);
+      const int ta = t0 + (lane & 15), kpb = kp0 + (lane & 15);
+      v4d acc = {0.0, 0.0, 0.0, 0.0};
+      for (int ks = 0; ks < nks; ++ks) {
+        const int k = 4 * ks + (lane >> 4);
+        const double a = (ta < r && k < r) ? A.cf2it[ta * r + k] * ggs[(size_t)k * nn + c] : 0.0;
+        const double bb = (k < r && kpb < r) ? A.W[k * r + kpb] : 0.0;
+        acc = mfma16x16x4(a, bb, acc);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int t = t0 + 4 * v + (lane >> 4), kp = kpb;
+        if (t < r && kp < r) {
+          const double sv = -A.cf2it[t * r + kp] * wf[(size_t)kp * nn + c] + A.tcf2it[t * r + kp] * ggs[(size_t)kp * nn + c] - acc[v];
+          S[(size_t)(t * n + i) * Np + kp * n + ipp] = A.beta * sv + (i == ipp ? A.cf2it[t * r + kp] : 0.0);
+        }
+      }
     }
-    S[x] = v;
   }
   __syncthreads();
   // ---- blocked LU with partial pivoting
   for (int k0 = 0; k0 < Np; k0 += 16) {
     for (int j = k0; j < k0 + 16; ++j) {
-      // pivot: first row of maximal |S[i][j]|, i >= j
+      // pivot: first row of maximal |S[i][j]|, i >= j (thread scan, wave shuffles, then the 16 wave
+      // results in a fixed order)
       double best = -1.0;
       int bi = Np;
       for (int i = j + tid; i < Np; i += NT) {
         const double v = fabs(S[(size_t)i * Np + j]);
         if (v > best) { best = v; bi = i; }
       }
-      red_v[tid] = best;
-      red_i[tid] = bi;
-      __syncthreads();
-      for (int h = NT / 2; h > 0; h >>= 1) {
-        if (tid < h) {
-          const double v2 = red_v[tid + h];
-          const int i2 = red_i[tid + h];
-          if (v2 > red_v[tid] || (v2 == red_v[tid] && i2 < red_i[tid])) { red_v[tid] = v2; red_i[tid] = i2; }
-        }
-        __syncthreads();
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double v2 = __shfl_xor(best, off, 64);
+        const int i2 = __shfl_xor(bi, off, 64);
+        if (v2 > best || (v2 == best && i2 < bi)) { best = v2; bi = i2; }
       }
-      const int p = red_i[0];
+      if (lane == 0) { red_v[wv] = best; red_i[wv] = bi; }
+      __syncthreads();
+      best = red_v[0];
+      bi = red_i[0];
+      for (int w = 1; w < NWV; ++w)
+        if (red_v[w] > best || (red_v[w] == best && red_i[w] < bi)) { best = red_v[w]; bi = red_i[w]; }
+      const int p = bi;
       if (tid == 0) ip[j] = p;
       if (p != j)
         for (int c = tid; c < Np; c += NT) {

@@ -286,6 +286,21 @@ __global__ void __launch_bounds__(64 * NW) tab_kernel(TabBatch B) {
   }
 }
 
+// Wave-uniform copy of a descriptor read from global memory: the compiler treats values loaded
+// from writable global memory as per-lane, which would turn every address derived from them into
+// vector arithmetic; every 32-bit word is moved to a scalar register instead.
+template <class T>
+__device__ __forceinline__ T uniform_load(const T *p) {
+  static_assert(sizeof(T) % 4 == 0, "descriptor size");
+  T v = *p;
+  unsigned w[sizeof(T) / 4];
+  __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (unsigned i = 0; i < sizeof(T) / 4; ++i) w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+  __builtin_memcpy(&v, w, sizeof(T));
+  return v;
+}
+
 // -------------------------------------------------------------------------------------------
 // Chain kernel: persistent workgroups drain a queue of chunks (runs of tiles of one program); each
 // chunk writes its own partial slot, so the result does not depend on which workgroup took it.
@@ -311,10 +326,10 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
   __syncthreads();
   if (tid == 0) qnext = atomicAdd(B.queue, 1);
   __syncthreads();
-  const int gchunk = qnext;
+  const int gchunk = __builtin_amdgcn_readfirstlane(qnext);  // wave-uniform: descriptors load through the scalar cache
   if (gchunk >= B.total) break;
-  const int sidx = find_segment(B.prefix, B.nsegs, gchunk);
-  const ChainArgs A = B.segs[sidx];
+  const int sidx = __builtin_amdgcn_readfirstlane(find_segment(B.prefix, B.nsegs, gchunk));
+  const ChainArgs A = uniform_load(B.segs + sidx);
   const DlrFrag &D = A.D;
   const int r = A.r;
   const int nk = (r + 3) >> 2;
@@ -325,8 +340,8 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
     cur = D.LiP;
     __syncthreads();
   }
-  const DChunk ch = A.chunks[gchunk - B.prefix[sidx]];
-  const DProg P = A.progs[ch.prog];
+  const DChunk ch = uniform_load(A.chunks + __builtin_amdgcn_readfirstlane(gchunk - B.prefix[sidx]));
+  const DProg P = uniform_load(A.progs + ch.prog);
   const int64_t slot = ch.slot;
   bool first_tile = true;
   for (int ti = ch.t0; ti < ch.t1; ++ti) {
@@ -366,23 +381,27 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
     double *Scur = S0, *Soth = S1;
     int rows = P.b1, cols = P.b1;
     for (int oi = 0; oi < P.nops; ++oi) {
-      const DOp op = A.ops[P.op0 + oi];
+      const DOp op = uniform_load(A.ops + P.op0 + oi);
       if (op.type == 0) {  // vertex: T' = K F T
         const int bin = op.bin, bout = op.bout, win = bin * cols, wout = bout * cols;
-        for (int x = tid; x < r * P_items * cols; x += NT) {
-          const int t = x / (P_items * cols), rem = x % (P_items * cols), it = rem / cols, j = rem % cols;
-          double h[kMaxB];
-#pragma unroll
-          for (int k = 0; k < kMaxB; ++k) h[k] = (k < bin) ? Scur[(size_t)t * LDN + it * win + k * cols + j] : 0.0;
+        // one output row (t, item, i) per thread: the operator row is loaded once and applied to
+        // every column; lanes of one (t, item) read the same T entries (LDS broadcast)
+        for (int x = tid; x < r * P_items * bout; x += NT) {
+          const int t = x / (P_items * bout), rem = x % (P_items * bout), it = rem / bout, i = rem % bout;
           const double *F = A.opmats + op.fbase + (op.bar_line >= 0 ? (int64_t)ipole[it * kMaxLines + op.bar_line] * op.fstride_l : 0);
+          double fr[kMaxB];
+#pragma unroll
+          for (int k = 0; k < kMaxB; ++k) fr[k] = (k < bin) ? F[i * bin + k] : 0.0;
           double kf = 1.0;
           if (op.ks != 0) kf = (op.ks > 0 ? A.kp : A.km)[(size_t)ipole[it * kMaxLines + op.kx] * r + t];
-          for (int i = 0; i < bout; ++i) {
+          const double *Tin = Scur + (size_t)t * LDN + it * win;
+          double *Tout = Soth + (size_t)t * LDN + it * wout + i * cols;
+          for (int j = 0; j < cols; ++j) {
             double s = 0.0;
 #pragma unroll
             for (int k = 0; k < kMaxB; ++k)
-              if (k < bin) s += F[i * bin + k] * h[k];
-            Soth[(size_t)t * LDN + it * wout + i * cols + j] = kf * s;
+              if (k < bin) s += fr[k] * Tin[k * cols + j];
+            Tout[j] = kf * s;
           }
         }
         rows = bout;
@@ -416,35 +435,62 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
           store_rows<NCB>(acc, Scur, LDN, rbw, cbw, ncw, lane, r);
         }
         __syncthreads();
-        // c = -(Wf o g + f o Wg) -> Soth, d = f o g -> Scur
-        for (int x = tid; x < r * P_items * cols; x += NT) {
-          const int t = x / (P_items * cols), rem = x % (P_items * cols), it = rem / cols, j = rem % cols;
-          double g[kMaxB], wg[kMaxB];
+        // c = -(Wf o g + f o Wg) -> Soth, d = f o g -> Scur, in place. One output row (t, item, i)
+        // per thread with the table rows f_i, (W f)_i in registers; rounds hold whole (t, item)
+        // groups and 4 columns, whose reads all happen before any of them is overwritten.
+        {
+          const int ngroups = r * P_items, gpr = NT / b;
+          const int gl = tid / b, i = tid % b;
+          for (int g0 = 0; g0 < ngroups; g0 += gpr) {
+            const int gi = g0 + gl;
+            const bool act = gl < gpr && gi < ngroups;
+            const int t = act ? gi / P_items : 0, it = act ? gi % P_items : 0;
+            double fr[kMaxB], wfr[kMaxB];
+            if (act) {
+              int64_t e = op.ebase;
 #pragma unroll
-          for (int k = 0; k < kMaxB; ++k) {
-            g[k] = (k < b) ? Soth[(size_t)t * LDN + it * w + k * cols + j] : 0.0;
-            wg[k] = (k < b) ? Scur[(size_t)t * LDN + it * w + k * cols + j] : 0.0;
-          }
-          int64_t e = op.ebase;
+              for (int q = 0; q < kMaxLines; ++q)
+                if (q < op.nlines) e += (int64_t)ipole[it * kMaxLines + op.lines[q]] * op.estride[q];
+              const double *f = A.ftab + e + (size_t)t * b * b + i * b;
+              const double *wf = f + (size_t)r * b * b;
 #pragma unroll
-          for (int q = 0; q < kMaxLines; ++q)
-            if (q < op.nlines) e += (int64_t)ipole[it * kMaxLines + op.lines[q]] * op.estride[q];
-          const double *f = A.ftab + e + (size_t)t * b * b;
-          const double *wf = f + (size_t)r * b * b;
-          for (int i = 0; i < b; ++i) {
-            double s1 = 0.0, s2 = 0.0;
-#pragma unroll
-            for (int k = 0; k < kMaxB; ++k)
-              if (k < b) {
-                const double fv = f[i * b + k];
-                s1 += wf[i * b + k] * g[k] + fv * wg[k];
-                s2 += fv * g[k];
+              for (int k = 0; k < kMaxB; ++k) {
+                fr[k] = (k < b) ? f[k] : 0.0;
+                wfr[k] = (k < b) ? wf[k] : 0.0;
               }
-            Soth[(size_t)t * LDN + it * w + i * cols + j] = -s1;
-            Scur[(size_t)t * LDN + it * w + i * cols + j] = s2;
+            }
+            double *gcol = Soth + (size_t)t * LDN + it * w, *wgcol = Scur + (size_t)t * LDN + it * w;
+            for (int j0 = 0; j0 < cols; j0 += 4) {
+              double cj[4], dj[4];
+#pragma unroll
+              for (int jj = 0; jj < 4; ++jj) {
+                double s1 = 0.0, s2 = 0.0;
+                const int j = j0 + jj;
+                if (act && j < cols) {
+#pragma unroll
+                  for (int k = 0; k < kMaxB; ++k)
+                    if (k < b) {
+                      const double g = gcol[k * cols + j], wg = wgcol[k * cols + j];
+                      s1 += wfr[k] * g + fr[k] * wg;
+                      s2 += fr[k] * g;
+                    }
+                }
+                cj[jj] = -s1;
+                dj[jj] = s2;
+              }
+              __syncthreads();
+              if (act) {
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj)
+                  if (j0 + jj < cols) {
+                    gcol[i * cols + j0 + jj] = cj[jj];
+                    wgcol[i * cols + j0 + jj] = dj[jj];
+                  }
+              }
+              __syncthreads();
+            }
           }
         }
-        __syncthreads();
         // h = beta (cf2it c + tcf2it d) -> Soth, stored after every wave has read both slabs
         {
           v4d acc[NCB];
@@ -461,36 +507,36 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
         const int bv = op.bin, bvp = op.bout, b1 = op.b1, b0 = op.cols, qs = op.qs, win = bv * b1, wout = bvp * b0;
         const int n = A.n;
         const int *orbs = A.ops[P.op0 + oi].orb;
+        // out(t)[i][j] = sum_mu sum_a Fm_mu[i][a] sum_c T(t)[a][c] Fh_mu(t)[c][j] with
+        // Fh_mu(t) = sum_kk Delta(t)[mu][kk] F_kk: the kappa sum folded into one column of F per mu
+        // (registers), the mu operators read with wave-uniform addresses
         for (int x = tid; x < r * P_items * b0; x += NT) {
           const int t = x / (P_items * b0), rem = x % (P_items * b0), it = rem / b0, j = rem % b0;
+          const double *Tt = Scur + (size_t)t * LDN + it * win;
           double out[kMaxB];
 #pragma unroll
           for (int i = 0; i < kMaxB; ++i) out[i] = 0.0;
-          for (int a = 0; a < bv; ++a) {
-            double tk[kMaxSet];
+          for (int mu = 0; mu < qs; ++mu) {
+            double fh[kMaxB];
 #pragma unroll
-            for (int kk = 0; kk < kMaxSet; ++kk) {
-              tk[kk] = 0.0;
-              if (kk < qs) {
-                const double *Fk = A.opmats + op.fbase + (int64_t)kk * b1 * b0;
-                double s = 0.0;
-                for (int c = 0; c < b1; ++c) s += Scur[(size_t)t * LDN + it * win + a * b1 + c] * Fk[c * b0 + j];
-                tk[kk] = s;
-              }
+            for (int c = 0; c < kMaxB; ++c) fh[c] = 0.0;
+            for (int kk = 0; kk < qs; ++kk) {
+              const double hv = op.forward ? A.delta[((size_t)t * n + orbs[mu]) * n + orbs[kk]]
+                                           : -A.delta_refl[((size_t)t * n + orbs[kk]) * n + orbs[mu]];
+              const double *Fk = A.opmats + op.fbase + (int64_t)kk * b1 * b0 + j;
+#pragma unroll
+              for (int c = 0; c < kMaxB; ++c)
+                if (c < b1) fh[c] += hv * Fk[c * b0];
             }
-            for (int mu = 0; mu < qs; ++mu) {
-              double tm = 0.0;
+            const double *Fm = A.opmats + op.mubase + (int64_t)mu * bvp * bv;
+            for (int a = 0; a < bv; ++a) {
+              double y = 0.0;
 #pragma unroll
-              for (int kk = 0; kk < kMaxSet; ++kk)
-                if (kk < qs) {
-                  const double hv = op.forward ? A.delta[((size_t)t * n + orbs[mu]) * n + orbs[kk]]
-                                               : -A.delta_refl[((size_t)t * n + orbs[kk]) * n + orbs[mu]];
-                  tm += hv * tk[kk];
-                }
-              const double *Fm = A.opmats + op.mubase + (int64_t)mu * bvp * bv;
+              for (int c = 0; c < kMaxB; ++c)
+                if (c < b1) y += Tt[a * b1 + c] * fh[c];
 #pragma unroll
               for (int i = 0; i < kMaxB; ++i)
-                if (i < bvp) out[i] += Fm[i * bv + a] * tm;
+                if (i < bvp) out[i] += Fm[i * bv + a] * y;
             }
           }
 #pragma unroll
