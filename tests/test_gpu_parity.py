@@ -104,6 +104,25 @@ def test_every_tile_variant(variant, case, order):
                     assert used[0] in (v, 0)
 
 
+@pytest.mark.parametrize("chunk_tiles", ["1", "16"])
+def test_shared_prefix_third_order(chunk_tiles, monkeypatch):
+    """Third order with items that share the first line's pole: the ops before the second line
+    enters run once per tile (or are read back from the workgroup's cache when consecutive tiles of
+    a chunk share the pole). Long chunks exercise the cache."""
+    monkeypatch.setenv("XCA_CHUNK_TILES", chunk_tiles)
+    P = models.two_band_discrete_bath()
+    ev = _evaluator(P)
+    orc = po.OracleEvaluator(P, complex_arith=False)
+    for v in (0, 2):
+        ev.set_tile_variant(v)
+        for sign, topo in all_connected_pairings(3):
+            t = np.array(topo, dtype=np.int32)
+            e = rel_err(ev.compute_self_energy(P.G0, t), [x.real for x in orc.sigma(P.G0, t)])
+            tol = max(TOL, 4 * oracle_noise_floor(orc, P.G0, t, None))
+            print(f"[gpu] shared prefix chunk {chunk_tiles} variant {v} topo={topo} rel_err={e:.2e} tol={tol:.1e}", flush=True)
+            assert e <= tol
+
+
 def test_flat_index_subset_matches_oracle():
     """compute_self_energy(G, topology, f_ix_vec) on a scattered subset (the MPI-chunk contract)."""
     P = models.oca_2orb()

@@ -286,7 +286,9 @@ class Batch {
         queues_.push_back(std::make_unique<DevBuf<int>>());
         queues_.back()->alloc(1);
       }
-      g->B = ChainBatch{g->segs.ptr, g->prefix.ptr, (int)g->h_segs.size(), g->h_prefix.back(), queues_[qi]->ptr};
+      const int64_t stride = 16LL * g->RB * tile_columns(g->variant);
+      g->scratch.alloc((size_t)chain_resident(g->RB, g->variant) * stride);
+      g->B = ChainBatch{g->segs.ptr, g->prefix.ptr, (int)g->h_segs.size(), g->h_prefix.back(), queues_[qi]->ptr, g->scratch.ptr, stride};
       ++qi;
       chain_groups_.push_back(std::move(g));
     }
@@ -336,6 +338,7 @@ class Batch {
     std::vector<int> h_prefix;
     DevBuf<ChainArgs> segs;
     DevBuf<int> prefix;
+    DevBuf<double> scratch;
     ChainBatch B{};
   };
 

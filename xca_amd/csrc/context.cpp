@@ -23,7 +23,14 @@
 namespace xca {
 
 // tiles per chunk: enough chunks to cover the GPU several times, at most 64 tiles per chunk
-static int chunk_tiles(int64_t total_tiles) { return (int)std::max<int64_t>(1, std::min<int64_t>(64, total_tiles / 2048)); }
+// (XCA_CHUNK_TILES overrides: longer chunks reuse a shared prefix across more tiles)
+static int chunk_tiles(int64_t total_tiles) {
+  if (const char *e = std::getenv("XCA_CHUNK_TILES")) {
+    const int v = std::atoi(e);
+    if (v > 0) return v;
+  }
+  return (int)std::max<int64_t>(1, std::min<int64_t>(64, total_tiles / 2048));
+}
 
 }  // namespace xca
 
@@ -277,6 +284,25 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
       ntup = (int64_t)sp.tuples.size();
     }
     d.ntup = (int)ntup;
+    // ops before the first one that involves a line other than the first depend on the first
+    // line's pole only; items sharing that pole share them (not for explicit subsets)
+    d.split = 0;
+    d.share_len = 1;
+    if (nl >= 2 && !fix) {
+      int split = (int)pg.ops.size();
+      for (size_t i = 0; i < pg.ops.size() && split == (int)pg.ops.size(); ++i) {
+        const auto &op = pg.ops[i];
+        bool other = false;
+        if (op.type == OP_V) other = op.bar_line >= 1 || (op.ks != 0 && op.kx >= 1);
+        else if (op.type == OP_E)
+          for (int q = 0; q < op.nlines; ++q) other = other || op.lines[q] >= 1;
+        if (other) split = (int)i;
+      }
+      if (split > 0 && split < (int)pg.ops.size()) {
+        d.split = split;
+        d.share_len = (int)(ntup / d.set_len[0]);
+      }
+    }
     for (auto &op : pg.ops) {
       DOp o{};
       o.type = op.type; o.bin = op.bin; o.bout = op.bout; o.cols = op.cols;
@@ -451,7 +477,10 @@ void autotune(xca_ctx *c, DevPlan *dp) {
     DevBuf<int> dpre;
     dA.upload(A, s);
     dpre.upload(pre, s);
-    const ChainBatch B{dA.ptr, dpre.ptr, 1, A[0].nchunks, dp->queue.ptr};
+    DevBuf<double> scr;
+    const int64_t sstride = (int64_t)c->RP * tile_columns(v);
+    scr.alloc((size_t)chain_resident(c->RB, v) * sstride);
+    const ChainBatch B{dA.ptr, dpre.ptr, 1, A[0].nchunks, dp->queue.ptr, scr.ptr, sstride};
     float best_ms = 1e30f;
     for (int rep = 0; rep < 2; ++rep) {
       HIP_CHECK(hipEventRecord(c->ev0, s));
@@ -515,7 +544,9 @@ void accumulate(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t 
     V->seg_prefix.upload(V->h_seg_prefix, s);
   }
   HIP_CHECK(hipEventRecord(c->ev0, s));
-  HIP_CHECK(launch_chain(c->RB, dp->chosen, ChainBatch{V->seg.ptr, V->seg_prefix.ptr, 1, V->nchunks, dp->queue.ptr}, s));
+  const int64_t stride = (int64_t)c->RP * V->NT;
+  if (!V->scratch.ptr) V->scratch.alloc((size_t)chain_resident(c->RB, dp->chosen) * stride);
+  HIP_CHECK(launch_chain(c->RB, dp->chosen, ChainBatch{V->seg.ptr, V->seg_prefix.ptr, 1, V->nchunks, dp->queue.ptr, V->scratch.ptr, stride}, s));
   HIP_CHECK(hipEventRecord(c->ev1, s));
   int maxsz = 0;
   for (int b = 0; b < c->M.P.nb; ++b) maxsz = std::max(maxsz, c->M.P.r * c->M.P.dims[b] * c->M.P.dims[b]);

@@ -322,6 +322,8 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
   const int rbw = wv % RB, grp = wv / RB;  // this wave's row block and column group
   constexpr int NGRP = NW / RB;              // column groups
   const double *cur = nullptr;               // DLR fragments of the problem whose U sits in LDS
+  double *cache = B.scratch + (size_t)blockIdx.x * B.scratch_stride;  // this workgroup's shared-prefix T
+  int cache_seg = -1, cache_prog = -1, cache_key = -1, cache_rows = 0, cache_cols = 0;
   for (;;) {
   __syncthreads();
   if (tid == 0) qnext = atomicAdd(B.queue, 1);
@@ -347,6 +349,12 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
   for (int ti = ch.t0; ti < ch.t1; ++ti) {
     const int tstart = ti * P.items_per_tile;
     const int P_items = min(P.items_per_tile, P.ntup - tstart);
+    // Items whose first-line pole agrees share the ops [0, split): those run on item 0 only, or
+    // are read back from this workgroup's cache of the last shared prefix, and the result is
+    // copied to the other items. Pole tuples run with the first line slowest.
+    const int pkey = P.split > 0 ? tstart / P.share_len : -1;
+    const bool shared = P.split > 0 && pkey == (tstart + P_items - 1) / P.share_len;
+    const bool hit = shared && cache_seg == sidx && cache_prog == ch.prog && cache_key == pkey;
     __syncthreads();
     // pole tuple of each item, prefactor
     for (int it = tid; it < P_items; it += NT) {
@@ -354,7 +362,7 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
       double pref = 1.0;
       int ls[kMaxLines];
 #pragma unroll
-      for (int x = 0; x < kMaxLines; ++x) {
+      for (int x = kMaxLines - 1; x >= 0; --x) {
         ls[x] = 0;
         if (x < P.nlines) {
           ls[x] = A.plist[P.set_off[x] + u % P.set_len[x]];
@@ -369,25 +377,48 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
       ipref[it] = pref * (double)P.sign;
       imax[it] = 0ull;
     }
-    // T = G_b1 for every item
-    {
+    int nit = shared ? 1 : P_items;  // items the next ops run on
+    int rows = P.b1, cols = P.b1, oi0 = 0;
+    if (hit) {
+      rows = cache_rows;
+      cols = cache_cols;
+      oi0 = P.split;
+      const int w = rows * cols;
+      for (int x = tid; x < r * w; x += NT) S0[(size_t)(x / w) * LDN + x % w] = cache[x];
+    } else {  // T = G_b1
       const int b1 = P.b1, w = b1 * b1;
-      for (int x = tid; x < r * P_items * w; x += NT) {
-        const int t = x / (P_items * w), c = x % (P_items * w);
+      for (int x = tid; x < r * nit * w; x += NT) {
+        const int t = x / (nit * w), c = x % (nit * w);
         S0[(size_t)t * LDN + c] = A.G[P.g_init + (size_t)t * w + c % w];
       }
     }
     __syncthreads();
     double *Scur = S0, *Soth = S1;
-    int rows = P.b1, cols = P.b1;
-    for (int oi = 0; oi < P.nops; ++oi) {
+    for (int oi = oi0; oi < P.nops; ++oi) {
+      if (shared && oi == P.split) {
+        const int w = rows * cols;
+        if (!hit) {
+          for (int x = tid; x < r * w; x += NT) cache[x] = Scur[(size_t)(x / w) * LDN + x % w];
+          cache_seg = sidx;
+          cache_prog = ch.prog;
+          cache_key = pkey;
+          cache_rows = rows;
+          cache_cols = cols;
+        }
+        for (int x = tid; x < r * (P_items - 1) * w; x += NT) {
+          const int t = x / ((P_items - 1) * w), c = x % ((P_items - 1) * w);
+          Scur[(size_t)t * LDN + w + c] = Scur[(size_t)t * LDN + c % w];
+        }
+        nit = P_items;
+        __syncthreads();
+      }
       const DOp op = uniform_load(A.ops + P.op0 + oi);
       if (op.type == 0) {  // vertex: T' = K F T
         const int bin = op.bin, bout = op.bout, win = bin * cols, wout = bout * cols;
         // one output row (t, item, i) per thread: the operator row is loaded once and applied to
         // every column; lanes of one (t, item) read the same T entries (LDS broadcast)
-        for (int x = tid; x < r * P_items * bout; x += NT) {
-          const int t = x / (P_items * bout), rem = x % (P_items * bout), it = rem / bout, i = rem % bout;
+        for (int x = tid; x < r * nit * bout; x += NT) {
+          const int t = x / (nit * bout), rem = x % (nit * bout), it = rem / bout, i = rem % bout;
           const double *F = A.opmats + op.fbase + (op.bar_line >= 0 ? (int64_t)ipole[it * kMaxLines + op.bar_line] * op.fstride_l : 0);
           double fr[kMaxB];
 #pragma unroll
@@ -411,7 +442,7 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
         const int b = op.bin, w = b * cols;
         // only the column blocks this tile's items occupy: group g of the waves takes blocks
         // [g cpg, min((g + 1) cpg, ncb)) of every row block
-        const int ncb = (P_items * w + 15) >> 4;
+        const int ncb = (nit * w + 15) >> 4;
         const int cpg = (ncb + NGRP - 1) / NGRP;
         const int cbw = grp * cpg, ncw = min(cpg, ncb - cbw);
         {  // y = (L^{-1} P) T -> Soth
@@ -439,12 +470,12 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
         // per thread with the table rows f_i, (W f)_i in registers; rounds hold whole (t, item)
         // groups and 4 columns, whose reads all happen before any of them is overwritten.
         {
-          const int ngroups = r * P_items, gpr = NT / b;
+          const int ngroups = r * nit, gpr = NT / b;
           const int gl = tid / b, i = tid % b;
           for (int g0 = 0; g0 < ngroups; g0 += gpr) {
             const int gi = g0 + gl;
             const bool act = gl < gpr && gi < ngroups;
-            const int t = act ? gi / P_items : 0, it = act ? gi % P_items : 0;
+            const int t = act ? gi / nit : 0, it = act ? gi % nit : 0;
             double fr[kMaxB], wfr[kMaxB];
             if (act) {
               int64_t e = op.ebase;
@@ -510,8 +541,8 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
         // out(t)[i][j] = sum_mu sum_a Fm_mu[i][a] sum_c T(t)[a][c] Fh_mu(t)[c][j] with
         // Fh_mu(t) = sum_kk Delta(t)[mu][kk] F_kk: the kappa sum folded into one column of F per mu
         // (registers), the mu operators read with wave-uniform addresses
-        for (int x = tid; x < r * P_items * b0; x += NT) {
-          const int t = x / (P_items * b0), rem = x % (P_items * b0), it = rem / b0, j = rem % b0;
+        for (int x = tid; x < r * nit * b0; x += NT) {
+          const int t = x / (nit * b0), rem = x % (nit * b0), it = rem / b0, j = rem % b0;
           const double *Tt = Scur + (size_t)t * LDN + it * win;
           double out[kMaxB];
 #pragma unroll
@@ -655,10 +686,17 @@ static hipError_t launch_tab_t(const TabBatch &B, hipStream_t s) {
   return hipGetLastError();
 }
 
-// persistent grid: as many workgroups as fit on the device at once, at most one per chunk
+// workgroups of this instance resident on the device at once (set on first use, with the LDS
+// attribute, outside any stream capture)
 template <int RB, int NW, int NTILE>
-static hipError_t launch_chain_t(const ChainBatch &B, hipStream_t s) {
-  static int resident = 0;  // workgroups resident on the device for this instance (set on first use)
+static int &resident_of() {
+  static int resident = 0;
+  return resident;
+}
+
+template <int RB, int NW, int NTILE>
+static hipError_t init_chain_t() {
+  int &resident = resident_of<RB, NW, NTILE>();
   const size_t sh = lds_bytes<RB, NW, NTILE>();
   const void *fn = (const void *)chain_kernel<RB, NW, NTILE>;
   if (resident == 0) {
@@ -670,6 +708,16 @@ static hipError_t launch_chain_t(const ChainBatch &B, hipStream_t s) {
     if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 64 * NW, sh)) != hipSuccess) return e;
     resident = std::max(1, ncu * std::max(per, 1));
   }
+  return hipSuccess;
+}
+
+// persistent grid: as many workgroups as fit on the device at once, at most one per chunk
+template <int RB, int NW, int NTILE>
+static hipError_t launch_chain_t(const ChainBatch &B, hipStream_t s) {
+  hipError_t e = init_chain_t<RB, NW, NTILE>();
+  if (e != hipSuccess) return e;
+  const int resident = resident_of<RB, NW, NTILE>();
+  const size_t sh = lds_bytes<RB, NW, NTILE>();
   if (B.total <= 0) return hipSuccess;
   hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, s, B.queue);
   const int grid = std::min(B.total, resident);
@@ -712,6 +760,25 @@ hipError_t launch_chain(int RB, int variant, const ChainBatch &B, hipStream_t s)
     }
   }
   return hipErrorInvalidValue;
+}
+
+int chain_resident(int RB, int variant) {
+  hipError_t e = hipErrorInvalidValue;
+  int res = 0;
+  if (RB == 2) {
+    switch (variant) {
+      case 0: e = init_chain_t<2, 8, 128>(); res = resident_of<2, 8, 128>(); break;
+      case 1: e = init_chain_t<2, 4, 64>(); res = resident_of<2, 4, 64>(); break;
+      case 2: e = init_chain_t<2, 4, 32>(); res = resident_of<2, 4, 32>(); break;
+    }
+  } else if (RB == 4) {
+    switch (variant) {
+      case 0: e = init_chain_t<4, 8, 128>(); res = resident_of<4, 8, 128>(); break;
+      case 1: e = init_chain_t<4, 4, 64>(); res = resident_of<4, 4, 64>(); break;
+      case 2: e = init_chain_t<4, 4, 32>(); res = resident_of<4, 4, 32>(); break;
+    }
+  }
+  return e == hipSuccess ? res : 0;
 }
 
 hipError_t launch_reduce(const int64_t *slot, const int *red_chunks, const I2 *red_range, const int64_t *sig_off, const int *sig_size,

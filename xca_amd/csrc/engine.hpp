@@ -27,7 +27,10 @@ struct DProg {
   int pk_sign[kMaxLines], pk_exp[kMaxLines];
   int set_off[kMaxLines], set_len[kMaxLines];  // pole lists per line (into the pole-list array)
   int ntup;                                     // pole tuples (items) of the program
-  int items_per_tile, max_w, pad0;
+  int items_per_tile, max_w;
+  int split;      // ops [0, split) depend on the first line's pole only (0: nothing shared)
+  int share_len;  // consecutive items with the same first-line pole
+  int pad0;
 };
 
 // A run of consecutive tiles [t0, t1) of one program; tile t holds items
@@ -81,6 +84,8 @@ struct ChainBatch {
   const int *prefix;  // (nsegs + 1)
   int nsegs, total;
   int *queue;         // chunk counter (reset before each launch)
+  double *scratch;    // per-workgroup cache of a shared prefix, scratch_stride doubles each
+  int64_t scratch_stride;
 };
 
 // Edge-table work of one plan.

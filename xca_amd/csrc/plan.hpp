@@ -14,6 +14,8 @@ int num_tile_variants();
 int tile_columns(int variant);
 hipError_t launch_tab(int RB, const TabBatch &B, hipStream_t s);
 hipError_t launch_chain(int RB, int variant, const ChainBatch &B, hipStream_t s);
+// workgroups of one chain launch resident on the device at once (its grid never exceeds this)
+int chain_resident(int RB, int variant);
 hipError_t launch_fill(int64_t n, double v, double *p, hipStream_t s);
 hipError_t launch_reduce(const int64_t *slot, const int *red_chunks, const I2 *red_range, const int64_t *sig_off, const int *sig_size,
                          int nblocks, int max_size, const double *partial, double *sigma, double scale, hipStream_t s);
@@ -33,6 +35,7 @@ struct DevVariant {
   DevBuf<int> red_chunks;
   DevBuf<I2> red_range;
   DevBuf<double> partial;
+  DevBuf<double> scratch;  // per-workgroup shared-prefix caches
   int nchunks = 0;
   double ms = -1.0;  // autotuner estimate of the launch time
   std::vector<DProg> h_progs;
