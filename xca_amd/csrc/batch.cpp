@@ -255,7 +255,9 @@ class Batch {
     dyson_.upload(h_dyson_, s);
     h_dys_prefix_ = dys_prefix;
     dys_prefix_.upload(h_dys_prefix_, s);
-    dys_ = DysonBatch{dyson_.ptr, dys_prefix_.ptr, (int)items_.size(), dys_prefix.back()};
+    int np_max = 16;
+    for (BatchItem *it : items_) np_max = std::max(np_max, dyson_np_max(it->c));
+    dys_ = DysonBatch{dyson_.ptr, dys_prefix_.ptr, (int)items_.size(), dys_prefix.back(), np_max};
     iter_.upload(h_iter_, s);
     iter_b_ = IterBatch{iter_.ptr, dys_prefix_.ptr, (int)items_.size(), dys_prefix.back(), max_total};
     // tab groups

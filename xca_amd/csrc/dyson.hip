@@ -45,6 +45,7 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ v4d mfma16x16x4(double a, double b, v4d c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
 constexpr int kDysonThreads = 1024;
+constexpr int kDysonMaxNp = 784;  // padded r n: DLR rank <= 64, blocks <= 12
 
 // Dyson solve of one block per workgroup (1024 threads), in the coefficient form of
 // dyson_it_ppsc::solve (dlr_dyson_ppsc.hpp:135-168):
@@ -62,6 +63,7 @@ constexpr int kDysonThreads = 1024;
 __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
   __shared__ double red_v[kDysonThreads / 64];
   __shared__ int red_i[kDysonThreads / 64];
+  extern __shared__ double Pn[];  // LU panel: (Np - k0) rows x 16 columns
   const int pidx = find_seg(Bt.prefix, Bt.nprobs, blockIdx.x);
   const DysonArgs &A = Bt.probs[pidx];
   const int b = blockIdx.x - Bt.prefix[pidx], n = A.dims[b], r = A.r, nn = n * n, N = r * n;
@@ -153,15 +155,20 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
     }
   }
   __syncthreads();
-  // ---- blocked LU with partial pivoting
+  // ---- blocked LU with partial pivoting. Each 16-column panel is factorised in LDS (rows k0..Np-1),
+  // written back, its row interchanges applied to the other columns together with U12 = L11^{-1} A12
+  // (one column per thread), then A22 -= L21 U12 on MFMA tiles with L21 read from LDS.
   for (int k0 = 0; k0 < Np; k0 += 16) {
-    for (int j = k0; j < k0 + 16; ++j) {
-      // pivot: first row of maximal |S[i][j]|, i >= j (thread scan, wave shuffles, then the 16 wave
+    const int m = Np - k0;
+    for (int x = tid; x < m * 16; x += NT) Pn[x] = S[(size_t)(k0 + (x >> 4)) * Np + k0 + (x & 15)];
+    __syncthreads();
+    for (int jj = 0; jj < 16; ++jj) {
+      // pivot: first row of maximal |P[i][jj]|, i >= jj (thread scan, wave shuffles, then the wave
       // results in a fixed order)
       double best = -1.0;
       int bi = Np;
-      for (int i = j + tid; i < Np; i += NT) {
-        const double v = fabs(S[(size_t)i * Np + j]);
+      for (int i = jj + tid; i < m; i += NT) {
+        const double v = fabs(Pn[i * 16 + jj]);
         if (v > best) { best = v; bi = i; }
       }
 #pragma unroll
@@ -177,43 +184,53 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
       for (int w = 1; w < NWV; ++w)
         if (red_v[w] > best || (red_v[w] == best && red_i[w] < bi)) { best = red_v[w]; bi = red_i[w]; }
       const int p = bi;
-      if (tid == 0) ip[j] = p;
-      if (p != j)
-        for (int c = tid; c < Np; c += NT) {
-          const double a = S[(size_t)j * Np + c];
-          S[(size_t)j * Np + c] = S[(size_t)p * Np + c];
-          S[(size_t)p * Np + c] = a;
-        }
+      if (tid == 0) ip[k0 + jj] = k0 + p;
+      if (p != jj && tid < 16) {
+        const double a = Pn[jj * 16 + tid];
+        Pn[jj * 16 + tid] = Pn[p * 16 + tid];
+        Pn[p * 16 + tid] = a;
+      }
       __syncthreads();
-      const double pv = S[(size_t)j * Np + j];
-      // scale the column and update the rest of the panel (columns j+1 .. k0+15)
-      const int pw = k0 + 15 - j;  // panel columns right of j
-      for (int i = j + 1 + tid; i < Np; i += NT) {
-        const double l = S[(size_t)i * Np + j] / pv;
-        S[(size_t)i * Np + j] = l;
-        for (int c = 1; c <= pw; ++c) S[(size_t)i * Np + j + c] -= l * S[(size_t)j * Np + j + c];
+      const double pv = Pn[jj * 16 + jj];
+      for (int i = jj + 1 + tid; i < m; i += NT) {
+        const double l = Pn[i * 16 + jj] / pv;
+        Pn[i * 16 + jj] = l;
+        for (int c = jj + 1; c < 16; ++c) Pn[i * 16 + c] -= l * Pn[jj * 16 + c];
       }
       __syncthreads();
     }
+    for (int x = tid; x < m * 16; x += NT) S[(size_t)(k0 + (x >> 4)) * Np + k0 + (x & 15)] = Pn[x];
+    // the panel's row interchanges on every other column (in order, per column), and U12
     const int c1 = k0 + 16;
-    if (c1 >= Np) break;
-    // U12 = L11^{-1} A12 (unit lower 16 x 16), one column per thread
-    for (int c = c1 + tid; c < Np; c += NT) {
-      double x[16];
+    for (int c = tid; c < Np - 16; c += NT) {
+      const int col = c < k0 ? c : c + 16;
 #pragma unroll
-      for (int q = 0; q < 16; ++q) x[q] = S[(size_t)(k0 + q) * Np + c];
-#pragma unroll
-      for (int q = 1; q < 16; ++q) {
-        double s = x[q];
-#pragma unroll
-        for (int q0 = 0; q0 < q; ++q0)
-          if (q0 < q) s -= S[(size_t)(k0 + q) * Np + k0 + q0] * x[q0];
-        x[q] = s;
+      for (int q = 0; q < 16; ++q) {
+        const int pr = ip[k0 + q];
+        if (pr != k0 + q) {
+          const double a = S[(size_t)(k0 + q) * Np + col];
+          S[(size_t)(k0 + q) * Np + col] = S[(size_t)pr * Np + col];
+          S[(size_t)pr * Np + col] = a;
+        }
       }
+      if (col >= c1) {
+        double x[16];
 #pragma unroll
-      for (int q = 1; q < 16; ++q) S[(size_t)(k0 + q) * Np + c] = x[q];
+        for (int q = 0; q < 16; ++q) x[q] = S[(size_t)(k0 + q) * Np + col];
+#pragma unroll
+        for (int q = 1; q < 16; ++q) {
+          double sacc = x[q];
+#pragma unroll
+          for (int q0 = 0; q0 < 16; ++q0)
+            if (q0 < q) sacc -= Pn[q * 16 + q0] * x[q0];
+          x[q] = sacc;
+        }
+#pragma unroll
+        for (int q = 1; q < 16; ++q) S[(size_t)(k0 + q) * Np + col] = x[q];
+      }
     }
     __syncthreads();
+    if (c1 >= Np) break;
     // A22 -= L21 U12 on 16 x 16 MFMA tiles, one tile per wave at a time
     const int nt = (Np - c1) >> 4;
     for (int q = wv; q < nt * nt; q += NWV) {
@@ -223,7 +240,7 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
       for (int v = 0; v < 4; ++v) acc[v] = S[(size_t)(ti + 4 * v + (lane >> 4)) * Np + tj + (lane & 15)];
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks) {
-        const double a = -S[(size_t)(ti + (lane & 15)) * Np + k0 + 4 * ks + (lane >> 4)];
+        const double a = -Pn[(ti - k0 + (lane & 15)) * 16 + 4 * ks + (lane >> 4)];
         const double bb = S[(size_t)(k0 + 4 * ks + (lane >> 4)) * Np + tj + (lane & 15)];
         acc = mfma16x16x4(a, bb, acc);
       }
@@ -378,8 +395,16 @@ __global__ void fill_kernel(int64_t n, double v, double *p) {
 
 // host-callable launchers
 hipError_t launch_dyson(const DysonBatch &B, hipStream_t s) {
+  static bool ready = false;  // attribute set once, outside any stream capture after the first launch
+  if (!ready) {
+    hipError_t e = hipFuncSetAttribute((const void *)dyson_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)(sizeof(double) * kDysonMaxNp * 16));
+    if (e != hipSuccess) return e;
+    ready = true;
+  }
   if (B.total <= 0) return hipSuccess;
-  hipLaunchKernelGGL(dyson_kernel, dim3(B.total), dim3(kDysonThreads), 0, s, B);
+  if (B.np_max > kDysonMaxNp) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(dyson_kernel, dim3(B.total), dim3(kDysonThreads), sizeof(double) * B.np_max * 16, s, B);
   return hipGetLastError();
 }
 

@@ -31,6 +31,7 @@ struct DysonBatch {
   const DysonArgs *probs;
   const int *prefix;
   int nprobs, total;
+  int np_max;               // largest padded system dimension among the blocks (LDS panel rows)
 };
 
 // Layout of the per-problem iteration status (doubles) read back after every iteration.

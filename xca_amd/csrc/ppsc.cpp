@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -45,6 +46,12 @@ IterArgs iter_args(xca_ctx *c, double *status, double tol, double mix) {
   return A;
 }
 
+int dyson_np_max(const xca_ctx *c) {
+  int np = 16;
+  for (int b = 0; b < c->M.P.nb; ++b) np = std::max(np, (c->M.P.r * c->M.P.dims[b] + 15) / 16 * 16);
+  return np;
+}
+
 int64_t dyson_bytes(const xca_ctx *c) {
   const Problem &P = c->M.P;
   int64_t sys = 0, rhs = 0, piv = 0, ndiag = 0;
@@ -67,6 +74,7 @@ void require_dyson(xca_ctx *c) {
 extern "C" int xca_dyson_init(xca_ctx *c, const double *G0) {
   try {
     HIP_CHECK(hipSetDevice(c->device));
+    if (dyson_np_max(c) > 784) throw std::invalid_argument("xca: the Dyson solve supports r * (block dimension) <= 784");
     xca_dyson_release(c);
     auto *d = new XcaDyson();
     c->dyson = d;
@@ -155,7 +163,7 @@ extern "C" int xca_dyson_solve(xca_ctx *c, const double *sigma, double eta, doub
     DevBuf<int> dpre;
     dA.upload(A, s);
     dpre.upload(pre, s);
-    HIP_CHECK(launch_dyson(DysonBatch{dA.ptr, dpre.ptr, 1, c->M.P.nb}, s));
+    HIP_CHECK(launch_dyson(DysonBatch{dA.ptr, dpre.ptr, 1, c->M.P.nb, dyson_np_max(c)}, s));
     HIP_CHECK(hipMemcpyAsync(G_out, d->gnew.ptr, c->M.G_size * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     return 0;

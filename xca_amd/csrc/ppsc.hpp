@@ -21,6 +21,8 @@ namespace xca {
 dev::DysonArgs dyson_args(xca_ctx *c, const double *eta_dev, const double *sigma_dev);
 // Normalisation / convergence / mixing descriptor of one problem writing its status at `status`.
 dev::IterArgs iter_args(xca_ctx *c, double *status, double tol, double mix);
+// Largest padded Dyson system dimension (r n rounded up to 16) over the blocks.
+int dyson_np_max(const xca_ctx *c);
 // Device bytes xca_dyson_init allocates for this problem (without allocating).
 int64_t dyson_bytes(const xca_ctx *c);
 void require_dyson(xca_ctx *c);
