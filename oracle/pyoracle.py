@@ -52,6 +52,9 @@ def lib():
         L.orc_interp.argtypes = [C.c_int, d_p, d_p, d_p, C.c_int, d_p, C.c_int, d_p]
         L.orc_dyson.argtypes = [C.c_double, C.c_int, d_p, d_p, C.c_int, d_p, d_p, C.c_double, d_p]
         L.orc_backbone.argtypes = [i_p, C.c_int, C.c_int, d_p, C.c_int, C.c_int, i_p, i_p, i_p, i_p]
+        L.orc_num_correlator_backbones.restype = C.c_long
+        L.orc_num_correlator_backbones.argtypes = [C.c_void_p, C.c_int]
+        L.orc_correlator.argtypes = [C.c_void_p, d_p, C.c_int, i_p, i_p, C.c_long, C.c_int, i_p, d_p, C.c_int, i_p, d_p, d_p]
         _lib = L
     return _lib
 
@@ -187,6 +190,48 @@ class OracleEvaluator:
             st = lib().orc_sigma(self.h, _dp(g), topo.shape[0], _ip(topo), _ip(fx), len(fx), _dp(out))
         _chk(st)
         return prob.unpack(out)
+
+
+    def num_correlator_backbones(self, m):
+        return lib().orc_num_correlator_backbones(self.h, m)
+
+    def correlator(self, G, topology, fix=None, mu_ops=None, kap_ops=None):
+        """compute_one_time_correlator(G, topology[, f_ix_vec]) for block operators given as
+        (conn (nops, nblocks), packed matrices), or compute_single_ptcle_gf when mu_ops is None:
+        (r, nmu, nkap) complex, unsigned (the caller applies the order and topology sign)."""
+        prob = self.prob
+        g = _z(np.concatenate([np.asarray(b).ravel() for b in G]))
+        topo = np.ascontiguousarray(topology, dtype=np.int32)
+        fx = None if fix is None else np.ascontiguousarray(fix, dtype=np.int32)
+        if mu_ops is None:
+            nmu = nkap = prob.n
+            args = (0, None, None, 0, None, None)
+        else:
+            mc, mm = np.ascontiguousarray(mu_ops[0], dtype=np.int32), _z(mu_ops[1])
+            kc, km = np.ascontiguousarray(kap_ops[0], dtype=np.int32), _z(kap_ops[1])
+            self._keep += [mc, mm, kc, km]
+            nmu, nkap = mc.shape[0], kc.shape[0]
+            args = (nmu, _ip(mc), _dp(mm), nkap, _ip(kc), _dp(km))
+        out = np.zeros((prob.grid.r, nmu, nkap), dtype=np.complex128)
+        st = lib().orc_correlator(self.h, _dp(g), topo.shape[0], _ip(topo), None if fx is None else _ip(fx),
+                                  0 if fx is None else len(fx), *args, _dp(out))
+        _chk(st)
+        return out
+
+    def spgf(self, G, topology, fix=None):
+        """compute_single_ptcle_gf(G, topology[, f_ix_vec])."""
+        return self.correlator(G, topology, fix)
+
+
+def spgf_sum(orc, G, max_order):
+    """eval_single_particle_greens_function (block_sparse_solver.py:657-677): the sum over orders
+    and connected topologies of (-1)^order * s_topo * compute_single_ptcle_gf."""
+    from xca_amd.diag import all_connected_pairings
+    tot = 0
+    for order in range(1, max_order + 1):
+        for s, t in all_connected_pairings(order):
+            tot = tot + (-1) ** order * s * orc.spgf(G, np.array(t, dtype=np.int32))
+    return tot
 
 
 # ------------------------------------------------------------------ self-consistency loop

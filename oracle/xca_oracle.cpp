@@ -8,6 +8,8 @@
 //   BlockOpSymQuartet / bars      c++/triqs_xca/block_sparse.cpp:443-577
 //   BlockDiagOpFun                c++/triqs_xca/block_sparse.cpp:21-147
 //   DiagramEvaluator (sigma)      c++/triqs_xca/block_sparse_backbone.cpp:127-459
+//   CorrelatorBackbone            c++/triqs_xca/backbone.cpp:239-292
+//   DiagramEvaluator (correlator) c++/triqs_xca/block_sparse_backbone.cpp:459-706,732-902
 //   symmetry-set labelling        c++/triqs_xca/atom_diag_utils.cpp:42-68,150-185
 //   dyson_it_ppsc::solve          c++/triqs_xca/dlr_dyson_ppsc.hpp:92-168
 // The DLR operations (vals2coefs by LU, time-ordered convolution, reflection, evaluation,
@@ -231,6 +233,13 @@ template <class S> struct BlockOpSymSetBar {  // (q, p, b_out, b_in)
   int q = 0, p = 0;
 };
 
+// one block operator: target block per source block (-1: none) and its (b_out, b_in) matrices
+template <class S> struct BlockOp {
+  std::vector<int> block_indices;
+  std::vector<std::vector<S>> blocks;
+  std::vector<int> bout, bin;
+};
+
 template <class S> struct BlockOpSymQuartet {  // block_sparse.cpp:496-577
   std::vector<BlockOpSymSet<S>> Fs, F_dags;
   std::vector<BlockOpSymSetBar<S>> F_dag_bars, F_bars_refl;
@@ -274,13 +283,17 @@ struct Vertex {
   int hyb_ind = 0, Ksign = 0, orb = 0;
 };
 
+// correlator == true gives CorrelatorBackbone (backbone.cpp:239-292): no line on vertex 0, whose
+// operator is a creator, the vertex paired with it an annihilator, 2^(m-1) direction patterns.
 struct Backbone {
   int m, n, fb_ix_max, o_ix_max, prefactor_sign = 1;
+  bool correlator = false;
   std::vector<int> topo;  // (m, 2)
   std::vector<int> prefactor_Ksigns, prefactor_Kexps, edges, pole_inds, orb_inds, fb;
   std::vector<Vertex> vertices;
-  Backbone(const int *topology, int m_, int n_) : m(m_), n(n_), topo(topology, topology + 2 * m_) {
-    fb_ix_max = 1 << m;
+  Backbone(const int *topology, int m_, int n_, bool corr = false)
+      : m(m_), n(n_), correlator(corr), topo(topology, topology + 2 * m_) {
+    fb_ix_max = corr ? 1 << (m - 1) : 1 << m;
     o_ix_max = (int)std::pow(n, m - 1);
     prefactor_Ksigns.assign(m - 1, 0);
     prefactor_Kexps.assign(m - 1, 0);
@@ -301,8 +314,13 @@ struct Backbone {
     if (T(0, 0) != 0) throw std::invalid_argument("topology(0,0) must be 0");
     vertices[0].bar = false;
     vertices[T(0, 1)].bar = false;
-    vertices[0].dag = (fbv[0] != 1);
-    vertices[T(0, 1)].dag = (fbv[0] == 1);
+    if (correlator) {  // backbone.cpp:253-265; fb[0] = -1 marks the missing line
+      vertices[0].dag = true;
+      vertices[T(0, 1)].dag = false;
+    } else {
+      vertices[0].dag = (fbv[0] != 1);
+      vertices[T(0, 1)].dag = (fbv[0] == 1);
+    }
     for (int i = 1; i < m; i++) {
       vertices[T(i, 0)].bar = false;
       vertices[T(i, 1)].bar = true;
@@ -312,7 +330,12 @@ struct Backbone {
   }
   void set_directions(int fb_ix) {
     std::vector<int> v(m);
-    for (int i = 0; i < m; i++) { v[i] = fb_ix % 2; fb_ix /= 2; }
+    if (correlator) {  // backbone.cpp:241-249: bits for lines 1..m-1
+      v[0] = -1;
+      for (int i = 1; i < m; i++) { v[i] = fb_ix % 2; fb_ix /= 2; }
+    } else {
+      for (int i = 0; i < m; i++) { v[i] = fb_ix % 2; fb_ix /= 2; }
+    }
     set_directions(v);
   }
   void set_pole_inds(const std::vector<int> &pv, const std::vector<double> &poles) {
@@ -741,6 +764,230 @@ template <class S> struct DiagramEvaluator {
 
   long num_backbones(int m) const { return (long)(1 << m) * (long)std::pow(n, m - 1) * (long)std::pow(hyb.p, m - 1); }
 
+  // ------------------------------------------------------------------ correlators
+
+  // hyb.cpp:75-92 (multiply_kernels_prefactor)
+  double prefactor(const Backbone &bb) const {
+    double pref = 1.0;
+    for (int x = 0; x < bb.m - 1; x++) {
+      int e = bb.prefactor_Kexps[x];
+      if (e != 0) {
+        int l = bb.pole_inds[x];
+        double k0 = bb.prefactor_Ksigns[x] > 0 ? hyb.k_0_p[l] : hyb.k_0_m[l];
+        pref *= std::pow(k0, -e);
+      }
+    }
+    return pref;
+  }
+
+  // multiply_right_vertex (block_sparse_backbone.cpp:459-493): U (rows x b_out) <- U F, then the
+  // vertex kernel with the opposite sign (hyb.cpp:46-57 with sign -1)
+  void multiply_right_vertex(std::vector<S> &U, int rows, int &cols, Backbone &bb, int v, const std::vector<int> &ind_path,
+                             const std::vector<int> &block_dims) {
+    int o = bb.vertices[v].orb;
+    int qi = (int)Fq.sym_set_labels[o], qo = (int)Fq.sym_set_inds[o];
+    int l = bb.pole_inds[bb.vertices[v].hyb_ind];
+    int b = ind_path[v - 1];
+    int bin = block_dims[v], bout = block_dims[v + 1];
+    const S *F;
+    if (bb.vertices[v].bar) {
+      auto &B = bb.vertices[v].dag ? Fq.F_dag_bars[qi] : Fq.F_bars_refl[qi];
+      F = &B.blocks[b][((size_t)qo * B.p + l) * bout * bin];
+    } else {
+      auto &B = bb.vertices[v].dag ? Fq.F_dags[qi] : Fq.Fs[qi];
+      F = &B.blocks[b][(size_t)qo * bout * bin];
+    }
+    if (cols != bout) throw std::logic_error("right vertex shape");
+    std::vector<S> out((size_t)r * rows * bin, S(0));
+    for (int t = 0; t < r; ++t)
+      for (int i = 0; i < rows; ++i)
+        for (int j = 0; j < bin; ++j) {
+          S s(0);
+          for (int k = 0; k < bout; ++k) s += U[((size_t)t * rows + i) * bout + k] * F[k * bin + j];
+          out[((size_t)t * rows + i) * bin + j] = s;
+        }
+    int Ks = bb.vertices[v].Ksign;
+    if (Ks != 0) {
+      auto &kt = (-Ks > 0) ? hyb.k_it_p : hyb.k_it_m;
+      for (int t = 0; t < r; ++t)
+        for (int x = 0; x < rows * bin; ++x) out[(size_t)t * rows * bin + x] *= kt[(size_t)t * hyb.p + l];
+    }
+    U.swap(out);
+    cols = bin;
+  }
+
+  // integrate_right_edge (block_sparse_backbone.cpp:495-514): U <- convolve_TO(U, G_b K...)
+  void integrate_right_edge(std::vector<S> &U, int rows, int cols, const BlockDiagOpFun<S> &Gt, Backbone &bb, int e,
+                            const std::vector<int> &ind_path) {
+    int b = ind_path[e];
+    int d = Gt.dims[b];
+    std::vector<S> GK = Gt.blocks[b];
+    for (int x = 0; x < bb.m - 1; x++) {  // hyb.cpp:59-73
+      int Ks = bb.get_edge(e, x);
+      if (Ks != 0) {
+        int l = bb.pole_inds[x];
+        auto &kt = (Ks > 0) ? hyb.k_it_p : hyb.k_it_m;
+        for (int t = 0; t < r; ++t)
+          for (int y = 0; y < d * d; ++y) GK[(size_t)t * d * d + y] *= kt[(size_t)t * hyb.p + l];
+      }
+    }
+    if (d != cols) throw std::logic_error("right edge shape");
+    ops.vals2coefs(U.data(), rows * cols);
+    ops.vals2coefs(GK.data(), d * d);
+    U = ops.convolve_to(beta, U, GK, rows, d, d);
+  }
+
+  // eval_correlator for one flat index (block_sparse_backbone.cpp:526-706); adds into corr
+  // (r, nmu, nkap): sum over Sigma-free paths of Tr[left(t) O_mu right(t) O_kap], where right is
+  // the [tau, 0] side and left the reflected [beta, tau] side.
+  void eval_correlator(const BlockDiagOpFun<S> &Gt, Backbone &bb, const std::vector<BlockOp<S>> &mu_ops,
+                       const std::vector<BlockOp<S>> &kap_ops, int f_ix, std::vector<S> &corr) {
+    const int m = bb.m, vct0 = bb.T(0, 1);
+    const int nb = (int)Gt.blocks.size(), nmu = (int)mu_ops.size(), nkap = (int)kap_ops.size();
+    bb.set_flat_index(f_ix, hyb.poles);
+    std::vector<int> ind_path(2 * m, 0), block_dims(2 * m + 1, 0);
+    std::vector<std::vector<S>> right(nb), left(nb);
+    std::vector<int> right_inds(nb, -1), left_inds(nb, -1);
+    auto Fw = [&](int w) -> const BlockOpSymSet<S> & { return Fsel(w, (int)Fq.sym_set_labels[bb.orb_inds[w]], bb); };
+    auto incomplete = [&](int ip) { return ip == -1 || Gt.zero[ip] == -1; };
+    const double pref = prefactor(bb) * (double)bb.prefactor_sign;
+    // [tau, 0] side
+    for (int b_ix = 0; b_ix < nb; ++b_ix) {
+      bool inc = false;
+      int ip = b_ix;
+      ind_path[0] = ip;
+      for (int w = 1; w < vct0; w++) {
+        ip = Fw(w).get_block_index(ip);
+        inc = incomplete(ip);
+        if (inc) break;
+        ind_path[w] = ip;
+      }
+      if (inc) continue;
+      block_dims[1] = Gt.dims[b_ix];
+      for (int w = 1; w < vct0; w++) block_dims[w + 1] = Fw(w).get_block_size(ind_path[w - 1], 0);
+      std::vector<S> T = Gt.blocks[b_ix];
+      int rows = block_dims[1], cols = block_dims[1];
+      for (int v = 1; v < vct0; v++) {
+        multiply_left_vertex(T, rows, cols, bb, v, ind_path, block_dims);
+        integrate_left_edge(T, rows, cols, Gt, bb, v, ind_path);
+      }
+      for (auto &x : T) x *= pref;
+      right[b_ix] = std::move(T);  // (block_dims[vct0], block_dims[1])
+      right_inds[b_ix] = ind_path[vct0 - 1];
+    }
+    // [beta, tau] side
+    for (int b_ix = 0; b_ix < nb; ++b_ix) {
+      bool inc = false;
+      int ip = b_ix;
+      ind_path[vct0] = ip;
+      for (int w = vct0 + 1; w < 2 * m; w++) {
+        ip = Fw(w).get_block_index(ip);
+        inc = incomplete(ip);
+        if (inc) break;
+        ind_path[w] = ip;
+      }
+      if (inc) continue;
+      block_dims[vct0 + 1] = Gt.dims[b_ix];
+      for (int w = vct0 + 1; w < 2 * m; w++) block_dims[w + 1] = Fw(w).get_block_size(ind_path[w - 1], 0);
+      std::vector<S> U = Gt.blocks[ind_path[2 * m - 1]];
+      const int rows = block_dims[2 * m];
+      int cols = rows;
+      for (int v = 2 * m - 1; v > vct0; v--) {
+        multiply_right_vertex(U, rows, cols, bb, v, ind_path, block_dims);
+        integrate_right_edge(U, rows, cols, Gt, bb, v - 1, ind_path);
+      }
+      left[b_ix] = ops.reflect(U, rows * cols);  // (block_dims[2m], block_dims[vct0 + 1])
+      left_inds[b_ix] = ind_path[2 * m - 1];
+    }
+    // contraction
+    std::vector<S> c((size_t)r * nmu * nkap, S(0));
+    for (int mu = 0; mu < nmu; ++mu) {
+      const BlockOp<S> &Om = mu_ops[mu];
+      for (int b_ix = 0; b_ix < nb; ++b_ix) {
+        int e0 = right_inds[b_ix];
+        if (incomplete(e0)) continue;
+        int e1 = Om.block_indices[e0];
+        if (incomplete(e1)) continue;
+        int cx = left_inds[e1];
+        if (incomplete(cx)) continue;
+        const int db = Gt.dims[b_ix], d0 = Gt.dims[e0], d1 = Gt.dims[e1], dc = Gt.dims[cx];
+        const std::vector<S> &R = right[b_ix], &L = left[e1];
+        const S *M = Om.blocks[e0].data();  // (d1, d0)
+        // Tmuop(t) = L(t) M R(t): (dc, db)
+        std::vector<S> Tm((size_t)r * dc * db, S(0)), MR((size_t)d1 * db);
+        for (int t = 0; t < r; ++t) {
+          for (int i = 0; i < d1; ++i)
+            for (int j = 0; j < db; ++j) {
+              S s(0);
+              for (int k = 0; k < d0; ++k) s += M[i * d0 + k] * R[((size_t)t * d0 + k) * db + j];
+              MR[(size_t)i * db + j] = s;
+            }
+          for (int i = 0; i < dc; ++i)
+            for (int j = 0; j < db; ++j) {
+              S s(0);
+              for (int k = 0; k < d1; ++k) s += L[((size_t)t * dc + i) * d1 + k] * MR[(size_t)k * db + j];
+              Tm[((size_t)t * dc + i) * db + j] = s;
+            }
+        }
+        for (int kap = 0; kap < nkap; ++kap) {
+          if (kap_ops[kap].block_indices[cx] != b_ix) continue;
+          const S *K = kap_ops[kap].blocks[cx].data();  // (db, dc)
+          for (int t = 0; t < r; ++t) {
+            S tr(0);
+            for (int i = 0; i < dc; ++i)
+              for (int j = 0; j < db; ++j) tr += Tm[((size_t)t * dc + i) * db + j] * K[j * dc + i];
+            c[((size_t)t * nmu + mu) * nkap + kap] += tr;
+          }
+        }
+      }
+    }
+    for (size_t x = 0; x < c.size(); ++x) corr[x] += c[x];
+    bb.reset_all_inds();
+  }
+
+  long num_correlator_backbones(int m) const { return (long)(1 << (m - 1)) * (long)std::pow(n, m - 1) * (long)std::pow(hyb.p, m - 1); }
+
+  // setup_mu_ops / setup_kap_ops_for_single_ptcle_gf (block_sparse_backbone.cpp:737-777): one
+  // operator per orbital, in orbital order
+  std::vector<BlockOp<S>> orbital_ops(bool dag) const {
+    std::vector<BlockOp<S>> out;
+    for (int o = 0; o < n; ++o) {
+      const auto &F = dag ? Fq.F_dags[Fq.sym_set_labels[o]] : Fq.Fs[Fq.sym_set_labels[o]];
+      int i = (int)Fq.sym_set_inds[o];
+      BlockOp<S> op;
+      op.block_indices = F.block_indices;
+      op.bout = F.bout;
+      op.bin = F.bin;
+      op.blocks.resize(F.blocks.size());
+      for (size_t j = 0; j < F.blocks.size(); ++j) {
+        if (F.block_indices[j] == -1) continue;
+        size_t sz = (size_t)F.bout[j] * F.bin[j];
+        op.blocks[j].assign(F.blocks[j].begin() + i * sz, F.blocks[j].begin() + (i + 1) * sz);
+      }
+      out.push_back(std::move(op));
+    }
+    return out;
+  }
+
+  // compute_one_time_correlator / compute_single_ptcle_gf (block_sparse_backbone.cpp:779-902):
+  // sum over all flat indices (fix == nullptr) or the listed ones
+  std::vector<S> compute_correlator(const BlockDiagOpFun<S> &Gt, const int *topo, int m, const int *fix, long nfix,
+                                    const std::vector<BlockOp<S>> &mu_ops, const std::vector<BlockOp<S>> &kap_ops) {
+    Backbone bb(topo, m, n, true);
+    std::vector<S> corr((size_t)r * mu_ops.size() * kap_ops.size(), S(0));
+    long fmax = num_correlator_backbones(m);
+    if (fix == nullptr) {
+      for (long f = 0; f < fmax; ++f) eval_correlator(Gt, bb, mu_ops, kap_ops, (int)f, corr);
+    } else {
+      for (long i = 0; i < nfix; ++i) {
+        // the reference decodes any integer; an index past the end would alias another backbone
+        if (fix[i] < 0 || fix[i] >= fmax) throw std::runtime_error("DiagramEvaluator::eval_correlator: f_ix out of range");
+        eval_correlator(Gt, bb, mu_ops, kap_ops, fix[i], corr);
+      }
+    }
+    return corr;
+  }
+
   // compute_self_energy (block_sparse_backbone.cpp:376-459); fix == nullptr means all f_ix
   void compute_self_energy(const BlockDiagOpFun<S> &Gt, const int *topo, int m, const int *fix, long nfix) {
     Sigma = Gt;
@@ -847,8 +1094,7 @@ std::vector<S> dyson_solve(const ImTimeOps &ops, double beta, const std::vector<
 using namespace orc;
 
 struct orc_handle;
-template <class S> static int sigma_impl(DiagramEvaluator<S> &D, const Problem &P, const double *G, int m, const int *topo, const int *fix,
-                                         long nfix, double *out) {
+template <class S> static BlockDiagOpFun<S> make_G(const Problem &P, const double *G) {
   BlockDiagOpFun<S> Gt;
   Gt.r = P.r;
   Gt.dims = P.dims;
@@ -862,6 +1108,55 @@ template <class S> static int sigma_impl(DiagramEvaluator<S> &D, const Problem &
   }
   Gt.zero.assign(P.nblocks, 0);
   Gt.set_zero_block_indices();
+  return Gt;
+}
+
+// block operators from (nops, nblocks) connections and matrices packed in (op, source block) order
+template <class S> static std::vector<BlockOp<S>> make_ops(const Problem &P, int nops, const int *conn, const double *mats) {
+  std::vector<BlockOp<S>> out(nops);
+  const cplx *m = reinterpret_cast<const cplx *>(mats);
+  for (int o = 0; o < nops; ++o) {
+    BlockOp<S> &op = out[o];
+    op.block_indices.assign(conn + (size_t)o * P.nblocks, conn + (size_t)(o + 1) * P.nblocks);
+    op.blocks.resize(P.nblocks);
+    op.bout.assign(P.nblocks, 0);
+    op.bin.assign(P.nblocks, 0);
+    for (int j = 0; j < P.nblocks; ++j) {
+      int t = op.block_indices[j];
+      if (t < -1 || t >= P.nblocks) throw std::invalid_argument("correlator operator: block index out of range");
+      if (t < 0) continue;
+      op.bout[j] = P.dims[t];
+      op.bin[j] = P.dims[j];
+      size_t sz = (size_t)P.dims[t] * P.dims[j];
+      op.blocks[j].resize(sz);
+      for (size_t x = 0; x < sz; ++x) op.blocks[j][x] = DiagramEvaluator<S>::to_s(m[x]);
+      m += sz;
+    }
+  }
+  return out;
+}
+
+template <class S> static int correlator_impl(DiagramEvaluator<S> &D, const Problem &P, const double *G, int m, const int *topo, const int *fix,
+                                              long nfix, int nmu, const int *mu_conn, const double *mu_mats, int nkap, const int *kap_conn,
+                                              const double *kap_mats, double *out) {
+  auto Gt = make_G<S>(P, G);
+  std::vector<BlockOp<S>> mu, kap;
+  if (mu_conn == nullptr) {  // single-particle Green's function: c_o and c^dag_o per orbital
+    mu = D.orbital_ops(false);
+    kap = D.orbital_ops(true);
+  } else {
+    mu = make_ops<S>(P, nmu, mu_conn, mu_mats);
+    kap = make_ops<S>(P, nkap, kap_conn, kap_mats);
+  }
+  auto c = D.compute_correlator(Gt, topo, m, fix, nfix, mu, kap);
+  cplx *o = reinterpret_cast<cplx *>(out);
+  for (size_t x = 0; x < c.size(); ++x) o[x] = cplx(c[x]);
+  return 0;
+}
+
+template <class S> static int sigma_impl(DiagramEvaluator<S> &D, const Problem &P, const double *G, int m, const int *topo, const int *fix,
+                                         long nfix, double *out) {
+  auto Gt = make_G<S>(P, G);
   D.compute_self_energy(Gt, topo, m, fix, nfix);
   cplx *o = reinterpret_cast<cplx *>(out);
   for (int b = 0; b < P.nblocks; ++b) {
@@ -956,6 +1251,26 @@ int orc_sigma(void *hv, const double *G, int m, const int *topo, const int *fix,
     auto h = static_cast<orc_handle *>(hv);
     if (h->dc) return sigma_impl(*h->dc, h->P, G, m, topo, fix, nfix, out);
     return sigma_impl(*h->dr, h->P, G, m, topo, fix, nfix, out);
+  } catch (std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+long orc_num_correlator_backbones(void *hv, int m) {
+  auto h = static_cast<orc_handle *>(hv);
+  return h->dc ? h->dc->num_correlator_backbones(m) : h->dr->num_correlator_backbones(m);
+}
+
+// One-time correlator <T O_mu(tau) O_kap(0)> diagrams of one topology, out (r, nmu, nkap) complex.
+// mu_conn == NULL selects the single-particle Green's function (O_mu = c_mu, O_kap = c^dag_kap,
+// nmu = nkap = n). Operators: (nops, nblocks) target blocks and matrices packed as for orc_problem.
+int orc_correlator(void *hv, const double *G, int m, const int *topo, const int *fix, long nfix, int nmu, const int *mu_conn,
+                   const double *mu_mats, int nkap, const int *kap_conn, const double *kap_mats, double *out) {
+  try {
+    auto h = static_cast<orc_handle *>(hv);
+    if (h->dc) return correlator_impl(*h->dc, h->P, G, m, topo, fix, nfix, nmu, mu_conn, mu_mats, nkap, kap_conn, kap_mats, out);
+    return correlator_impl(*h->dr, h->P, G, m, topo, fix, nfix, nmu, mu_conn, mu_mats, nkap, kap_conn, kap_mats, out);
   } catch (std::exception &e) {
     g_err = e.what();
     return -1;

@@ -83,17 +83,31 @@ def two_fermion(beta=2.0, lam=40.0, eps=1e-12, U=3.0, mu=0.0, pole=0.0):
     return ImpurityProblem(grid=grid, ad=ad, poles=np.array([pole]), coeffs=np.eye(2)[None], G0=G0, eta0=eta0, name="two_fermion")
 
 
-def n_fermion_h0(nflav, beta=2.0, lam=20.0, eps=1e-12):
+def n_fermion_h0(nflav, beta=2.0, lam=20.0, eps=1e-12, qn="none"):
     """n flavours, H = 0, constant hybridization Delta = -1/2 on the diagonal (a single pole at 0 with
-    unit weight), one block: the analytic model of test/python/analytic_diagrams.py:82-122."""
+    unit weight), one block (or blocks by N): the analytic model of
+    test/python/analytic_diagrams.py:82-122."""
     grid = DLRGrid(beta, lam, eps)
     fs = FockSpace(nflav)
-    ad = AtomDiag(fs, sp.csr_matrix((fs.dim, fs.dim)), qn="none")
+    ad = AtomDiag(fs, sp.csr_matrix((fs.dim, fs.dim)), qn=qn)
     G0, eta0 = ad.atomic_G0(beta, grid.tau)
     return ImpurityProblem(grid=grid, ad=ad, poles=np.array([0.0]), coeffs=np.eye(nflav)[None], G0=G0, eta0=eta0, name=f"n{nflav}_h0")
 
 
-def two_band_discrete_bath(beta=2.0, lam=40.0, eps=1e-6, U=2.0, J=0.2, s=0.5, levels=(-2.3, 2.3)):
+def one_spinful_fermion(V=0.1, beta=10.0, U=1.0, mu=0.0, eps1=-0.1, eps2=0.1, eps=1e-12, w_max=40.0):
+    """One spinful level, H = U n_up n_do + (eps1 - mu) N, blocks by N, coupled to one bath level:
+    Delta(i w) = V^2 / (i w - eps2) on the diagonal, i.e. one pole at eps2 with weight V^2
+    (test/python/one_spinful_fermion.py:67-90)."""
+    grid = DLRGrid(beta, w_max * beta, eps)
+    fs = FockSpace(2)
+    H = U * (fs.nop(0) @ fs.nop(1)) + (eps1 - mu) * (fs.nop(0) + fs.nop(1))
+    ad = AtomDiag(fs, H, qn="N")
+    G0, eta0 = ad.atomic_G0(beta, grid.tau)
+    return ImpurityProblem(grid=grid, ad=ad, poles=np.array([eps2]), coeffs=V * V * np.eye(2)[None], G0=G0, eta0=eta0,
+                           name="one_spinful_fermion")
+
+
+def two_band_discrete_bath(beta=2.0, lam=40.0, eps=1e-6, U=2.0, J=0.2, s=0.5, levels=(-2.3, 2.3), qn="N"):
     """Two-band Kanamori atom (U = 2, J = 0.2, mu = (3U - 5J)/2 - 1.5), blocks by N, coupled to a
     discrete bath with levels +-2.3: Delta_ij(tau) = sum_e K(tau, e) (1 on the diagonal, s between the
     two orbitals of one spin); G = the normalised atomic propagator
@@ -101,7 +115,7 @@ def two_band_discrete_bath(beta=2.0, lam=40.0, eps=1e-6, U=2.0, J=0.2, s=0.5, le
     grid = DLRGrid(beta, lam, eps)
     fs = FockSpace(4)
     H = kanamori_hamiltonian(fs, 2, U, J, (3 * U - 5 * J) / 2 - 1.5)
-    ad = AtomDiag(fs, H, qn="N")
+    ad = AtomDiag(fs, H, qn=qn)
     jt = sum(np.array([kernel(t, beta * e) for t in grid.it]) for e in levels)
     M = np.eye(4)
     M[0, 1] = M[1, 0] = M[2, 3] = M[3, 2] = s
