@@ -162,6 +162,52 @@ def serving(n, seed=7):
             "iterations_run": out["stats"]["ticks"], "max_active_slots": 64}
 
 
+def spgf_leg(ev, P, G, max_order, n, cpu):
+    """Single-particle Green's function g(tau) of the same workload and G: every connected topology of
+    orders 1..max_order with the solver's signs (block_sparse_solver.py:657-677), timed over n
+    evaluations; the chain launch of the costliest topology is priced on the FP64 peak. With cpu, one
+    oracle evaluation (single core, complex128) is timed beside it."""
+    from xca_amd.diag import all_connected_pairings
+    topos = [(np.array(t, dtype=np.int32), (-1) ** o * s) for o in range(1, max_order + 1) for s, t in all_connected_pairings(o)]
+
+    ev.set_G(G)  # resident in HBM for every evaluation
+
+    def run(record=None):
+        tot = 0
+        for t, sc in topos:
+            tot = tot + sc * ev.compute_single_ptcle_gf(None, t)
+            if record is not None:
+                record.append(ev.last_stats())
+        return tot
+
+    run()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        g = run()
+    ms = 1e3 * (time.perf_counter() - t0) / n
+    recs = []
+    run(recs)
+    dom = max(range(len(recs)), key=lambda i: recs[i][2])
+    items, macs, kms = recs[dom]
+    achieved = 2.0 * macs / (kms * 1e-3) / 1e12
+    out = {"workload": f"g(tau) of the benchmark workload, orders 1-{max_order}, all connected topologies, same G",
+           "evals_per_s": 1e3 / ms, "ms_per_eval": ms, "items_per_eval": int(sum(x[0] for x in recs)),
+           "canonical_macs_per_eval": float(sum(x[1] for x in recs)),
+           "dominant": {"kernel": "chain_kernel (store mode)", "topology": topos[dom][0].tolist(), "kernel_ms": kms,
+                        "flops_per_launch": 2.0 * macs, "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK_FP64},
+           "g_max": float(np.abs(g).max())}
+    if cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import pyoracle as po
+        orc = po.OracleEvaluator(P, complex_arith=True)
+        t0 = time.perf_counter()
+        po.spgf_sum(orc, G, max_order)
+        out["cpu_oracle_s_per_eval"] = time.perf_counter() - t0
+        out["cpu_cores"] = 1
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +218,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle CPU work for cpu_baseline")
     ap.add_argument("--solves", type=int, default=50, help="NCA self-consistent solves timed for the latency figures (0: skip)")
     ap.add_argument("--serving", type=int, default=512, help="requests in the served stream (0: skip)")
+    ap.add_argument("--spgf", type=int, default=5, help="timed single-particle Green's function evaluations (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (reference dispatches for the HBM byte counters)")
     args = ap.parse_args()
@@ -250,6 +297,9 @@ def main():
                      "traffic": measured_traffic(args.workload), "kernel": "chain_kernel", "kernel_ms": kms,
                      "flops_per_launch": 2.0 * macs, "topology": topos[dom][0].tolist()},
     }
+    if args.spgf > 0:
+        out["spgf"] = spgf_leg(ev, P, G, max_order, args.spgf, cpu=not args.no_cpu and args.workload == "oca2")
+        log(f"spgf {out['spgf']['evals_per_s']:.1f} evals/s, dominant chain {out['spgf']['dominant']['frac']:.3f} of peak")
     if args.solves > 0:
         out["solve"] = solve_latency(args.solves)
         log(f"solve latency p50 {out['solve']['p50_ms']:.2f} ms p99 {out['solve']['p99_ms']:.2f} ms")

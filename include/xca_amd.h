@@ -79,6 +79,27 @@ int xca_num_sym_sets(xca_ctx *ctx);
  * order/topology prefactor applied by the Python caller (block_sparse_solver.py:618,636). */
 int xca_sigma(xca_ctx *ctx, const double *G, int m, const int *topo, const int32_t *fix, int64_t nfix, double *sigma_out);
 
+/* get_num_single_ptcle_gf_backbones (c++/triqs_xca/block_sparse_backbone.cpp:732-735):
+ * 2^(m-1) n^(m-1) p^(m-1), the flat indices of CorrelatorBackbone (backbone.cpp:239-292). */
+int64_t xca_num_correlator_backbones(xca_ctx *ctx, int m);
+
+/* compute_single_ptcle_gf(G, topology[, f_ix_vec]) (c++/triqs_xca/block_sparse_backbone.cpp:779-825,
+ * eval_correlator :526-706 with O_mu = c_mu, O_kap = c^dag_kap per orbital, :737-777): out is
+ * (r, n, n) real, indexed by orbital. fix == NULL evaluates every flat index; otherwise the nfix listed
+ * ones (error "f_ix out of range" outside [0, xca_num_correlator_backbones); the reference decodes such
+ * an index silently into another backbone). Carries the per-backbone signs but not the order/topology
+ * factor (-1)^order s_topo applied by the caller (block_sparse_solver.py:667-677). G == NULL uses the
+ * G last uploaded (xca_set_G, xca_sigma or a previous call). */
+int xca_spgf(xca_ctx *ctx, const double *G, int m, const int *topo, const int32_t *fix, int64_t nfix, double *out);
+
+/* compute_one_time_correlator(G, ops_tau, ops_0, topology, f_ix_vec)
+ * (c++/triqs_xca/block_sparse_backbone.cpp:850-876): <T O_mu(tau) O_kap(0)> diagrams for nmu operators
+ * at tau and nkap at 0, each a block operator: conn[(op, j)] is the block it maps block j to (-1: none)
+ * and mats concatenate the (dims[conn], dims[j]) matrices in (op, j) order over the connected j (the
+ * layout of xca_problem's c_mats). out is (r, nmu, nkap) real. G == NULL as for xca_spgf. */
+int xca_correlator(xca_ctx *ctx, const double *G, int m, const int *topo, const int32_t *fix, int64_t nfix, int nmu, const int *mu_conn,
+                   const double *mu_mats, int nkap, const int *kap_conn, const double *kap_mats, double *out);
+
 /* Device-resident form for benchmarking and the iteration loop: upload G once, evaluate any number
  * of topologies into the device Sigma accumulator with weight `scale` (Sigma += scale * S_topo),
  * then download. xca_sigma_zero clears the accumulator. */

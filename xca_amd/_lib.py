@@ -50,6 +50,9 @@ EXPORTS = {
     "xca_dyson_init": (C.c_int, [C.c_void_p, d_p]),
     "xca_dyson_solve": (C.c_int, [C.c_void_p, d_p, C.c_double, d_p]),
     "xca_ppsc_solve": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_int, C.c_double, d_p, d_p, C.POINTER(C.c_int), d_p]),
+    "xca_num_correlator_backbones": (C.c_int64, [C.c_void_p, C.c_int]),
+    "xca_spgf": (C.c_int, [C.c_void_p, d_p, C.c_int, i_p, i32_p, C.c_int64, d_p]),
+    "xca_correlator": (C.c_int, [C.c_void_p, d_p, C.c_int, i_p, i32_p, C.c_int64, C.c_int, i_p, d_p, C.c_int, i_p, d_p, d_p]),
 }
 
 

@@ -24,7 +24,7 @@ namespace xca {
 
 // tiles per chunk: enough chunks to cover the GPU several times, at most 64 tiles per chunk
 // (XCA_CHUNK_TILES overrides: longer chunks reuse a shared prefix across more tiles)
-static int chunk_tiles(int64_t total_tiles) {
+int chunk_tiles(int64_t total_tiles) {
   if (const char *e = std::getenv("XCA_CHUNK_TILES")) {
     const int v = std::atoi(e);
     if (v > 0) return v;
@@ -109,7 +109,11 @@ void build_frags(xca_ctx *c) {
       M2[(size_t)i * 2 * RP + k] = beta * o.cf2it[(size_t)i * r + k];
       M2[(size_t)i * 2 * RP + RP + k] = beta * o.tcf2it[(size_t)i * r + k];
     }
-  std::vector<double> fLiP, fW, fM2, fAU, fD;
+  std::vector<double> Ep((size_t)RP * RP, 0.0);
+  for (int i = 0; i < r; ++i)
+    for (int k = 0; k < r; ++k) Ep[(size_t)i * RP + k] = dlr::kernel(-o.it[i], o.rf[k]);  // coefficients -> values at -t_i
+  std::vector<double> fLiP, fW, fM2, fAU, fD, fE;
+  pack_frag(Ep, RP, RP, RB, KS, fE);
   pack_frag(LiP, RP, RP, RB, KS, fLiP);
   pack_frag(Wp, RP, RP, RB, KS, fW);
   pack_frag(M2, RP, 2 * RP, RB, 2 * KS, fM2);
@@ -149,7 +153,10 @@ void build_frags(xca_ctx *c) {
   size_t oU = all.size();
   for (int i = 0; i < RP; ++i)
     for (int k = i + 1; k < RP; ++k) all.push_back(-Up[(size_t)i * RP + k]);
+  size_t oE = all.size();
+  all.insert(all.end(), fE.begin(), fE.end());
   c->frag.upload(all, c->stream);
+  c->D.E = c->frag.ptr + oE;
   c->D.LiP = c->frag.ptr + oLiP;
   c->D.W = c->frag.ptr + oW;
   c->D.M2 = c->frag.ptr + oM2;
@@ -166,17 +173,6 @@ int64_t ipow(int64_t a, int e) {
   return r;
 }
 
-// canonical multiply-add count of one (path, pole tuple) evaluation
-double canonical_macs(const Program &pg, int r) {
-  double s = 0;
-  for (const auto &op : pg.ops) {
-    if (op.type == OP_V) s += (double)r * op.bout * op.bin * op.cols;
-    else if (op.type == OP_E) s += 4.0 * r * r * op.bin * op.cols + 3.0 * r * op.bin * op.bin * op.cols;
-    else s += (double)op.qs * r * op.bin * op.b1 * op.cols + (double)op.qs * op.qs * r * op.bin * op.cols + (double)op.qs * r * op.bout * op.bin * op.cols;
-  }
-  return s;
-}
-
 // Program template of one block path: ops resolved to device form, its pole-tuple count and (for an
 // explicit flat-index subset) the requested tuples.
 struct ProgSpec {
@@ -191,6 +187,59 @@ struct ProgSpec {
 }  // namespace
 
 namespace xca {
+
+// canonical multiply-add count of one (path, pole tuple) evaluation; a reflection is priced as
+// vals2coefs plus the evaluation at the reflected nodes (2 r^2 per column)
+double canonical_macs(const Program &pg, int r) {
+  double s = pg.refl ? 2.0 * r * r * pg.out_rows * pg.out_cols : 0.0;
+  for (const auto &op : pg.ops) {
+    if (op.type == OP_V) s += (double)r * op.bout * op.bin * op.cols;
+    else if (op.type == OP_E) s += 4.0 * r * r * op.bin * op.cols + 3.0 * r * op.bin * op.bin * op.cols;
+    else s += (double)op.qs * r * op.bin * op.b1 * op.cols + (double)op.qs * op.qs * r * op.bin * op.cols + (double)op.qs * r * op.bout * op.bin * op.cols;
+  }
+  return s;
+}
+
+// device form of one op
+DOp device_op(const Op &op, const std::vector<EdgeTable> &tables, const xca_ctx *c) {
+  const Problem &P = c->M.P;
+  DOp o{};
+  o.type = op.type; o.bin = op.bin; o.bout = op.bout; o.cols = op.cols;
+  o.fbase = op.fbase; o.fstride_l = op.fstride_l;
+  o.bar_line = op.bar_line; o.kx = op.kx; o.ks = op.ks; o.nlines = op.nlines;
+  for (int q = 0; q < op.nlines; ++q) o.lines[q] = op.lines[q];
+  if (op.type == OP_E) {
+    const auto &tb = tables[op.etab];
+    int64_t esz = 2 * (int64_t)P.r * P.dims[tb.block] * P.dims[tb.block];
+    o.ebase = tb.offset;
+    for (int q = 0; q < op.nlines; ++q) o.estride[q] = ipow(P.p, q) * esz;
+  }
+  o.mubase = op.mubase; o.qs = op.qs; o.forward = op.forward; o.b1 = op.b1; o.set = op.set;
+  if (op.type == OP_Z)
+    for (int k = 0; k < op.qs; ++k) o.orb[k] = c->M.S.to_orb[op.set][k];
+  return o;
+}
+
+// edge-table jobs: entries of each table in runs that fill one tile
+std::vector<DTabJob> tab_jobs(const std::vector<EdgeTable> &tables, const xca_ctx *c) {
+  const Problem &P = c->M.P;
+  std::vector<DTabJob> out;
+  for (const auto &tb : tables) {
+    int b = P.dims[tb.block], bb = b * b;
+    int per = std::max(1, tile_columns(0) / bb);
+    for (int64_t e0 = 0; e0 < tb.nentries; e0 += per) {
+      DTabJob j{};
+      j.block = tb.block; j.b = b; j.nlines = tb.nlines; j.e0 = (int)e0;
+      j.ne = (int)std::min<int64_t>(per, tb.nentries - e0);
+      j.g_off = c->M.G_off[tb.block];
+      j.tr = tb.tr;
+      for (int q = 0; q < tb.nlines; ++q) j.lsign[q] = tb.lsign[q];
+      j.out_off = tb.offset;
+      out.push_back(j);
+    }
+  }
+  return out;
+}
 
 static int64_t variant_bytes(const DevVariant &V) {
   return (int64_t)(V.progs.n * sizeof(DProg) + V.chunks.n * sizeof(DChunk) + V.slot.n * sizeof(int64_t) + V.red_chunks.n * sizeof(int) +
@@ -248,6 +297,8 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
     d.sign = pg.sign;
     d.nlines = nl;
     d.max_w = pg.max_w;
+    d.gtr = 0;
+    d.out_off = -1;
     sp.nl = nl;
     sp.cls = pg.cls;
     for (int i = 0, f = pg.fb_ix; i < m; ++i, f /= 2) sp.fb[i] = f % 2;
@@ -303,23 +354,7 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
         d.share_len = (int)(ntup / d.set_len[0]);
       }
     }
-    for (auto &op : pg.ops) {
-      DOp o{};
-      o.type = op.type; o.bin = op.bin; o.bout = op.bout; o.cols = op.cols;
-      o.fbase = op.fbase; o.fstride_l = op.fstride_l;
-      o.bar_line = op.bar_line; o.kx = op.kx; o.ks = op.ks; o.nlines = op.nlines;
-      for (int q = 0; q < op.nlines; ++q) o.lines[q] = op.lines[q];
-      if (op.type == OP_E) {
-        const auto &tb = plan.tables[op.etab];
-        int64_t esz = 2 * (int64_t)r * P.dims[tb.block] * P.dims[tb.block];
-        o.ebase = tb.offset;
-        for (int q = 0; q < op.nlines; ++q) o.estride[q] = ipow(p, q) * esz;
-      }
-      o.mubase = op.mubase; o.qs = op.qs; o.forward = op.forward; o.b1 = op.b1; o.set = op.set;
-      if (op.type == OP_Z)
-        for (int k = 0; k < op.qs; ++k) o.orb[k] = c->M.S.to_orb[op.set][k];
-      dp->h_ops.push_back(o);
-    }
+    for (auto &op : pg.ops) dp->h_ops.push_back(device_op(op, plan.tables, c));
     dp->items += ntup;
     dp->macs += canonical_macs(pg, r) * (double)ntup;
     specs.push_back(std::move(sp));
@@ -403,20 +438,7 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
     dp->var[v] = std::move(V);
     if (c->tile_variant >= 0 && dp->chosen < 0) dp->chosen = v;
   }
-  // edge-table jobs
-  for (const auto &tb : plan.tables) {
-    int b = P.dims[tb.block], bb = b * b;
-    int per = std::max(1, tile_columns(0) / bb);
-    for (int64_t e0 = 0; e0 < tb.nentries; e0 += per) {
-      DTabJob j{};
-      j.block = tb.block; j.b = b; j.nlines = tb.nlines; j.e0 = (int)e0;
-      j.ne = (int)std::min<int64_t>(per, tb.nentries - e0);
-      j.g_off = c->M.G_off[tb.block];
-      for (int q = 0; q < tb.nlines; ++q) j.lsign[q] = tb.lsign[q];
-      j.out_off = tb.offset;
-      dp->h_tab.push_back(j);
-    }
-  }
+  dp->h_tab = tab_jobs(plan.tables, c);
   dp->ops.upload(dp->h_ops, s);
   dp->tabjobs.upload(dp->h_tab, s);
   dp->ftab.alloc(std::max<int64_t>(plan.ftab_size, 1));
@@ -697,7 +719,7 @@ int64_t ctx_bytes(const xca_ctx *c) {
   const Problem &P = c->M.P;
   const int64_t RP = 16 * c->RB;
   return 8 * ((int64_t)c->M.opmats.size() + c->M.kp.size() + c->M.km.size() + c->M.delta.size() + c->M.delta_refl.size() +
-              2 * c->maxexp * P.p + 2 * c->M.G_size + 12 * RP * RP) +
+              2 * c->maxexp * P.p + 2 * c->M.G_size + 13 * RP * RP) +
          4 * (4 * P.p + 2 * P.nb);
 }
 
@@ -748,6 +770,7 @@ extern "C" void xca_ctx_destroy(xca_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->plans.clear();
+  c->corr_plans.clear();
   xca_dyson_release(c);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -829,6 +852,7 @@ extern "C" int xca_set_tile_variant(xca_ctx *c, int variant) {
   }
   c->tile_variant = variant;
   c->plans.clear();
+  c->corr_plans.clear();
   return 0;
 }
 

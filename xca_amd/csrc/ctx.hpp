@@ -45,6 +45,7 @@ template <class T> struct DevBuf {
 };
 
 struct DevPlan;
+struct CorrDevPlan;
 
 }  // namespace xca
 
@@ -71,6 +72,8 @@ struct xca_ctx {
   std::vector<int> G_zero;
   bool have_G = false;
   Plans plans;
+  std::map<std::vector<int>, std::unique_ptr<xca::CorrDevPlan>> corr_plans;
+  xca::DevBuf<double> corr_out;
   XcaDyson *dyson = nullptr;
   int64_t last_items = 0;
   double last_macs = 0, last_ms = 0;

@@ -237,7 +237,7 @@ __global__ void __launch_bounds__(64 * NW) tab_kernel(TabBatch B) {
   // GK(t) for each entry
   for (int x = tid; x < r * ncol; x += NT) {
     const int t = x / ncol, c = x % ncol, e = c / bb, ij = c % bb;
-    double v = G[J.g_off + (size_t)t * bb + ij];
+    double v = G[J.g_off + (size_t)t * bb + (J.tr ? (ij % b) * b + ij / b : ij)];  // G_b or its transpose
     int ee = J.e0 + e;
     for (int q = 0; q < J.nlines; ++q) {
       const int l = ee % p;
@@ -283,6 +283,48 @@ __global__ void __launch_bounds__(64 * NW) tab_kernel(TabBatch B) {
           if (t < r) out[(size_t)t * bb + ij] = acc[rb][v];
         }
     }
+  }
+}
+
+// -------------------------------------------------------------------------------------------
+// Reflection t -> 1 - t of stored correlator outputs, in place: X <- E U^{-1} (L^{-1} P) X along
+// the time index for every column of a job (values at the reflected nodes from the coefficients).
+// One wave owns 16 columns; columns of different jobs never overlap.
+template <int RB, int NW>
+__global__ void __launch_bounds__(64 * NW) refl_kernel(ReflArgs A) {
+  constexpr int RP = 16 * RB, NT = 64 * NW, LDN = 16 * NW + 8;
+  extern __shared__ double lds[];
+  double *SA = lds, *SB = lds + RP * LDN;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const DlrFrag D = A.D;
+  const RJob J = A.jobs[blockIdx.x];
+  const int r = D.r, w = J.w, ncol = J.ncol;
+  for (int x = tid; x < RP * LDN; x += NT) { SA[x] = 0.0; SB[x] = 0.0; }
+  __syncthreads();
+  for (int x = tid; x < r * ncol; x += NT) {
+    const int t = x / ncol, c = x % ncol, col = J.c0 + c;
+    SA[(size_t)t * LDN + c] = A.store[J.base + ((size_t)(col / w) * r + t) * w + col % w];
+  }
+  __syncthreads();
+  v4d acc[RB];
+  const int c0 = 16 * wv;
+  vals2coefs_block<RB>(D, SA, LDN, c0, acc, lane);
+  store_acc<RB>(acc, SB, LDN, c0, lane, r);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  zero_acc(acc);
+  gemm_acc<RB>(D.E, 4 * RB, 0, (r + 3) >> 2, SB, LDN, c0, acc, lane);
+  const int c = c0 + (lane & 15);
+  if (c < ncol) {
+    const int col = J.c0 + c;
+    double *out = A.store + J.base + (size_t)(col / w) * r * w + col % w;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int t = acc_row(rb, v, lane);
+        if (t < r) out[(size_t)t * w] = acc[rb][v];
+      }
   }
 }
 
@@ -385,11 +427,11 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
       oi0 = P.split;
       const int w = rows * cols;
       for (int x = tid; x < r * w; x += NT) S0[(size_t)(x / w) * LDN + x % w] = cache[x];
-    } else {  // T = G_b1
+    } else {  // T = G_b1, or its transpose
       const int b1 = P.b1, w = b1 * b1;
       for (int x = tid; x < r * nit * w; x += NT) {
-        const int t = x / (nit * w), c = x % (nit * w);
-        S0[(size_t)t * LDN + c] = A.G[P.g_init + (size_t)t * w + c % w];
+        const int t = x / (nit * w), c = x % (nit * w), e = c % w;
+        S0[(size_t)t * LDN + c] = A.G[P.g_init + (size_t)t * w + (P.gtr ? (e % b1) * b1 + e / b1 : e)];
       }
     }
     __syncthreads();
@@ -580,6 +622,15 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
         __syncthreads();
       }
     }
+    if (P.out_off >= 0) {  // store mode: every item's T with its prefactor
+      const int w = rows * cols, rw = r * w;
+      double *o = A.store + P.out_off + (size_t)tstart * rw;
+      for (int x = tid; x < P_items * rw; x += NT) {
+        const int it = x / rw, rem = x % rw;
+        o[x] = Scur[(size_t)(rem / w) * LDN + it * w + rem % w] * ipref[it];
+      }
+      continue;
+    }
     // final: per item max |T| (with the prefactor) and accumulation into the chunk's partial slot
     const int b0 = P.b0, w = b0 * b0;
     for (int x = tid; x < P_items * r; x += NT) {
@@ -660,6 +711,62 @@ __global__ void __launch_bounds__(256) reduce_assign_kernel(RedBatch B) {
 
 __global__ void reset_queue_kernel(int *q) {
   if (threadIdx.x == 0) q[0] = 0;
+}
+
+// One-time correlator contraction. Workgroup = chunk (pole tuples [k0, k1) of one group); thread =
+// (output run, t). A thread sums its run's terms Tr[L O_mu R O_kap](t) = sum_i sum_a (O_mu R)_ia
+// (L^T O_kap^T)_ia over the chunk's tuples in a fixed order, so the result does not depend on the
+// schedule. Every chunk writes its whole (r, nout) slot.
+__global__ void __launch_bounds__(256) corr_kernel(CorrArgs A) {
+  const CChunk ch = A.chunks[blockIdx.x];
+  const int r = A.r, nrun = ch.run1 - ch.run0;
+  double *slot = A.partial + (size_t)blockIdx.x * r * A.nout;
+  for (int x = threadIdx.x; x < r * A.nout; x += blockDim.x) slot[x] = 0.0;
+  __syncthreads();
+  for (int x = threadIdx.x; x < nrun * r; x += blockDim.x) {
+    const int t = x % r;
+    const CRun run = A.runs[ch.run0 + x / r];
+    double sum = 0.0;
+    for (int k = ch.k0; k < ch.k1; ++k)
+      for (int q = run.t0; q < run.t1; ++q) {
+        const CTerm T = A.terms[q];
+        const int d0 = T.d0, d1 = T.d1, db = T.db, dc = T.dc;
+        const double *R = A.store + T.r_off + ((size_t)k * r + t) * d0 * db;
+        const double *L = A.store + T.l_off + ((size_t)k * r + t) * d1 * dc;
+        const double *Mu = A.cops + T.mu_off, *Kp = A.cops + T.kap_off;
+        for (int i = 0; i < d1; ++i) {
+          double nr[kMaxB], qr[kMaxB];
+#pragma unroll
+          for (int a = 0; a < kMaxB; ++a) nr[a] = qr[a] = 0.0;
+          for (int kk = 0; kk < d0; ++kk) {
+            const double m = Mu[i * d0 + kk];
+#pragma unroll
+            for (int a = 0; a < kMaxB; ++a)
+              if (a < db) nr[a] += m * R[kk * db + a];
+          }
+          for (int j = 0; j < dc; ++j) {
+            const double l = L[i * dc + j];
+#pragma unroll
+            for (int a = 0; a < kMaxB; ++a)
+              if (a < db) qr[a] += l * Kp[a * dc + j];
+          }
+#pragma unroll
+          for (int a = 0; a < kMaxB; ++a)
+            if (a < db) sum += nr[a] * qr[a];
+        }
+      }
+    slot[(size_t)t * A.nout + run.out] = sum;
+  }
+}
+
+// out[e] (=, or += when accumulate) sum over chunks c, in order, of partial[c][e]
+__global__ void __launch_bounds__(256) corr_reduce_kernel(const double *__restrict__ partial, int nchunks, int n, double *out,
+                                                          int accumulate) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double s = accumulate ? out[e] : 0.0;
+  for (int c = 0; c < nchunks; ++c) s += partial[(size_t)c * n + e];
+  out[e] = s;
 }
 
 // -------------------------------------------------------------------------------------------
@@ -796,5 +903,38 @@ hipError_t launch_reduce_assign(const RedBatch &B, int max_size, hipStream_t s) 
 }
 
 size_t chain_args_size() { return sizeof(ChainArgs); }
+
+template <int RB, int NW>
+static hipError_t launch_refl_t(const ReflArgs &A, hipStream_t s) {
+  constexpr int RP = 16 * RB, LDN = 16 * NW + 8;
+  const size_t sh = sizeof(double) * ((size_t)2 * RP * LDN);
+  static bool ready = false;
+  if (!ready) {
+    hipError_t e = hipFuncSetAttribute((const void *)refl_kernel<RB, NW>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e != hipSuccess) return e;
+    ready = true;
+  }
+  if (A.njobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL((refl_kernel<RB, NW>), dim3(A.njobs), dim3(64 * NW), sh, s, A);
+  return hipGetLastError();
+}
+
+int refl_columns() { return 16 * 8; }
+
+hipError_t launch_refl(int RB, const ReflArgs &A, hipStream_t s) {
+  return RB == 2 ? launch_refl_t<2, 8>(A, s) : launch_refl_t<4, 8>(A, s);
+}
+
+hipError_t launch_corr(const CorrArgs &A, hipStream_t s) {
+  if (A.nchunks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(corr_kernel, dim3(A.nchunks), dim3(256), 0, s, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_corr_reduce(const double *partial, int nchunks, int n, double *out, int accumulate, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(corr_reduce_kernel, dim3((n + 255) / 256), dim3(256), 0, s, partial, nchunks, n, out, accumulate);
+  return hipGetLastError();
+}
 
 }  // namespace xca::dev

@@ -30,7 +30,9 @@ struct DProg {
   int items_per_tile, max_w;
   int split;      // ops [0, split) depend on the first line's pole only (0: nothing shared)
   int share_len;  // consecutive items with the same first-line pole
-  int pad0;
+  int gtr;        // start from the transposed G block
+  int64_t out_off;  // < 0: accumulate into the chunk's Sigma slot; else store item u's T (r, rows,
+                    // cols) at store + out_off + u r rows cols
 };
 
 // A run of consecutive tiles [t0, t1) of one program; tile t holds items
@@ -44,7 +46,7 @@ struct DChunk {
 // Edge-function table job: entries [e0, e0 + ne) of one table
 struct DTabJob {
   int block, b, nlines, e0;
-  int ne, g_off, pad0, pad1;
+  int ne, g_off, tr, pad1;  // tr: table of the transposed G block
   int lsign[kMaxLines], pad2;
   int64_t out_off;   // doubles, entry 0 of this table
 };
@@ -61,6 +63,7 @@ struct DlrFrag {
   const double *AU;    // upper-triangular trailing-update fragments, per (step, row block)
   const double *Dinv;  // (RP/4, 16): inverses of the 4x4 diagonal blocks of U
   const double *Upk;   // -U strictly above the diagonal, packed by rows (RP (RP - 1) / 2)
+  const double *E;     // (RB, KS, 64): K(-t_i, om_k), values at the reflected nodes from coefficients
   int r, RP;
 };
 
@@ -72,6 +75,7 @@ struct ChainArgs {
   const int *plist;            // pole lists
   const double *G, *opmats, *ftab, *kp, *km, *delta, *delta_refl, *kpow;
   double *partial;
+  double *store;               // per-item outputs of store-mode programs
   int nchunks;
   int r, p, n, maxexp;
 };
@@ -125,6 +129,48 @@ struct RedBatch {
   const RedProb *probs;
   const int *prefix;  // (problem, block) prefix over problems (nprobs + 1)
   int nprobs, total;
+};
+
+// One-time correlator contraction. A term adds Tr[L(t) O_mu R(t) O_kap] for every pole tuple of its
+// group, with R (d0 x db) the [tau, 0] side, L^T (d1 x dc) the reflected [beta, tau] side stored
+// transposed, O_mu (d1 x d0) and O_kap (db x dc).
+struct CTerm {
+  int64_t r_off, l_off;      // item 0 of the right / left program in the store (doubles)
+  int64_t mu_off, kap_off;   // operator blocks in the correlator operator buffer
+  int d0, d1, db, dc;
+};
+
+// Terms [t0, t1) of one group that add into output out = mu * nkap + kap.
+struct CRun {
+  int out, t0, t1, pad0;
+};
+
+// A run of pole tuples [k0, k1) of one group with runs [run0, run1): one workgroup each.
+struct CChunk {
+  int run0, run1, k0, k1;
+};
+
+// Columns [c0, c0 + ncol) of a stored block of items (r, w) each at base: column c is element c % w
+// of item c / w.
+struct RJob {
+  int64_t base;
+  int w, c0, ncol, pad0;
+};
+
+struct ReflArgs {
+  DlrFrag D;
+  const RJob *jobs;
+  double *store;
+  int njobs, pad0;
+};
+
+struct CorrArgs {
+  const CTerm *terms;
+  const CRun *runs;
+  const CChunk *chunks;
+  const double *store, *cops;
+  double *partial;   // (nchunks, r, nout)
+  int nchunks, r, nout, pad0;
 };
 
 }  // namespace xca::dev

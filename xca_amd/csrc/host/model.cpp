@@ -137,6 +137,30 @@ void Model::init(const Problem &Pin) {
     }
   }
 
+  // transposed copies (the [beta, tau] side of correlators multiplies from the right)
+  off_FT.assign(off_F.size(), -1);
+  off_barT.assign(off_bar.size(), -1);
+  for (int s = 0; s < S.q; ++s)
+    for (int dag = 0; dag < 2; ++dag)
+      for (int j = 0; j < nb; ++j) {
+        const size_t key = ((size_t)s * 2 + dag) * nb + j;
+        const int t = conn_set[key];
+        if (t < 0) continue;
+        const int dout = P.dims[t], din = P.dims[j];
+        const size_t sz = (size_t)dout * din;
+        const int nmat[2] = {S.size[s], S.size[s] * p};
+        const int64_t src[2] = {off_F[key], off_bar[key]};
+        int64_t *dst[2] = {&off_FT[key], &off_barT[key]};
+        for (int w = 0; w < 2; ++w) {
+          *dst[w] = (int64_t)opmats.size();
+          const size_t base = opmats.size();
+          opmats.resize(base + nmat[w] * sz);
+          for (int k = 0; k < nmat[w]; ++k)
+            for (int a = 0; a < dout; ++a)
+              for (int b = 0; b < din; ++b) opmats[base + k * sz + (size_t)b * dout + a] = opmats[src[w] + k * sz + (size_t)a * din + b];
+        }
+      }
+
   // hybridization tables (hyb.cpp:15-44, 94-113)
   kp.assign((size_t)p * r, 0); km.assign((size_t)p * r, 0);
   k0p.assign(p, 0); k0m.assign(p, 0);
@@ -370,6 +394,217 @@ void build_programs(const Model &M, const PoleSets &ps, const int *topo, int m, 
             plan.progs.push_back(std::move(pg));
           }
         }
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------------------------
+
+void build_corr_programs(const Model &M, const PoleSets &ps, const int *topo, int m, const std::vector<int> &G_zero,
+                         const CorrOps &co, CorrPlanHost &plan) {
+  const Problem &P = M.P;
+  const int n = P.n, nb = P.nb;
+  auto T = [&](int i, int j) { return topo[2 * i + j]; };
+  for (int i = 0; i < m; ++i)
+    if (T(i, 0) >= T(i, 1)) throw std::invalid_argument("first row of topology must contain smaller-numbered vertices");
+  if (T(0, 0) != 0) throw std::invalid_argument("topology(0,0) must be 0");
+  const int vct0 = T(0, 1);
+  const int nlines = m - 1;
+  std::vector<int> la(std::max(nlines, 1)), lb(std::max(nlines, 1));
+  for (int x = 0; x < nlines; ++x) { la[x] = T(x + 1, 0); lb[x] = T(x + 1, 1); }
+  std::vector<int> bar_line(2 * m, -1);
+  for (int x = 0; x < nlines; ++x) bar_line[lb[x]] = x;
+  auto get_table = [&](int block, int nl, const int *sg, int tr) {
+    for (size_t i = 0; i < plan.tables.size(); ++i) {
+      const auto &tb = plan.tables[i];
+      if (tb.block == block && tb.tr == tr && tb.nlines == nl && std::equal(sg, sg + nl, tb.lsign)) return (int)i;
+    }
+    EdgeTable tb;
+    tb.block = block;
+    tb.tr = tr;
+    tb.nlines = nl;
+    for (int i = 0; i < nl; ++i) tb.lsign[i] = sg[i];
+    tb.nentries = ipow(P.p, nl);
+    tb.offset = plan.ftab_size;
+    plan.ftab_size += tb.nentries * 2 * (int64_t)P.r * P.dims[block] * P.dims[block];
+    plan.tables.push_back(tb);
+    return (int)plan.tables.size() - 1;
+  };
+  auto zero = [&](int b) { return b < 0 || G_zero[b]; };
+
+  const int64_t o_max = ipow(n, m - 1);
+  for (int fb_ix = 0; fb_ix < (1 << nlines); ++fb_ix) {
+    // backbone.cpp:253-265
+    std::vector<int> fb(m, -1), dag(2 * m, 0), bar(2 * m, 0);
+    for (int i = 1, f = fb_ix; i < m; ++i, f /= 2) fb[i] = f % 2;
+    dag[0] = 1;
+    dag[vct0] = 0;
+    for (int i = 1; i < m; ++i) {
+      bar[T(i, 1)] = 1;
+      dag[T(i, 0)] = (fb[i] != 1);
+      dag[T(i, 1)] = (fb[i] == 1);
+    }
+    for (int64_t o_ix = 0; o_ix < o_max; ++o_ix) {
+      std::vector<int> orb(2 * m, -1);
+      for (int i = 1, oo = (int)o_ix; i < m; ++i, oo /= n) { orb[T(i, 0)] = oo % n; orb[T(i, 1)] = oo % n; }
+      auto conn_w = [&](int w, int ip) { return M.conn(M.S.label[orb[w]], dag[w], ip); };
+      // block walks of both sides (block_sparse_backbone.cpp:565-575, 606-616)
+      std::vector<std::vector<int>> rind(nb), lind(nb);
+      std::vector<int> rend(nb, -1), lend(nb, -1);
+      for (int b = 0; b < nb; ++b) {
+        std::vector<int> ind(2 * m, -1);
+        bool inc = false;
+        int ip = b;
+        ind[0] = b;
+        for (int w = 1; w < vct0 && !inc; ++w) {
+          ip = conn_w(w, ip);
+          inc = zero(ip);
+          if (!inc) ind[w] = ip;
+        }
+        if (!inc) { rind[b] = ind; rend[b] = ind[vct0 - 1]; }
+        std::fill(ind.begin(), ind.end(), -1);
+        inc = false;
+        ip = b;
+        ind[vct0] = b;
+        for (int w = vct0 + 1; w < 2 * m && !inc; ++w) {
+          ip = conn_w(w, ip);
+          inc = zero(ip);
+          if (!inc) ind[w] = ip;
+        }
+        if (!inc) { lind[b] = ind; lend[b] = ind[2 * m - 1]; }
+      }
+      // contraction terms (block_sparse_backbone.cpp:639-702), independent of the pole tuple
+      std::vector<CorrTerm> terms;
+      for (int mu = 0; mu < co.nmu; ++mu)
+        for (int kap = 0; kap < co.nkap; ++kap)
+          for (int b = 0; b < nb; ++b) {
+            if (rind[b].empty()) continue;
+            int e0 = rend[b];
+            if (zero(e0)) continue;
+            int e1 = co.mu_conn[(size_t)mu * nb + e0];
+            if (zero(e1) || lind[e1].empty()) continue;
+            int c = lend[e1];
+            if (zero(c)) continue;
+            if (co.kap_conn[(size_t)kap * nb + c] != b) continue;
+            CorrTerm t;
+            t.mu = mu; t.kap = kap; t.b = b; t.e0 = e0; t.e1 = e1; t.c = c;
+            terms.push_back(t);
+          }
+      if (terms.empty()) continue;
+      for (int cls = 0; cls < (1 << nlines); ++cls) {
+        bool empty = false;
+        for (int x = 0; x < nlines; ++x)
+          if (ps.get(fb[x + 1], (cls >> x) & 1).empty()) empty = true;
+        if (empty) continue;
+        // K placement per line (backbone.cpp:86-139)
+        std::vector<int> vK(2 * m, 0), vKx(2 * m, -1);
+        std::vector<std::vector<int>> eK(2 * m - 1, std::vector<int>(std::max(nlines, 1), 0));
+        int psign = 1, pks[3] = {0, 0, 0}, pke[3] = {0, 0, 0};
+        for (int x = 0; x < nlines; ++x) {
+          bool rb = (cls >> x) & 1;
+          bool fwd = fb[x + 1] == 1;
+          if (!fwd) psign = -psign;
+          if (!rb) {
+            vK[la[x]] = fwd ? -1 : 1; vKx[la[x]] = x;
+            vK[lb[x]] = fwd ? 1 : -1; vKx[lb[x]] = x;
+            pks[x] = fwd ? -1 : 1;
+            pke[x] = 1;
+          } else {
+            for (int j = la[x]; j < lb[x]; ++j) eK[j][x] = fwd ? 1 : -1;
+            pks[x] = fwd ? 1 : -1;
+            pke[x] = lb[x] - la[x] - 1;
+          }
+        }
+        // vertex v on a walk with block path ind and dimensions bd (F maps ind[v-1] -> ind[v])
+        auto vertex_op = [&](int v, const std::vector<int> &ind, int cols, bool transposed) {
+          Op op;
+          op.type = OP_V;
+          const int din = P.dims[ind[v - 1]], dout = P.dims[ind[v]];
+          op.bin = transposed ? dout : din;
+          op.bout = transposed ? din : dout;
+          op.cols = cols;
+          op.tr = transposed ? 1 : 0;
+          int set = M.S.label[orb[v]], qo = M.S.ind[orb[v]];
+          size_t sz = (size_t)din * dout;
+          if (bar[v]) {
+            op.fbase = (transposed ? M.bar_offset_T(set, dag[v], ind[v - 1]) : M.bar_offset(set, dag[v], ind[v - 1])) + (int64_t)qo * P.p * sz;
+            op.fstride_l = (int64_t)sz;
+            op.bar_line = bar_line[v];
+          } else {
+            op.fbase = (transposed ? M.F_offset_T(set, dag[v], ind[v - 1]) : M.F_offset(set, dag[v], ind[v - 1])) + (int64_t)qo * sz;
+          }
+          if (op.fbase < 0) throw std::logic_error("missing operator block");
+          // [beta, tau] vertices take the kernel with the opposite sign (block_sparse_backbone.cpp:492)
+          if (vK[v] != 0) { op.kx = vKx[v]; op.ks = transposed ? -vK[v] : vK[v]; }
+          return op;
+        };
+        auto edge_op = [&](int e, const std::vector<int> &ind, int cols, bool transposed) {
+          Op op;
+          op.type = OP_E;
+          op.block = ind[e];
+          op.bin = op.bout = P.dims[ind[e]];
+          op.cols = cols;
+          op.tr = transposed ? 1 : 0;
+          int sg[3] = {0, 0, 0};
+          for (int x = 0; x < nlines; ++x)
+            if (eK[e][x] != 0) { op.lines[op.nlines] = x; op.lsign[op.nlines] = eK[e][x]; sg[op.nlines] = eK[e][x]; op.nlines++; }
+          op.etab = get_table(op.block, op.nlines, sg, op.tr);
+          return op;
+        };
+        auto finish = [&](Program &pg) {
+          int rows = P.dims[pg.init_block], cols = rows;
+          pg.max_w = rows * cols;
+          for (auto &op : pg.ops) {
+            pg.max_w = std::max(pg.max_w, std::max(op.bin * cols, op.bout * cols));
+            rows = op.bout;
+          }
+          pg.out_rows = rows;
+          pg.out_cols = cols;
+        };
+        CorrGroup g;
+        g.fb_ix = fb_ix; g.o_ix = (int)o_ix; g.cls = cls;
+        g.prog0 = (int)plan.progs.size();
+        std::vector<int> rprog(nb, -1), lprog(nb, -1);
+        for (auto &t : terms) {
+          if (rprog[t.b] < 0) {  // [tau, 0] side from G_b (block_sparse_backbone.cpp:577-591)
+            const auto &ind = rind[t.b];
+            Program pg;
+            pg.m = m; pg.sigma_block = -1; pg.init_block = t.b; pg.fb_ix = fb_ix; pg.o_ix = (int)o_ix; pg.cls = cls;
+            pg.b1 = P.dims[t.b];
+            pg.sign = psign;
+            for (int x = 0; x < nlines; ++x) { pg.pk_sign[x] = pks[x]; pg.pk_exp[x] = pke[x]; }
+            for (int v = 1; v < vct0; ++v) {
+              pg.ops.push_back(vertex_op(v, ind, pg.b1, false));
+              pg.ops.push_back(edge_op(v, ind, pg.b1, false));
+            }
+            finish(pg);
+            rprog[t.b] = (int)plan.progs.size();
+            plan.progs.push_back(std::move(pg));
+          }
+          if (lprog[t.e1] < 0) {  // [beta, tau] side, transposed (block_sparse_backbone.cpp:618-633)
+            const auto &ind = lind[t.e1];
+            Program pg;
+            pg.m = m; pg.sigma_block = -1; pg.init_block = ind[2 * m - 1]; pg.fb_ix = fb_ix; pg.o_ix = (int)o_ix; pg.cls = cls;
+            pg.b1 = P.dims[pg.init_block];
+            pg.gtr = 1;
+            pg.sign = 1;
+            for (int v = 2 * m - 1; v > vct0; --v) {
+              pg.ops.push_back(vertex_op(v, ind, pg.b1, true));
+              pg.ops.push_back(edge_op(v - 1, ind, pg.b1, true));
+            }
+            pg.refl = 1;
+            finish(pg);
+            lprog[t.e1] = (int)plan.progs.size();
+            plan.progs.push_back(std::move(pg));
+          }
+          CorrTerm u = t;
+          u.rprog = rprog[t.b];
+          u.lprog = lprog[t.e1];
+          g.terms.push_back(u);
+        }
+        g.prog1 = (int)plan.progs.size();
+        plan.groups.push_back(std::move(g));
       }
     }
   }

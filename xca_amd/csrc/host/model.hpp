@@ -58,6 +58,7 @@ struct Op {
   // Z: kappa operators (b1 x b0) at fbase, mu operators (bvp x bv) at mubase, set size qs
   int64_t mubase = 0;
   int qs = 0, forward = 0, set = -1, b1 = 0;
+  int tr = 0;  // V, E: transposed operator / edge function (the [beta, tau] side of a correlator)
 };
 
 struct Program {
@@ -72,12 +73,16 @@ struct Program {
   int pk_exp[3] = {0, 0, 0};    // prefactor K exponent per line
   std::vector<Op> ops;
   int max_w = 0;                // max rows*cols over the program
+  int gtr = 0;                  // start from the transposed G block
+  int refl = 0;                 // reflect the result t -> 1 - t (after the prefactor)
+  int out_rows = 0, out_cols = 0;  // shape of the final T
 };
 
 // Table of edge functions for one (block, K-line pattern): entries indexed by the poles of the
 // lines in the pattern; each entry holds f = vals2coefs(G_b K...) and W f, both (r, b, b).
 struct EdgeTable {
   int block = 0;
+  int tr = 0;                   // table of G_block^T
   int nlines = 0;
   int lsign[3] = {0, 0, 0};
   int64_t nentries = 1;
@@ -91,9 +96,10 @@ struct Model {
   std::vector<double> kp, km;   // (p, r): K(t_i, +beta*pole_l), K(t_i, -beta*pole_l)
   std::vector<double> k0p, k0m; // (p)
   std::vector<double> delta, delta_refl;  // (r, n, n)
-  std::vector<double> opmats;   // F, F^dag (q_s, bout, bin) and barred (q_s, p, bout, bin) blocks
+  std::vector<double> opmats;   // F, F^dag (q_s, bout, bin) and barred (q_s, p, bout, bin) blocks,
+                                // then the same blocks transposed ((q_s, bin, bout) and (q_s, p, bin, bout))
   // offsets into opmats: [set][dag][block], -1 if absent
-  std::vector<int64_t> off_F, off_bar;
+  std::vector<int64_t> off_F, off_bar, off_FT, off_barT;
   std::vector<int> conn_set;    // [set*2+dag][block] target block or -1
   std::vector<int> G_off;       // offsets (in doubles) of G blocks in a packed (r,b,b) array
   int64_t G_size = 0;
@@ -102,6 +108,8 @@ struct Model {
   void init(const Problem &P);
   int64_t F_offset(int set, int dag, int block) const { return off_F[((size_t)set * 2 + dag) * P.nb + block]; }
   int64_t bar_offset(int set, int dag, int block) const { return off_bar[((size_t)set * 2 + dag) * P.nb + block]; }
+  int64_t F_offset_T(int set, int dag, int block) const { return off_FT[((size_t)set * 2 + dag) * P.nb + block]; }
+  int64_t bar_offset_T(int set, int dag, int block) const { return off_barT[((size_t)set * 2 + dag) * P.nb + block]; }
   int conn(int set, int dag, int block) const { return conn_set[((size_t)set * 2 + dag) * P.nb + block]; }
   int64_t num_backbones(int m) const;
 };
@@ -125,5 +133,46 @@ struct Plan {
 // flags, appending programs (one per path and non-empty rule class) and edge tables to plan.
 void build_programs(const Model &M, const PoleSets &ps, const int *topo, int m, int topo_id, const std::vector<int> &G_zero,
                     Plan &plan);
+
+// ---------------------------------------------------------------------------------------------
+// One-time correlators <T O_mu(tau) O_kap(0)> (c++/triqs_xca/block_sparse_backbone.cpp:526-706,
+// backbone.cpp:239-292). Vertex 0 carries O_kap, vertex vct0 = topology(0, 1) carries O_mu, and
+// the flat index f_ix = o_ix + n^(m-1) (p_ix + p^(m-1) fb_ix) runs over 2^(m-1) direction patterns
+// of the lines 1..m-1. For one (fb_ix, o_ix, rule class) "group" and every pole tuple:
+//   right[b] = the [tau, 0] side started from G_b (vertices 1..vct0-1), times the prefactor;
+//   left[b]  = the [beta, tau] side started from G at the path's end (vertices 2m-1..vct0+1,
+//              multiplied from the right), then reflected; it is built transposed, as a left
+//              product with transposed operators and edge functions;
+//   C(t, mu, kap) += Tr[left[e1](t) O_mu right[b](t) O_kap] for e0 = end of right[b],
+//              e1 = O_mu(e0), c = end of left[e1] and O_kap(c) = b.
+
+// Block operators: target block per source block (-1: none), matrices at offsets into `mats`.
+struct CorrOps {
+  int nmu = 0, nkap = 0;
+  std::vector<int> mu_conn, kap_conn;    // (nops, nb)
+  std::vector<int64_t> mu_off, kap_off;  // (nops, nb), -1 if absent
+  std::vector<double> mats;
+};
+
+struct CorrTerm {
+  int mu = 0, kap = 0, b = 0, e0 = 0, e1 = 0, c = 0;
+  int rprog = 0, lprog = 0;  // indices into CorrPlanHost::progs
+};
+
+struct CorrGroup {
+  int fb_ix = 0, o_ix = 0, cls = 0;
+  int prog0 = 0, prog1 = 0;  // programs [prog0, prog1) of the group (right and left)
+  std::vector<CorrTerm> terms;  // sorted by (mu, kap)
+};
+
+struct CorrPlanHost {
+  std::vector<Program> progs;
+  std::vector<EdgeTable> tables;
+  int64_t ftab_size = 0;
+  std::vector<CorrGroup> groups;
+};
+
+void build_corr_programs(const Model &M, const PoleSets &ps, const int *topo, int m, const std::vector<int> &G_zero,
+                         const CorrOps &co, CorrPlanHost &plan);
 
 } // namespace xca

@@ -20,6 +20,10 @@ hipError_t launch_fill(int64_t n, double v, double *p, hipStream_t s);
 hipError_t launch_reduce(const int64_t *slot, const int *red_chunks, const I2 *red_range, const int64_t *sig_off, const int *sig_size,
                          int nblocks, int max_size, const double *partial, double *sigma, double scale, hipStream_t s);
 hipError_t launch_reduce_assign(const RedBatch &B, int max_size, hipStream_t s);
+hipError_t launch_corr(const CorrArgs &A, hipStream_t s);
+hipError_t launch_refl(int RB, const ReflArgs &A, hipStream_t s);
+int refl_columns();  // columns of one reflection job
+hipError_t launch_corr_reduce(const double *partial, int nchunks, int n, double *out, int accumulate, hipStream_t s);
 }  // namespace xca::dev
 
 namespace xca {
@@ -72,7 +76,46 @@ struct DevPlan {
   std::vector<int> h_tseg_prefix;
 };
 
+// Correlator programs of one batch (a run of groups whose per-item outputs fit the store): chain
+// programs and chunks (store mode) and the contraction terms, runs and chunks.
+struct CorrBatchDev {
+  DevBuf<DProg> progs;
+  DevBuf<DChunk> chunks;
+  DevBuf<CTerm> terms;
+  DevBuf<CRun> runs;
+  DevBuf<CChunk> cchunks;
+  DevBuf<RJob> rjobs;
+  DevBuf<ChainArgs> seg;
+  DevBuf<int> seg_prefix;
+  int nchunks = 0, ncchunks = 0, nrjobs = 0;
+  int64_t store_size = 0;
+  std::vector<DProg> h_progs;
+  std::vector<DChunk> h_chunks;
+};
+
+// Everything one correlator evaluation (topology, zero-block pattern, flat-index subset, operator
+// connections) needs on the device.
+struct CorrDevPlan {
+  DevBuf<DOp> ops;
+  DevBuf<DTabJob> tabjobs;
+  DevBuf<double> ftab, store, partial, cops;
+  DevBuf<int> queue;
+  DevBuf<TabSeg> tseg;
+  DevBuf<int> tseg_prefix;
+  std::vector<std::unique_ptr<CorrBatchDev>> batches;
+  int ntabjobs = 0, nmu = 0, nkap = 0, variant = 0;
+  int64_t items = 0;       // (path, pole tuple) chain evaluations
+  double macs = 0;         // canonical multiply-adds of the chains and the contraction
+  int64_t device_bytes = 0;
+  std::vector<DTabJob> h_tab;
+};
+
 DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t nfix);
+int chunk_tiles(int64_t total_tiles);
+double canonical_macs(const Program &pg, int r);
+DOp device_op(const Op &op, const std::vector<EdgeTable> &tables, const xca_ctx *c);
+std::vector<DTabJob> tab_jobs(const std::vector<EdgeTable> &tables, const xca_ctx *c);
+void corr_release(xca_ctx *c);
 ChainArgs chain_args(xca_ctx *c, DevPlan *dp, DevVariant *V);
 TabSeg tab_seg(xca_ctx *c, DevPlan *dp);
 void autotune(xca_ctx *c, DevPlan *dp);

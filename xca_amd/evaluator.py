@@ -112,6 +112,66 @@ class DiagramEvaluator:
     def reset(self):
         check(lib().xca_sigma_zero(self._h))
 
+    # ---------------------------------------------------------------- correlators
+    def get_num_single_ptcle_gf_backbones(self, topology):
+        """2^(m-1) n^(m-1) p^(m-1) (block_sparse_backbone.cpp:732-735)."""
+        return int(lib().xca_num_correlator_backbones(self._h, len(topology)))
+
+    def _fix(self, f_ix):
+        if f_ix is None:
+            return None, 0
+        fx = as_i32(np.atleast_1d(f_ix))
+        self._keep_fix = fx
+        return fx.ctypes.data_as(C.POINTER(C.c_int32)), len(fx)
+
+    def compute_single_ptcle_gf(self, G, topology, f_ix=None):
+        """compute_single_ptcle_gf(G, topology[, f_ix | f_ix_vec]) (block_sparse_backbone.cpp:779-825):
+        (r, n, n) real, indexed by orbital, without the order/topology sign the solver applies.
+        G None uses the G already on the device (set_G or the previous call)."""
+        g = None if G is None else dptr(self.pack(G))
+        topo = as_i32(topology)
+        fx, nf = self._fix(f_ix)
+        out = np.zeros((self.r, self.n, self.n))
+        check(lib().xca_spgf(self._h, g, topo.shape[0], iptr(topo), fx, nf, dptr(out)))
+        return out
+
+    def compute_one_time_correlator(self, G, ops_tau, ops_0, topology, f_ix=None):
+        """compute_one_time_correlator(G, ops_tau, ops_0, topology, f_ix_vec)
+        (block_sparse_backbone.cpp:850-876) for block operators given as (conn (nops, nblocks),
+        packed matrices in (operator, source block) order): (r, len(ops_tau), len(ops_0)) real."""
+        g = None if G is None else dptr(self.pack(G))
+        topo = as_i32(topology)
+        fx, nf = self._fix(f_ix)
+        mc, mm = as_i32(ops_tau[0]), _real(ops_tau[1], "ops_tau")
+        kc, km = as_i32(ops_0[0]), _real(ops_0[1], "ops_0")
+        if mc.ndim != 2 or kc.ndim != 2 or mc.shape[1] != len(self.dims) or kc.shape[1] != len(self.dims):
+            raise XcaError("operator connections must have shape (nops, nblocks)")
+        out = np.zeros((self.r, mc.shape[0], kc.shape[0]))
+        check(lib().xca_correlator(self._h, g, topo.shape[0], iptr(topo), fx, nf, mc.shape[0], iptr(mc), dptr(mm), kc.shape[0],
+                                   iptr(kc), dptr(km), dptr(out)))
+        return out
+
+    def single_particle_greens_function(self, G, max_order):
+        """eval_single_particle_greens_function (block_sparse_solver.py:657-677): the sum over orders
+        1..max_order and connected topologies of (-1)^order * s_topo * compute_single_ptcle_gf."""
+        tot = np.zeros((self.r, self.n, self.n))
+        self.set_G(G)
+        for order in range(1, max_order + 1):
+            for sign, topo in all_connected_pairings(order):
+                tot += (-1) ** order * sign * self.compute_single_ptcle_gf(None, np.array(topo, dtype=np.int32))
+        return tot
+
+    def one_time_correlator(self, G, max_order, ops_tau, ops_0):
+        """eval_one_time_correlator (block_sparse_solver.py:700-737): <T O(tau) O(0)> summed over
+        orders and topologies with (-1)^order * s_topo."""
+        tot = None
+        self.set_G(G)
+        for order in range(1, max_order + 1):
+            for sign, topo in all_connected_pairings(order):
+                c = (-1) ** order * sign * self.compute_one_time_correlator(None, ops_tau, ops_0, np.array(topo, dtype=np.int32))
+                tot = c if tot is None else tot + c
+        return tot
+
     # ---------------------------------------------------------------- device-resident form
     def set_G(self, G):
         g = self.pack(G)

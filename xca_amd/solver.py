@@ -29,6 +29,11 @@ class BlockSparseSolver:
         self.G, self.eta, self.n_ppsc_iter, self.diff_G = G, eta, it, diff
         return self
 
+    def single_particle_greens_function(self, max_order):
+        """g(tau) from the current G (block_sparse_solver.py:653-665): (r, n, n) at the DLR nodes."""
+        self.spgf = self.d.single_particle_greens_function(self.G, max_order)
+        return self.spgf
+
     def partition_function(self):
         """Z = -Tr G(beta) exp(-beta eta)."""
         from .dlr import kernel
