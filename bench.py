@@ -219,6 +219,7 @@ def main():
     ap.add_argument("--solves", type=int, default=50, help="NCA self-consistent solves timed for the latency figures (0: skip)")
     ap.add_argument("--serving", type=int, default=512, help="requests in the served stream (0: skip)")
     ap.add_argument("--spgf", type=int, default=5, help="timed single-particle Green's function evaluations (0: skip)")
+    ap.add_argument("--spgf-only", action="store_true", help="run only the single-particle Green's function leg (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (reference dispatches for the HBM byte counters)")
     args = ap.parse_args()
@@ -236,6 +237,9 @@ def main():
     P, max_order = build_workload(args.workload)
     G = P.perturbed_G()
     ev = DiagramEvaluator(P.poles, P.coeffs, P.grid, P.ad)
+    if args.spgf_only:
+        print(json.dumps({"spgf": spgf_leg(ev, P, G, max_order, max(args.spgf, 1), cpu=False)}))
+        return
     ev.set_G(G)
     topos = [(np.array(t, dtype=np.int32), (-1) ** o * s * (-1) ** (o + 1)) for o in range(1, max_order + 1)
              for s, t in all_connected_pairings(o)]

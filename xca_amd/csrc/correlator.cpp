@@ -90,7 +90,7 @@ int64_t store_budget() {
     const long v = std::atol(e);
     if (v > 0) return (int64_t)v * (1 << 20) / 8;
   }
-  return (int64_t)1 << 27;  // 1 GiB
+  return (int64_t)1 << 29;  // 4 GiB
 }
 
 // one host group with its tuple selection: first-line poles [a, b) of the class enumeration, or the
@@ -114,11 +114,16 @@ static CorrDevPlan *get_corr_plan(xca_ctx *c, int m, const int *topo, const int3
   key.insert(key.end(), topo, topo + 2 * m);
   key.insert(key.end(), c->G_zero.begin(), c->G_zero.end());
   key.push_back(spgf ? -2 : -3);
-  if (!spgf) {
+  if (!spgf) {  // connections and the nonzero pattern of the operators
     key.push_back(co.nmu);
     key.push_back(co.nkap);
     key.insert(key.end(), co.mu_conn.begin(), co.mu_conn.end());
     key.insert(key.end(), co.kap_conn.begin(), co.kap_conn.end());
+    for (size_t x = 0; x < co.mats.size(); x += 32) {
+      int bits = 0;
+      for (size_t y = x; y < std::min(co.mats.size(), x + 32); ++y) bits |= (co.mats[y] != 0.0 ? 1 : 0) << (y - x);
+      key.push_back(bits);
+    }
   }
   if (fix) {
     key.push_back(-1);
@@ -248,6 +253,29 @@ static CorrDevPlan *get_corr_plan(xca_ctx *c, int m, const int *topo, const int3
     }
   }
 
+  // operator blocks as sparse rows: cidx holds row pointers then column indices, cval_src the
+  // position in co.mats of every stored value
+  std::map<int64_t, std::pair<int, int>> csr_of;  // dense offset -> (cidx offset, value offset)
+  std::vector<int> h_cidx;
+  auto csr = [&](int64_t dense, int rows, int cols) {
+    auto f = csr_of.find(dense);
+    if (f != csr_of.end()) return f->second;
+    std::pair<int, int> o{(int)h_cidx.size(), (int)dp->cval_src.size()};
+    std::vector<int> colv;
+    h_cidx.push_back(0);
+    for (int i = 0; i < rows; ++i) {
+      for (int j = 0; j < cols; ++j)
+        if (co.mats[dense + (int64_t)i * cols + j] != 0.0) {
+          colv.push_back(j);
+          dp->cval_src.push_back(dense + (int64_t)i * cols + j);
+        }
+      h_cidx.push_back((int)colv.size());
+    }
+    h_cidx.insert(h_cidx.end(), colv.begin(), colv.end());
+    csr_of[dense] = o;
+    return o;
+  };
+
   // batches
   int64_t max_store = 0, max_partial = 0;
   size_t gi0 = 0;
@@ -271,7 +299,7 @@ static CorrDevPlan *get_corr_plan(xca_ctx *c, int m, const int *topo, const int3
     std::vector<RJob> rjobs;
     int64_t off = 0, batch_items = 0;
     for (size_t k = gi0; k < gi1; ++k) batch_items += dgroups[k].ntup;
-    const int kc = (int)std::max<int64_t>(1, std::min<int64_t>(64, batch_items / 2048));
+    const int kc = (int)std::max<int64_t>(1, std::min<int64_t>(64, batch_items / 4096));
     int64_t est_tiles = 0;
     for (size_t k = gi0; k < gi1; ++k) {
       const DevGroup &dg = dgroups[k];
@@ -312,13 +340,17 @@ static CorrDevPlan *get_corr_plan(xca_ctx *c, int m, const int *topo, const int3
         CTerm ct{};
         ct.r_off = out_of[u.rprog];
         ct.l_off = out_of[u.lprog];
-        ct.mu_off = co.mu_off[(size_t)u.mu * nb + u.e0];
-        ct.kap_off = co.kap_off[(size_t)u.kap * nb + u.c];
         ct.d0 = P.dims[u.e0];
         ct.d1 = P.dims[u.e1];
         ct.db = P.dims[u.b];
         ct.dc = P.dims[u.c];
-        if (ct.mu_off < 0 || ct.kap_off < 0) throw std::logic_error("missing correlator operator block");
+        const int64_t mo = co.mu_off[(size_t)u.mu * nb + u.e0], ko = co.kap_off[(size_t)u.kap * nb + u.c];
+        if (mo < 0 || ko < 0) throw std::logic_error("missing correlator operator block");
+        const auto cm = csr(mo, ct.d1, ct.d0), ck = csr(ko, ct.db, ct.dc);
+        ct.mu_csr = cm.first;
+        ct.mu_val = cm.second;
+        ct.kap_csr = ck.first;
+        ct.kap_val = ck.second;
         const int out = u.mu * co.nkap + u.kap;
         if (runs.size() > (size_t)run0 && runs.back().out == out) runs.back().t1 = (int)terms.size() + 1;
         else runs.push_back(CRun{out, (int)terms.size(), (int)terms.size() + 1, 0});
@@ -360,10 +392,11 @@ static CorrDevPlan *get_corr_plan(xca_ctx *c, int m, const int *topo, const int3
   dp->ftab.alloc(std::max<int64_t>(hp.ftab_size, 1));
   dp->store.alloc(std::max<int64_t>(max_store, 1));
   dp->partial.alloc(std::max<int64_t>(max_partial, 1));
-  dp->cops.alloc(std::max<size_t>(co.mats.size(), 1));
+  dp->cidx.upload(h_cidx, s);
+  dp->cval.alloc(std::max<size_t>(dp->cval_src.size(), 1));
   dp->queue.alloc(1);
   dp->device_bytes += (int64_t)(dp->ops.n * sizeof(DOp) + dp->tabjobs.n * sizeof(DTabJob)) +
-                      8 * (int64_t)(dp->ftab.n + dp->store.n + dp->partial.n + dp->cops.n);
+                      8 * (int64_t)(dp->ftab.n + dp->store.n + dp->partial.n + dp->cval.n) + 4 * (int64_t)dp->cidx.n;
   // launch descriptors
   for (auto &B : dp->batches) {
     ChainArgs A{};
@@ -411,7 +444,9 @@ static void corr_eval(xca_ctx *c, const double *G, int m, const int *topo, const
   const int r = P.r, nout = dp->nmu * dp->nkap;
   hipStream_t s = c->stream;
   c->corr_out.alloc((size_t)r * nout);
-  if (!co.mats.empty()) HIP_CHECK(hipMemcpyAsync(dp->cops.ptr, co.mats.data(), co.mats.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  std::vector<double> cval(dp->cval_src.size());
+  for (size_t x = 0; x < cval.size(); ++x) cval[x] = co.mats[dp->cval_src[x]];
+  if (!cval.empty()) HIP_CHECK(hipMemcpyAsync(dp->cval.ptr, cval.data(), cval.size() * sizeof(double), hipMemcpyHostToDevice, s));
   if (dp->ntabjobs > 0) HIP_CHECK(launch_tab(c->RB, TabBatch{dp->tseg.ptr, dp->tseg_prefix.ptr, 1, dp->ntabjobs}, s));
   std::vector<hipEvent_t> ev;
   for (size_t bi = 0; bi < dp->batches.size(); ++bi) {
@@ -425,7 +460,7 @@ static void corr_eval(xca_ctx *c, const double *G, int m, const int *topo, const
     HIP_CHECK(launch_chain(c->RB, dp->variant, ChainBatch{B.seg.ptr, B.seg_prefix.ptr, 1, B.nchunks, dp->queue.ptr, nullptr, 0}, s));
     HIP_CHECK(hipEventRecord(e1, s));
     HIP_CHECK(launch_refl(c->RB, ReflArgs{c->D, B.rjobs.ptr, dp->store.ptr, B.nrjobs, 0}, s));
-    CorrArgs A{B.terms.ptr, B.runs.ptr, B.cchunks.ptr, dp->store.ptr, dp->cops.ptr, dp->partial.ptr, B.ncchunks, r, nout, 0};
+    CorrArgs A{B.terms.ptr, B.runs.ptr, B.cchunks.ptr, dp->store.ptr, dp->cval.ptr, dp->cidx.ptr, dp->partial.ptr, B.ncchunks, r, nout, 0};
     HIP_CHECK(launch_corr(A, s));
     HIP_CHECK(launch_corr_reduce(dp->partial.ptr, B.ncchunks, r * nout, c->corr_out.ptr, bi > 0 ? 1 : 0, s));
   }

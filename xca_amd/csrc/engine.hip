@@ -289,7 +289,8 @@ __global__ void __launch_bounds__(64 * NW) tab_kernel(TabBatch B) {
 // -------------------------------------------------------------------------------------------
 // Reflection t -> 1 - t of stored correlator outputs, in place: X <- E U^{-1} (L^{-1} P) X along
 // the time index for every column of a job (values at the reflected nodes from the coefficients).
-// One wave owns 16 columns; columns of different jobs never overlap.
+// Column c of a job is the r consecutive doubles at base + (c0 + c) r. One wave owns 16 columns;
+// columns of different jobs never overlap.
 template <int RB, int NW>
 __global__ void __launch_bounds__(64 * NW) refl_kernel(ReflArgs A) {
   constexpr int RP = 16 * RB, NT = 64 * NW, LDN = 16 * NW + 8;
@@ -298,12 +299,12 @@ __global__ void __launch_bounds__(64 * NW) refl_kernel(ReflArgs A) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const DlrFrag D = A.D;
   const RJob J = A.jobs[blockIdx.x];
-  const int r = D.r, w = J.w, ncol = J.ncol;
+  const int r = D.r, ncol = J.ncol;
   for (int x = tid; x < RP * LDN; x += NT) { SA[x] = 0.0; SB[x] = 0.0; }
   __syncthreads();
   for (int x = tid; x < r * ncol; x += NT) {
-    const int t = x / ncol, c = x % ncol, col = J.c0 + c;
-    SA[(size_t)t * LDN + c] = A.store[J.base + ((size_t)(col / w) * r + t) * w + col % w];
+    const int c = x / r, t = x % r;
+    SA[(size_t)t * LDN + c] = A.store[J.base + (size_t)(J.c0 + c) * r + t];
   }
   __syncthreads();
   v4d acc[RB];
@@ -316,14 +317,13 @@ __global__ void __launch_bounds__(64 * NW) refl_kernel(ReflArgs A) {
   gemm_acc<RB>(D.E, 4 * RB, 0, (r + 3) >> 2, SB, LDN, c0, acc, lane);
   const int c = c0 + (lane & 15);
   if (c < ncol) {
-    const int col = J.c0 + c;
-    double *out = A.store + J.base + (size_t)(col / w) * r * w + col % w;
+    double *out = A.store + J.base + (size_t)(J.c0 + c) * r;
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int t = acc_row(rb, v, lane);
-        if (t < r) out[(size_t)t * w] = acc[rb][v];
+        if (t < r) out[t] = acc[rb][v];
       }
   }
 }
@@ -622,12 +622,12 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
         __syncthreads();
       }
     }
-    if (P.out_off >= 0) {  // store mode: every item's T with its prefactor
-      const int w = rows * cols, rw = r * w;
-      double *o = A.store + P.out_off + (size_t)tstart * rw;
-      for (int x = tid; x < P_items * rw; x += NT) {
-        const int it = x / rw, rem = x % rw;
-        o[x] = Scur[(size_t)(rem / w) * LDN + it * w + rem % w] * ipref[it];
+    if (P.out_off >= 0) {  // store mode: every item's T with its prefactor, time index fastest
+      const int w = rows * cols;
+      double *o = A.store + P.out_off + (size_t)tstart * r * w;
+      for (int x = tid; x < P_items * w * r; x += NT) {
+        const int c = x / r, t = x % r;  // c = item * w + element
+        o[x] = Scur[(size_t)t * LDN + c] * ipref[c / w];
       }
       continue;
     }
@@ -713,60 +713,77 @@ __global__ void reset_queue_kernel(int *q) {
   if (threadIdx.x == 0) q[0] = 0;
 }
 
-// One-time correlator contraction. Workgroup = chunk (pole tuples [k0, k1) of one group); thread =
-// (output run, t). A thread sums its run's terms Tr[L O_mu R O_kap](t) = sum_i sum_a (O_mu R)_ia
-// (L^T O_kap^T)_ia over the chunk's tuples in a fixed order, so the result does not depend on the
-// schedule. Every chunk writes its whole (r, nout) slot.
+// One-time correlator contraction. Workgroup = chunk (pole tuples [k0, k1) of one group), wave = one
+// output run at a time, lane = time node t. Per term and tuple a lane adds
+//   Tr[L O_mu R O_kap](t) = sum_i sum_a (O_mu R)_ia (L^T O_kap^T)_ia
+// with the operators as sparse rows (wave-uniform, scalar loads) and R, L^T read with the time index
+// fastest (coalesced). Runs are summed over the chunk's tuples in a fixed order, so the result does
+// not depend on the schedule; every chunk writes its whole (r, nout) slot.
 __global__ void __launch_bounds__(256) corr_kernel(CorrArgs A) {
-  const CChunk ch = A.chunks[blockIdx.x];
-  const int r = A.r, nrun = ch.run1 - ch.run0;
+  const CChunk ch = uniform_load(A.chunks + blockIdx.x);
+  const int r = A.r, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, t = lane;
   double *slot = A.partial + (size_t)blockIdx.x * r * A.nout;
-  for (int x = threadIdx.x; x < r * A.nout; x += blockDim.x) slot[x] = 0.0;
+  for (int x = tid; x < r * A.nout; x += blockDim.x) slot[x] = 0.0;
   __syncthreads();
-  for (int x = threadIdx.x; x < nrun * r; x += blockDim.x) {
-    const int t = x % r;
-    const CRun run = A.runs[ch.run0 + x / r];
+  for (int ru = ch.run0 + wv; ru < ch.run1; ru += 4) {
+    const CRun run = uniform_load(A.runs + ru);
     double sum = 0.0;
-    for (int k = ch.k0; k < ch.k1; ++k)
-      for (int q = run.t0; q < run.t1; ++q) {
-        const CTerm T = A.terms[q];
-        const int d0 = T.d0, d1 = T.d1, db = T.db, dc = T.dc;
-        const double *R = A.store + T.r_off + ((size_t)k * r + t) * d0 * db;
-        const double *L = A.store + T.l_off + ((size_t)k * r + t) * d1 * dc;
-        const double *Mu = A.cops + T.mu_off, *Kp = A.cops + T.kap_off;
+    for (int q = run.t0; q < run.t1; ++q) {
+      const CTerm T = uniform_load(A.terms + q);
+      const int d0 = T.d0, d1 = T.d1, db = T.db, dc = T.dc;
+      const int *mp = A.cidx + T.mu_csr, *mc = mp + d1 + 1;
+      const int *kp = A.cidx + T.kap_csr, *kc = kp + db + 1;
+      const double *mv = A.cval + T.mu_val, *kv = A.cval + T.kap_val;
+      for (int k = ch.k0; k < ch.k1; ++k) {
+        if (t >= r) break;
+        const double *R = A.store + T.r_off + (size_t)k * d0 * db * r + t;  // element (x, a) at R[(x db + a) r]
+        const double *L = A.store + T.l_off + (size_t)k * d1 * dc * r + t;  // element (i, j) at L[(i dc + j) r]
         for (int i = 0; i < d1; ++i) {
-          double nr[kMaxB], qr[kMaxB];
+          double nr[kMaxB];
 #pragma unroll
-          for (int a = 0; a < kMaxB; ++a) nr[a] = qr[a] = 0.0;
-          for (int kk = 0; kk < d0; ++kk) {
-            const double m = Mu[i * d0 + kk];
-#pragma unroll
-            for (int a = 0; a < kMaxB; ++a)
-              if (a < db) nr[a] += m * R[kk * db + a];
-          }
-          for (int j = 0; j < dc; ++j) {
-            const double l = L[i * dc + j];
+          for (int a = 0; a < kMaxB; ++a) nr[a] = 0.0;
+          const int e0 = __builtin_amdgcn_readfirstlane(mp[i]), e1 = __builtin_amdgcn_readfirstlane(mp[i + 1]);
+          for (int e = e0; e < e1; ++e) {
+            const double v = mv[e];
+            const double *Rx = R + (size_t)__builtin_amdgcn_readfirstlane(mc[e]) * db * r;
 #pragma unroll
             for (int a = 0; a < kMaxB; ++a)
-              if (a < db) qr[a] += l * Kp[a * dc + j];
+              if (a < db) nr[a] += v * Rx[(size_t)a * r];
           }
+          const double *Li = L + (size_t)i * dc * r;
 #pragma unroll
           for (int a = 0; a < kMaxB; ++a)
-            if (a < db) sum += nr[a] * qr[a];
+            if (a < db) {
+              double qv = 0.0;
+              const int f0 = __builtin_amdgcn_readfirstlane(kp[a]), f1 = __builtin_amdgcn_readfirstlane(kp[a + 1]);
+              for (int e = f0; e < f1; ++e) qv += kv[e] * Li[(size_t)__builtin_amdgcn_readfirstlane(kc[e]) * r];
+              sum += nr[a] * qv;
+            }
         }
       }
-    slot[(size_t)t * A.nout + run.out] = sum;
+    }
+    if (t < r) slot[(size_t)t * A.nout + run.out] = sum;
   }
 }
 
-// out[e] (=, or += when accumulate) sum over chunks c, in order, of partial[c][e]
+// out[e] (=, or += when accumulate) + sum over chunks of partial[c][e]: thread (e, g) of a 16 x 16
+// workgroup sums chunks g, g + 16, ... of element e, and the 16 group sums are added in a fixed order.
 __global__ void __launch_bounds__(256) corr_reduce_kernel(const double *__restrict__ partial, int nchunks, int n, double *out,
                                                           int accumulate) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  double s = accumulate ? out[e] : 0.0;
-  for (int c = 0; c < nchunks; ++c) s += partial[(size_t)c * n + e];
-  out[e] = s;
+  __shared__ double part[16][17];
+  const int e = threadIdx.x & 15, g = threadIdx.x >> 4;
+  const int x = blockIdx.x * 16 + e;
+  double s = 0.0;
+  if (x < n)
+    for (int c = g; c < nchunks; c += 16) s += partial[(size_t)c * n + x];
+  part[g][e] = s;
+  __syncthreads();
+  if (g == 0 && x < n) {
+    double acc = accumulate ? out[x] : 0.0, tt = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) tt += part[q][e];
+    out[x] = acc + tt;
+  }
 }
 
 // -------------------------------------------------------------------------------------------
@@ -933,7 +950,7 @@ hipError_t launch_corr(const CorrArgs &A, hipStream_t s) {
 
 hipError_t launch_corr_reduce(const double *partial, int nchunks, int n, double *out, int accumulate, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(corr_reduce_kernel, dim3((n + 255) / 256), dim3(256), 0, s, partial, nchunks, n, out, accumulate);
+  hipLaunchKernelGGL(corr_reduce_kernel, dim3((n + 15) / 16), dim3(256), 0, s, partial, nchunks, n, out, accumulate);
   return hipGetLastError();
 }
 

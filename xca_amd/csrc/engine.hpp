@@ -133,10 +133,13 @@ struct RedBatch {
 
 // One-time correlator contraction. A term adds Tr[L(t) O_mu R(t) O_kap] for every pole tuple of its
 // group, with R (d0 x db) the [tau, 0] side, L^T (d1 x dc) the reflected [beta, tau] side stored
-// transposed, O_mu (d1 x d0) and O_kap (db x dc).
+// transposed, O_mu (d1 x d0) and O_kap (db x dc). Stored items are (rows, cols, r) with the time
+// index fastest. Operators are sparse rows: row pointers (rows + 1) then column indices in cidx,
+// values in cval.
 struct CTerm {
   int64_t r_off, l_off;      // item 0 of the right / left program in the store (doubles)
-  int64_t mu_off, kap_off;   // operator blocks in the correlator operator buffer
+  int mu_csr, kap_csr;       // row pointers of O_mu / O_kap in cidx
+  int mu_val, kap_val;       // their values in cval
   int d0, d1, db, dc;
 };
 
@@ -150,8 +153,8 @@ struct CChunk {
   int run0, run1, k0, k1;
 };
 
-// Columns [c0, c0 + ncol) of a stored block of items (r, w) each at base: column c is element c % w
-// of item c / w.
+// Columns [c0, c0 + ncol) of a stored program output at base; column c holds r doubles at
+// base + c r (item c / w, element c % w).
 struct RJob {
   int64_t base;
   int w, c0, ncol, pad0;
@@ -168,7 +171,8 @@ struct CorrArgs {
   const CTerm *terms;
   const CRun *runs;
   const CChunk *chunks;
-  const double *store, *cops;
+  const double *store, *cval;
+  const int *cidx;
   double *partial;   // (nchunks, r, nout)
   int nchunks, r, nout, pad0;
 };

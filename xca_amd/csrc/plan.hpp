@@ -98,8 +98,8 @@ struct CorrBatchDev {
 struct CorrDevPlan {
   DevBuf<DOp> ops;
   DevBuf<DTabJob> tabjobs;
-  DevBuf<double> ftab, store, partial, cops;
-  DevBuf<int> queue;
+  DevBuf<double> ftab, store, partial, cval;
+  DevBuf<int> queue, cidx;
   DevBuf<TabSeg> tseg;
   DevBuf<int> tseg_prefix;
   std::vector<std::unique_ptr<CorrBatchDev>> batches;
@@ -108,6 +108,7 @@ struct CorrDevPlan {
   double macs = 0;         // canonical multiply-adds of the chains and the contraction
   int64_t device_bytes = 0;
   std::vector<DTabJob> h_tab;
+  std::vector<int64_t> cval_src;  // operator value x sits at position cval_src[x] of the call's matrices
 };
 
 DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t nfix);
