@@ -159,6 +159,70 @@ def kanamori_spins(norb):
     return np.array([-1] * norb + [1] * norb)
 
 
+def wigner_3j(j1, j2, j3, m1, m2, m3):
+    """Wigner 3j symbol for integer arguments (Racah's formula)."""
+    from math import factorial, sqrt
+    if m1 + m2 + m3 != 0 or abs(m1) > j1 or abs(m2) > j2 or abs(m3) > j3:
+        return 0.0
+    if j3 < abs(j1 - j2) or j3 > j1 + j2:
+        return 0.0
+    f = factorial
+    tri = f(j1 + j2 - j3) * f(j1 - j2 + j3) * f(-j1 + j2 + j3) / f(j1 + j2 + j3 + 1)
+    pre = sqrt(tri * f(j1 + m1) * f(j1 - m1) * f(j2 + m2) * f(j2 - m2) * f(j3 + m3) * f(j3 - m3))
+    s = 0.0
+    for k in range(0, j1 + j2 + j3 + 1):
+        d = [k, j1 + j2 - j3 - k, j1 - m1 - k, j2 + m2 - k, j3 - j2 + m1 + k, j3 - j1 - m2 + k]
+        if min(d) < 0:
+            continue
+        s += (-1) ** k / (f(d[0]) * f(d[1]) * f(d[2]) * f(d[3]) * f(d[4]) * f(d[5]))
+    return (-1) ** (j1 - j2 - m3) * pre * s
+
+
+def slater_u_matrix(l, F):
+    """Slater-Condon U_{m1 m2 m3 m4} = sum_k a_k(m1, m2, m3, m4) F^k in the complex spherical-harmonic
+    basis (m = -l..l), with a_k = (2l+1)^2 (l k l; 0 0 0)^2 sum_q (-1)^(m1+m2+q)
+    (l k l; -m1 q m3)(l k l; -m2 -q m4), k = 0, 2, .., 2l (the standard Gaunt-coefficient form; real)."""
+    n = 2 * l + 1
+    ms = range(-l, l + 1)
+    U = np.zeros((n, n, n, n))
+    for ik, k in enumerate(range(0, 2 * l + 1, 2)):
+        ck = (2 * l + 1) ** 2 * wigner_3j(l, k, l, 0, 0, 0) ** 2
+        for a, m1 in enumerate(ms):
+            for b, m2 in enumerate(ms):
+                for c, m3 in enumerate(ms):
+                    for d, m4 in enumerate(ms):
+                        s = 0.0
+                        for q in range(-k, k + 1):
+                            s += (-1) ** (m1 + m2 + q) * wigner_3j(l, k, l, -m1, q, m3) * wigner_3j(l, k, l, -m2, -q, m4)
+                        U[a, b, c, d] += ck * s * F[ik]
+    return U
+
+
+def slater_hamiltonian(fs, l, F, mu):
+    """H = 1/2 sum U_{m1 m2 m3 m4} c^dag_{m1 s} c^dag_{m2 s'} c_{m4 s'} c_{m3 s} - mu N for one l shell,
+    flavours ordered (do, m = -l..l), (up, m = -l..l) (h_int_slater with off-diagonal terms, as used at
+    benchmarks/slater_condon/slater_condon_interaction.py:50-59)."""
+    n = 2 * l + 1
+    U = slater_u_matrix(l, F)
+    cd, c = fs.cdag, fs.c
+    H = sp.csr_matrix((fs.dim, fs.dim))
+    for s1 in range(2):
+        for s2 in range(2):
+            for a in range(n):
+                for b in range(n):
+                    for cc in range(n):
+                        for d in range(n):
+                            u = U[a, b, cc, d]
+                            if abs(u) < 1e-14:
+                                continue
+                            A, B, C, D = s1 * n + a, s2 * n + b, s1 * n + cc, s2 * n + d
+                            if A == B or C == D:
+                                continue
+                            H = H + 0.5 * u * (cd(A) @ cd(B) @ c(D) @ c(C))
+    N = sum(fs.nop(o) for o in range(2 * n))
+    return (H - mu * N).tocsr()
+
+
 # --------------------------------------------------------------------------- hybridization
 
 def semicircle_delta_tau(tau_abs, beta, D=1.0, t2=0.25, nquad=4000):

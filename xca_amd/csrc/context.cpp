@@ -541,6 +541,10 @@ TabSeg tab_seg(xca_ctx *c, DevPlan *dp) {
 
 void accumulate(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t nfix, double scale) {
   if (!c->have_G) throw std::runtime_error("xca: G has not been set");
+  if (c->wide) {
+    wide_accumulate(c, m, topo, fix, nfix, scale);
+    return;
+  }
   DevPlan *dp = get_plan(c, m, topo, fix, nfix);
   hipStream_t s = c->stream;
   c->last_items = dp->items;
@@ -665,6 +669,9 @@ xca_ctx *ctx_init_host(const xca_problem *desc) {
   c->RB = (P.r <= 32) ? 2 : 4;
   c->RP = 16 * c->RB;
   if (const char *tv = std::getenv("XCA_TILE_VARIANT")) c->tile_variant = std::atoi(tv);
+  // blocks above 12 rows go to the large-block engine (XCA_WIDE=1 forces it for any problem)
+  c->wide = *std::max_element(P.dims.begin(), P.dims.end()) > 12;
+  if (const char *w = std::getenv("XCA_WIDE")) c->wide = c->wide || std::atoi(w) != 0;
   if (c->tile_variant >= num_tile_variants()) c->tile_variant = -1;
   return c.release();
 }
@@ -771,6 +778,8 @@ extern "C" void xca_ctx_destroy(xca_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->plans.clear();
   c->corr_plans.clear();
+  c->wide_plans.clear();
+  c->widectx.reset();
   xca_dyson_release(c);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);

@@ -46,6 +46,8 @@ template <class T> struct DevBuf {
 
 struct DevPlan;
 struct CorrDevPlan;
+struct WidePlan;
+struct WideCtx;
 
 }  // namespace xca
 
@@ -73,6 +75,9 @@ struct xca_ctx {
   bool have_G = false;
   Plans plans;
   std::map<std::vector<int>, std::unique_ptr<xca::CorrDevPlan>> corr_plans;
+  bool wide = false;  // blocks above the tile engine's 12 rows: the large-block engine evaluates Sigma
+  std::map<std::vector<int>, std::unique_ptr<xca::WidePlan>> wide_plans;
+  std::unique_ptr<xca::WideCtx> widectx;
   xca::DevBuf<double> corr_out;
   XcaDyson *dyson = nullptr;
   int64_t last_items = 0;

@@ -8,6 +8,7 @@
 
 #include "ctx.hpp"
 #include "engine.hpp"
+#include "wide.hpp"
 
 namespace xca::dev {
 int num_tile_variants();
@@ -111,7 +112,25 @@ struct CorrDevPlan {
   std::vector<int64_t> cval_src;  // operator value x sits at position cval_src[x] of the call's matrices
 };
 
+// Large-block Σ plan: its launch list (edge tables first, then every program's ops and final
+// accumulation), the per-item operand offsets and prefactors, and the workspace.
+struct WidePlan {
+  std::vector<WLaunch> launches;
+  DevBuf<double> work, ftab, pref, vmax;
+  DevBuf<int64_t> offs;
+  int64_t items = 0;
+  double macs = 0;
+  int64_t device_bytes = 0;
+};
+
+// Problem-level DLR matrices of the large-block engine (plain row-major): L^{-1}P, U, W and
+// beta [cf2it | tcf2it].
+struct WideCtx {
+  DevBuf<double> LiP, U, W, M2;
+};
+
 DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t nfix);
+void wide_accumulate(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t nfix, double scale);
 int chunk_tiles(int64_t total_tiles);
 double canonical_macs(const Program &pg, int r);
 DOp device_op(const Op &op, const std::vector<EdgeTable> &tables, const xca_ctx *c);

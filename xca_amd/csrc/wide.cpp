@@ -1,0 +1,474 @@
+// Host orchestration of the large-block Σ engine. Programs come from the same block-path
+// enumeration as the tile engine (build_programs, following block_sparse_backbone.cpp:229-367); each
+// program runs over all its pole tuples at once as batched MFMA GEMMs with T resident in HBM:
+//   vertex      T[u](t) <- K(t, pole) F T[u](t)                 (batched over items and nodes)
+//   edge        y = (L^{-1}P) T, g = U^{-1} y, Wg = W g          (along the node index)
+//               c_k = -(Wf_k g_k + f_k Wg_k), d_k = f_k g_k      (batched over items and k)
+//               T <- beta [cf2it | tcf2it] [c; d]
+//   zero pair   T <- sum_mu F_mu T Fh_mu(t), Fh_mu(t) = sum_kap h(t, mu, kap) F_kap
+//   final       per-item max |T| (the reference's 1e-16 skip) and Sigma[b0] += scale sum pref T.
+// The launch list of a (topology, zero-block pattern, flat-index subset) is built once and replayed.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ctx.hpp"
+#include "host/model.hpp"
+#include "plan.hpp"
+
+using namespace xca;
+
+namespace {
+
+int64_t ipow(int64_t a, int e) {
+  int64_t r = 1;
+  for (int i = 0; i < e; ++i) r *= a;
+  return r;
+}
+
+void init_wide_ctx(xca_ctx *c) {
+  if (c->widectx) return;
+  const auto &o = c->M.ops;
+  const int r = o.r;
+  std::vector<double> LiP((size_t)r * r, 0.0), U((size_t)r * r, 0.0), M2((size_t)r * 2 * r, 0.0);
+  for (int i = 0; i < r; ++i) LiP[(size_t)i * r + i] = 1.0;
+  for (int k = 0; k < r; ++k)
+    if (o.piv[k] != k)
+      for (int j = 0; j < r; ++j) std::swap(LiP[(size_t)k * r + j], LiP[(size_t)o.piv[k] * r + j]);
+  for (int i = 1; i < r; ++i)
+    for (int k = 0; k < i; ++k) {
+      const double l = o.lu[(size_t)i * r + k];
+      if (l != 0.0)
+        for (int j = 0; j < r; ++j) LiP[(size_t)i * r + j] -= l * LiP[(size_t)k * r + j];
+    }
+  for (int i = 0; i < r; ++i)
+    for (int k = i; k < r; ++k) U[(size_t)i * r + k] = o.lu[(size_t)i * r + k];
+  for (int i = 0; i < r; ++i)
+    for (int k = 0; k < r; ++k) {
+      M2[(size_t)i * 2 * r + k] = c->M.P.beta * o.cf2it[(size_t)i * r + k];
+      M2[(size_t)i * 2 * r + r + k] = c->M.P.beta * o.tcf2it[(size_t)i * r + k];
+    }
+  auto w = std::make_unique<WideCtx>();
+  w->LiP.upload(LiP, c->stream);
+  w->U.upload(U, c->stream);
+  w->W.upload(o.W, c->stream);
+  w->M2.upload(M2, c->stream);
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->widectx = std::move(w);
+}
+
+WGemm gemm(const double *A, const double *B, double *C, int M, int N, int K, int lda, int ldb, int ldc, int nitem, int nt) {
+  WGemm g{};
+  g.A = A; g.B = B; g.C = C;
+  g.M = M; g.N = N; g.K = K; g.lda = lda; g.ldb = ldb; g.ldc = ldc;
+  g.nitem = nitem; g.nt = nt;
+  g.alpha = 1.0; g.beta = 0.0;
+  return g;
+}
+
+WLaunch of(const WGemm &g) {
+  WLaunch L{};
+  L.kind = W_GEMM;
+  L.g = g;
+  return L;
+}
+
+}  // namespace
+
+namespace xca {
+
+static WidePlan *get_wide_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t nfix) {
+  std::vector<int> key;
+  key.push_back(m);
+  key.insert(key.end(), topo, topo + 2 * m);
+  key.insert(key.end(), c->G_zero.begin(), c->G_zero.end());
+  if (fix) {
+    key.push_back(-1);
+    key.insert(key.end(), fix, fix + nfix);
+  }
+  auto found = c->wide_plans.find(key);
+  if (found != c->wide_plans.end()) return found->second.get();
+  if (m < 1 || m > kMaxLines + 1) throw std::invalid_argument("xca: diagram order must be between 1 and 4");
+  init_wide_ctx(c);
+  const Problem &P = c->M.P;
+  const int r = P.r, p = P.p, n = P.n, nl = m - 1;
+  Plan plan;
+  build_programs(c->M, c->ps, topo, m, 0, c->G_zero, plan);
+
+  // explicit flat-index subsets: (fb_ix, o_ix) -> requested p_ix
+  std::map<std::pair<int, int>, std::vector<int>> wanted;
+  const int64_t o_max = ipow(n, m - 1), p_max = ipow(p, m - 1), f_max = ipow(2, m) * o_max * p_max;
+  if (fix)
+    for (int64_t i = 0; i < nfix; ++i) {
+      const int64_t f = fix[i];
+      if (f < 0 || f >= f_max) throw std::runtime_error("DiagramEvaluator::eval_self_energy: f_ix out of range");
+      const int64_t rest = f / o_max;
+      wanted[{(int)(rest / p_max), (int)(f % o_max)}].push_back((int)(rest % p_max));
+    }
+
+  // items (pole tuples) of every program, first line slowest
+  struct ProgItems {
+    const Program *pg;
+    std::vector<std::vector<int>> poles;  // [item][line]
+  };
+  std::vector<ProgItems> progs;
+  int64_t max_ws = 0, max_fh = 0;  // workspace doubles per buffer, zero-pair operator sums
+  for (const Program &pg : plan.progs) {
+    int fb[kMaxLines + 1];
+    for (int i = 0, f = pg.fb_ix; i < m; ++i, f /= 2) fb[i] = f % 2;
+    std::vector<const std::vector<int> *> lists(nl);
+    for (int x = 0; x < nl; ++x) lists[x] = &c->ps.get(fb[x + 1], (pg.cls >> x) & 1);
+    ProgItems pi;
+    pi.pg = &pg;
+    if (!fix) {
+      int64_t ntup = 1;
+      for (int x = 0; x < nl; ++x) ntup *= (int64_t)lists[x]->size();
+      for (int64_t u = 0; u < ntup; ++u) {
+        std::vector<int> ls(nl);
+        int64_t v = u;
+        for (int x = nl - 1; x >= 0; --x) {
+          ls[x] = (*lists[x])[v % lists[x]->size()];
+          v /= (int64_t)lists[x]->size();
+        }
+        pi.poles.push_back(ls);
+      }
+    } else {
+      auto w = wanted.find({pg.fb_ix, pg.o_ix});
+      if (w == wanted.end()) continue;
+      for (int pix : w->second) {
+        std::vector<int> ls(nl);
+        int cls = 0, pp = pix;
+        for (int x = 0; x < nl; ++x) {
+          ls[x] = pp % p;
+          pp /= p;
+          const double wl = P.poles[ls[x]] * P.beta;
+          const int rb = fb[x + 1] ? (wl <= 0 ? 0 : 1) : (wl >= 0 ? 0 : 1);  // backbone.cpp:88,110
+          cls |= rb << x;
+        }
+        if (cls == pg.cls) pi.poles.push_back(ls);
+      }
+    }
+    if (pi.poles.empty()) continue;
+    const int64_t ni = (int64_t)pi.poles.size();
+    int rows = pg.b1, cols = pg.b1;
+    int64_t w = (int64_t)rows * cols;
+    for (const Op &op : pg.ops) {
+      w = std::max(w, std::max((int64_t)op.bin * cols, (int64_t)op.bout * cols));
+      if (op.type == OP_Z) {
+        w = std::max(w, std::max((int64_t)op.bin, (int64_t)op.bout) * op.cols);  // T Fh_mu and the result
+        max_fh = std::max(max_fh, (int64_t)op.qs * r * op.b1 * op.cols);
+        cols = op.cols;
+      }
+      rows = op.bout;
+    }
+    max_ws = std::max(max_ws, ni * r * w);
+    progs.push_back(std::move(pi));
+  }
+
+  auto wp = std::make_unique<WidePlan>();
+  // buffers: T0, T1, Y, WG (max_ws each), CD (2 max_ws), Fh, table scratch
+  int64_t max_tab = 0;
+  for (const auto &tb : plan.tables) max_tab = std::max(max_tab, tb.nentries * r * (int64_t)P.dims[tb.block] * P.dims[tb.block]);
+  const int64_t nwork = 6 * max_ws + max_fh + max_tab;
+  wp->work.alloc(std::max<int64_t>(nwork, 1));
+  double *T0 = wp->work.ptr, *T1 = T0 + max_ws, *Y = T1 + max_ws, *WG = Y + max_ws, *CD = WG + max_ws, *Fh = CD + 2 * max_ws,
+         *Tab = Fh + max_fh;
+  wp->ftab.alloc(std::max<int64_t>(plan.ftab_size, 1));
+  const WideCtx &W = *c->widectx;
+  double *G = c->G.ptr, *ftab = wp->ftab.ptr;
+
+  // edge tables: f = vals2coefs(G_b prod K) and W f (the tile engine's tab_kernel)
+  for (const auto &tb : plan.tables) {
+    const int b = P.dims[tb.block];
+    const int64_t bb = (int64_t)b * b;
+    const int ne = (int)tb.nentries;
+    WLaunch L{};
+    L.kind = W_GK;
+    L.src = G + c->M.G_off[tb.block];
+    L.dst = Tab;
+    L.n = bb;
+    L.nitem = ne;
+    L.nlines = tb.nlines;
+    for (int q = 0; q < 3; ++q) L.ls[q] = q < tb.nlines ? tb.lsign[q] : 0;
+    L.p = p;
+    wp->launches.push_back(L);
+    WGemm g = gemm(W.LiP.ptr, Tab, ftab + tb.offset, r, (int)bb, r, r, (int)bb, (int)bb, ne, 1);
+    g.b_si = r * bb;
+    g.c_si = 2 * r * bb;
+    wp->launches.push_back(of(g));
+    WLaunch S{};
+    S.kind = W_USOLVE;
+    S.dst = ftab + tb.offset;
+    S.ncol = (int)bb;
+    S.ld = (int)bb;
+    S.si = 2 * r * bb;
+    S.nitem = ne;
+    wp->launches.push_back(S);
+    WGemm gw = gemm(W.W.ptr, ftab + tb.offset, ftab + tb.offset + r * bb, r, (int)bb, r, r, (int)bb, (int)bb, ne, 1);
+    gw.b_si = gw.c_si = 2 * r * bb;
+    wp->launches.push_back(of(gw));
+  }
+
+  // per-item offsets and prefactors
+  std::vector<int64_t> offs;
+  std::vector<double> prefs;
+  int64_t nvmax = 0;
+  struct Pending {  // launches that take device pointers into offs / pref / vmax, fixed after upload
+    size_t launch;
+    int64_t a_off, s_off, pref, vmax;
+  };
+  std::vector<Pending> pend;
+  for (const ProgItems &pi : progs) {
+    const Program &pg = *pi.pg;
+    const int ni = (int)pi.poles.size();
+    // prefactor K(0)^(-e) per line and the sign (hyb.cpp:75-92, block_sparse_backbone.cpp:359-363)
+    const int64_t pref_off = (int64_t)prefs.size();
+    for (const auto &ls : pi.poles) {
+      double pr = (double)pg.sign;
+      for (int x = 0; x < nl; ++x)
+        if (pg.pk_exp[x] != 0) pr *= std::pow(pg.pk_sign[x] > 0 ? c->M.k0p[ls[x]] : c->M.k0m[ls[x]], -pg.pk_exp[x]);
+      prefs.push_back(pr);
+    }
+    WLaunch I{};
+    I.kind = W_INIT;
+    I.src = G + c->M.G_off[pg.init_block];
+    I.dst = T0;
+    I.n = (int64_t)r * pg.b1 * pg.b1;
+    I.nitem = ni;
+    wp->launches.push_back(I);
+    double *cur = T0, *oth = T1;
+    int cols = pg.b1;
+    for (const Op &op : pg.ops) {
+      if (op.type == OP_V) {
+        WGemm g = gemm(c->opmats.ptr + op.fbase, cur, oth, op.bout, cols, op.bin, op.bin, cols, cols, ni, r);
+        g.b_si = (int64_t)r * op.bin * cols;
+        g.b_st = (int64_t)op.bin * cols;
+        g.c_si = (int64_t)r * op.bout * cols;
+        g.c_st = (int64_t)op.bout * cols;
+        Pending pd{wp->launches.size(), -1, -1, -1, -1};
+        if (op.bar_line >= 0) {
+          pd.a_off = (int64_t)offs.size();
+          for (const auto &ls : pi.poles) offs.push_back((int64_t)ls[op.bar_line] * op.fstride_l);
+        }
+        if (op.ks != 0) {
+          g.S = op.ks > 0 ? c->kp.ptr : c->km.ptr;
+          pd.s_off = (int64_t)offs.size();
+          for (const auto &ls : pi.poles) offs.push_back((int64_t)ls[op.kx] * r);
+        }
+        wp->launches.push_back(of(g));
+        if (pd.a_off >= 0 || pd.s_off >= 0) pend.push_back(pd);
+        std::swap(cur, oth);
+      } else if (op.type == OP_E) {
+        const int b = op.bin;
+        const int64_t w = (int64_t)b * cols;
+        WGemm y = gemm(W.LiP.ptr, cur, Y, r, (int)w, r, r, (int)w, (int)w, ni, 1);
+        y.b_si = y.c_si = r * w;
+        wp->launches.push_back(of(y));
+        WLaunch S{};
+        S.kind = W_USOLVE;
+        S.dst = Y;
+        S.ncol = (int)w;
+        S.ld = (int)w;
+        S.si = r * w;
+        S.nitem = ni;
+        wp->launches.push_back(S);
+        WGemm wg = gemm(W.W.ptr, Y, WG, r, (int)w, r, r, (int)w, (int)w, ni, 1);
+        wg.b_si = wg.c_si = r * w;
+        wp->launches.push_back(of(wg));
+        // table entry (f part) of every item
+        const int64_t eoff = (int64_t)offs.size();
+        const auto &tb = plan.tables[op.etab];
+        const int64_t esz = 2 * (int64_t)r * b * b;
+        for (const auto &ls : pi.poles) {
+          int64_t e = tb.offset;
+          for (int q = 0; q < op.nlines; ++q) e += (int64_t)ls[op.lines[q]] * ipow(p, q) * esz;
+          offs.push_back(e);
+        }
+        // c = -(Wf g + f Wg), d = f g, per item and coefficient index k
+        auto local = [&](int64_t a_extra, const double *B, double *C, double alpha, double beta) {
+          WGemm g = gemm(ftab + a_extra, B, C, b, cols, b, b, cols, cols, ni, r);
+          g.a_st = (int64_t)b * b;
+          g.b_si = r * w;
+          g.b_st = w;
+          g.c_si = 2 * r * w;
+          g.c_st = w;
+          g.alpha = alpha;
+          g.beta = beta;
+          pend.push_back(Pending{wp->launches.size(), eoff, -1, -1, -1});
+          wp->launches.push_back(of(g));
+        };
+        local((int64_t)r * b * b, Y, CD, -1.0, 0.0);   // -Wf g
+        local(0, WG, CD, -1.0, 1.0);                     // -f Wg
+        local(0, Y, CD + r * w, 1.0, 0.0);               // f g
+        WGemm h = gemm(W.M2.ptr, CD, oth, r, (int)w, 2 * r, 2 * r, (int)w, (int)w, ni, 1);
+        h.b_si = 2 * r * w;
+        h.c_si = r * w;
+        wp->launches.push_back(of(h));
+        std::swap(cur, oth);
+      } else {  // zero-vertex pair (block_sparse_backbone.cpp:187-227)
+        const int bv = op.bin, bvp = op.bout, b1 = op.b1, b0 = op.cols, qs = op.qs;
+        WLaunch F{};
+        F.kind = W_FH;
+        F.dst = Fh;
+        F.src = c->opmats.ptr + op.fbase;
+        F.qs = qs;
+        F.b1 = b1;
+        F.b0 = b0;
+        F.forward = op.forward;
+        F.nflav = n;
+        for (int k = 0; k < qs; ++k) F.orb[k] = c->M.S.to_orb[op.set][k];
+        wp->launches.push_back(F);
+        for (int mu = 0; mu < qs; ++mu) {
+          WGemm g1 = gemm(cur, Fh + (int64_t)mu * r * b1 * b0, Y, bv, b0, b1, b1, b0, b0, ni, r);
+          g1.a_si = (int64_t)r * bv * b1;
+          g1.a_st = (int64_t)bv * b1;
+          g1.b_st = (int64_t)b1 * b0;
+          g1.c_si = (int64_t)r * bv * b0;
+          g1.c_st = (int64_t)bv * b0;
+          wp->launches.push_back(of(g1));
+          WGemm g2 = gemm(c->opmats.ptr + op.mubase + (int64_t)mu * bvp * bv, Y, oth, bvp, b0, bv, bv, b0, b0, ni, r);
+          g2.b_si = (int64_t)r * bv * b0;
+          g2.b_st = (int64_t)bv * b0;
+          g2.c_si = (int64_t)r * bvp * b0;
+          g2.c_st = (int64_t)bvp * b0;
+          g2.beta = mu == 0 ? 0.0 : 1.0;
+          wp->launches.push_back(of(g2));
+        }
+        cols = b0;
+        std::swap(cur, oth);
+      }
+    }
+    // final: skip rule and accumulation into the Sigma block
+    const int64_t nout = (int64_t)r * pg.b0 * pg.b0;
+    WLaunch X{};
+    X.kind = W_MAX;
+    X.src = cur;
+    X.n = nout;
+    X.nitem = ni;
+    pend.push_back(Pending{wp->launches.size(), -1, -1, -1, nvmax});
+    wp->launches.push_back(X);
+    WLaunch Fn{};
+    Fn.kind = W_FINAL;
+    Fn.src = cur;
+    Fn.n = nout;
+    Fn.nitem = ni;
+    Fn.dst = c->sigma.ptr + c->M.G_off[pg.sigma_block];
+    pend.push_back(Pending{wp->launches.size(), -1, -1, pref_off, nvmax});
+    wp->launches.push_back(Fn);
+    nvmax += ni;
+    wp->items += ni;
+    wp->macs += canonical_macs(pg, r) * (double)ni;
+  }
+  // every operand of every launch must stay inside its buffer: checked here, on the host, from the
+  // same strides and per-item offsets the kernels use
+  {
+    struct Ext {
+      const double *base;
+      int64_t n;
+    };
+    const Ext ext[] = {{c->G.ptr, (int64_t)c->G.n}, {c->sigma.ptr, (int64_t)c->sigma.n}, {c->opmats.ptr, (int64_t)c->opmats.n},
+                       {c->kp.ptr, (int64_t)c->kp.n}, {c->km.ptr, (int64_t)c->km.n}, {wp->work.ptr, (int64_t)wp->work.n},
+                       {wp->ftab.ptr, (int64_t)wp->ftab.n}, {W.LiP.ptr, (int64_t)W.LiP.n}, {W.W.ptr, (int64_t)W.W.n},
+                       {W.M2.ptr, (int64_t)W.M2.n}};
+    auto in = [&](const double *ptr, int64_t lo, int64_t hi, const char *what) {
+      for (const Ext &e : ext)
+        if (ptr >= e.base && ptr < e.base + e.n) {
+          if (ptr + lo < e.base || ptr + hi >= e.base + e.n) throw std::logic_error(std::string("wide plan: operand out of range: ") + what);
+          return;
+        }
+      throw std::logic_error(std::string("wide plan: operand outside every buffer: ") + what);
+    };
+    std::vector<std::pair<int64_t, int64_t>> aoff(wp->launches.size(), {0, 0}), soff(wp->launches.size(), {-1, -1});
+    for (const Pending &pd : pend) {
+      const WLaunch &L = wp->launches[pd.launch];
+      const int ni = L.kind == W_GEMM ? L.g.nitem : L.nitem;
+      if (pd.a_off >= 0) {
+        auto mm = std::minmax_element(offs.begin() + pd.a_off, offs.begin() + pd.a_off + ni);
+        aoff[pd.launch] = {*mm.first, *mm.second};
+      }
+      if (pd.s_off >= 0) {
+        auto mm = std::minmax_element(offs.begin() + pd.s_off, offs.begin() + pd.s_off + ni);
+        soff[pd.launch] = {*mm.first, *mm.second};
+      }
+      if (pd.pref >= 0 && pd.pref + ni > (int64_t)prefs.size()) throw std::logic_error("wide plan: prefactor range");
+      if (pd.vmax >= 0 && pd.vmax + ni > nvmax) throw std::logic_error("wide plan: vmax range");
+    }
+    for (size_t i = 0; i < wp->launches.size(); ++i) {
+      const WLaunch &L = wp->launches[i];
+      switch (L.kind) {
+        case W_GEMM: {
+          const WGemm &g = L.g;
+          const int64_t nu = g.nitem - 1, nt = g.nt - 1;
+          in(g.A, aoff[i].first, aoff[i].second + nu * g.a_si + nt * g.a_st + (int64_t)(g.M - 1) * g.lda + g.K - 1, "A");
+          in(g.B, 0, nu * g.b_si + nt * g.b_st + (int64_t)(g.K - 1) * g.ldb + g.N - 1, "B");
+          in(g.C, 0, nu * g.c_si + nt * g.c_st + (int64_t)(g.M - 1) * g.ldc + g.N - 1, "C");
+          if (g.S) in(g.S, soff[i].first, soff[i].second + nt, "S");
+          if (g.K > g.lda || g.N > g.ldb || g.N > g.ldc) throw std::logic_error("wide plan: leading dimension");
+          break;
+        }
+        case W_INIT:
+          in(L.src, 0, L.n - 1, "init src");
+          in(L.dst, 0, L.n * L.nitem - 1, "init dst");
+          break;
+        case W_USOLVE:
+          in(L.dst, 0, (int64_t)(L.nitem - 1) * L.si + (int64_t)(r - 1) * L.ld + L.ncol - 1, "usolve");
+          break;
+        case W_GK:
+          in(L.src, 0, (int64_t)r * L.n - 1, "gk src");
+          in(L.dst, 0, (int64_t)r * L.n * L.nitem - 1, "gk dst");
+          break;
+        case W_FH:
+          in(L.src, 0, (int64_t)L.qs * L.b1 * L.b0 - 1, "fh src");
+          in(L.dst, 0, (int64_t)L.qs * r * L.b1 * L.b0 - 1, "fh dst");
+          if (L.qs > 8) throw std::logic_error("wide plan: symmetry set above 8");
+          break;
+        case W_MAX:
+          in(L.src, 0, L.n * L.nitem - 1, "max src");
+          break;
+        case W_FINAL:
+          in(L.src, 0, L.n * L.nitem - 1, "final src");
+          in(L.dst, 0, L.n - 1, "final dst");
+          break;
+      }
+    }
+  }
+  hipStream_t s = c->stream;
+  wp->offs.upload(offs, s);
+  wp->pref.upload(prefs, s);
+  wp->vmax.alloc(std::max<int64_t>(nvmax, 1));
+  for (const Pending &pd : pend) {
+    WLaunch &L = wp->launches[pd.launch];
+    if (pd.a_off >= 0) L.g.a_off = wp->offs.ptr + pd.a_off;
+    if (pd.s_off >= 0) L.g.s_off = wp->offs.ptr + pd.s_off;
+    if (pd.pref >= 0) L.pref = wp->pref.ptr + pd.pref;
+    if (pd.vmax >= 0) L.vmax = wp->vmax.ptr + pd.vmax;
+  }
+  wp->device_bytes = 8 * (int64_t)(wp->work.n + wp->ftab.n + wp->pref.n + wp->vmax.n + wp->offs.n);
+  HIP_CHECK(hipStreamSynchronize(s));
+  WidePlan *raw = wp.get();
+  c->wide_plans[key] = std::move(wp);
+  return raw;
+}
+
+void wide_accumulate(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t nfix, double scale) {
+  WidePlan *wp = get_wide_plan(c, m, topo, fix, nfix);
+  const WideCtx &W = *c->widectx;
+  WArgs A{W.U.ptr, c->kp.ptr, c->km.ptr, c->delta.ptr, c->delta_refl.ptr, c->M.P.r, 0};
+  hipStream_t s = c->stream;
+  c->last_items = wp->items;
+  c->last_macs = wp->macs;
+  c->last_ms = 0;
+  c->last_variant = -1;
+  if (wp->items == 0) return;
+  HIP_CHECK(hipEventRecord(c->ev0, s));
+  for (const WLaunch &L : wp->launches) HIP_CHECK(launch_wide(L, A, scale, s));
+  HIP_CHECK(hipEventRecord(c->ev1, s));
+}
+
+}  // namespace xca

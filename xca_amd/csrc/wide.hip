@@ -1,0 +1,229 @@
+// Kernels of the large-block Σ engine: a batched FP64 MFMA GEMM with per-item operand offsets and a
+// per-(item, node) scale, the LU back substitution along the node index, edge-table inputs, the
+// zero-vertex operator sums, and the per-item skip test and Σ accumulation.
+#include <hip/hip_runtime.h>
+
+#include "wide.hpp"
+
+namespace xca::dev {
+
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+}  // namespace
+
+// 64 x 64 output tile per workgroup, 4 waves of 32 x 32 (2 x 2 MFMA blocks of 16 x 16), K staged
+// through LDS 16 at a time. MFMA f64 16x16x4 lane layout: A[i][k] in lane i + 16k, B[k][j] in lane
+// j + 16k, D(i, j) in lane j + 16 (i mod 4), register i / 4.
+__global__ void __launch_bounds__(256) wgemm_kernel(WGemm G) {
+  constexpr int TM = 64, TN = 64, TK = 16;
+  __shared__ double As[TK][TM + 1];
+  __shared__ double Bs[TK][TN + 1];
+  const int tilesN = (G.N + TN - 1) / TN;
+  const int tm = blockIdx.x / tilesN, tn = blockIdx.x % tilesN;
+  const int q = blockIdx.y, u = q / G.nt, t = q % G.nt;
+  const double *A = G.A + (G.a_off ? G.a_off[u] : 0) + (int64_t)u * G.a_si + (int64_t)t * G.a_st;
+  const double *B = G.B + (int64_t)u * G.b_si + (int64_t)t * G.b_st;
+  double *C = G.C + (int64_t)u * G.c_si + (int64_t)t * G.c_st;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int m0 = tm * TM, n0 = tn * TN;
+  const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
+  v4d acc[2][2];
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < G.K; k0 += TK) {
+    for (int x = tid; x < TM * TK; x += 256) {
+      const int i = x / TK, kk = x % TK, gi = m0 + i, gk = k0 + kk;
+      As[kk][i] = (gi < G.M && gk < G.K) ? A[(int64_t)gi * G.lda + gk] : 0.0;
+    }
+    for (int x = tid; x < TK * TN; x += 256) {
+      const int kk = x / TN, j = x % TN, gk = k0 + kk, gj = n0 + j;
+      Bs[kk][j] = (gk < G.K && gj < G.N) ? B[(int64_t)gk * G.ldb + gj] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < TK; kk += 4) {
+      const int kr = kk + (lane >> 4), c = lane & 15;
+      const double a0 = As[kr][wm + c], a1 = As[kr][wm + 16 + c];
+      const double b0 = Bs[kr][wn + c], b1 = Bs[kr][wn + 16 + c];
+      acc[0][0] = mfma(a0, b0, acc[0][0]);
+      acc[0][1] = mfma(a0, b1, acc[0][1]);
+      acc[1][0] = mfma(a1, b0, acc[1][0]);
+      acc[1][1] = mfma(a1, b1, acc[1][1]);
+    }
+    __syncthreads();
+  }
+  const double s = G.S ? G.S[G.s_off[u] + t] : 1.0;
+  const double al = G.alpha * s;
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = m0 + wm + 16 * bi + 4 * v + (lane >> 4), j = n0 + wn + 16 * bj + (lane & 15);
+        if (i < G.M && j < G.N) {
+          double *cp = C + (int64_t)i * G.ldc + j;
+          const double val = al * acc[bi][bj][v];
+          *cp = (G.beta != 0.0) ? val + G.beta * *cp : val;
+        }
+      }
+}
+
+// dst[u][x] = src[x] for x < n
+__global__ void winit_kernel(const double *__restrict__ src, double *__restrict__ dst, int64_t n, int nitem) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n * nitem) return;
+  dst[g] = src[g % n];
+}
+
+// U x = y per column, columns of 128 staged in LDS: X[u] holds r rows of ld doubles, columns
+// [0, ncol); U is r x r upper triangular (row-major).
+__global__ void __launch_bounds__(128) wusolve_kernel(const double *__restrict__ U, double *X, int r, int ncol, int ld, int64_t si, int nitem) {
+  __shared__ double Us[64 * 64];
+  __shared__ double Xs[64][129];
+  const int tid = threadIdx.x;
+  const int cblocks = (ncol + 127) / 128;
+  const int u = blockIdx.x / cblocks, c0 = (blockIdx.x % cblocks) * 128;
+  if (u >= nitem) return;
+  for (int x = tid; x < r * r; x += 128) Us[x] = U[x];
+  double *Xu = X + (int64_t)u * si;
+  const int nc = min(128, ncol - c0);
+  for (int x = tid; x < r * nc; x += 128) {
+    const int i = x / nc, j = x % nc;
+    Xs[i][j] = Xu[(int64_t)i * ld + c0 + j];
+  }
+  __syncthreads();
+  if (tid < nc) {
+    for (int i = r - 1; i >= 0; --i) {
+      double s = Xs[i][tid];
+      for (int k = i + 1; k < r; ++k) s -= Us[i * r + k] * Xs[k][tid];
+      Xs[i][tid] = s / Us[i * r + i];
+    }
+  }
+  __syncthreads();
+  for (int x = tid; x < r * nc; x += 128) {
+    const int i = x / nc, j = x % nc;
+    Xu[(int64_t)i * ld + c0 + j] = Xs[i][j];
+  }
+}
+
+// Edge-table inputs: dst[e] = G (r x n) times the K factors of the table's lines for entry e
+// (line q's pole = (e / p^q) mod p, sign ls[q])
+__global__ void wgk_kernel(const double *__restrict__ G, double *__restrict__ dst, int r, int64_t n, int nent, int nlines, int ls0, int ls1,
+                           int ls2, int p, const double *__restrict__ kp, const double *__restrict__ km) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)r * n;
+  if (g >= per * nent) return;
+  const int e = (int)(g / per);
+  const int64_t x = g % per;
+  const int t = (int)(x / n);
+  double v = G[x];
+  const int ls[3] = {ls0, ls1, ls2};
+  int ee = e;
+  for (int q = 0; q < nlines; ++q) {
+    const int l = ee % p;
+    ee /= p;
+    v *= (ls[q] > 0 ? kp : km)[(int64_t)l * r + t];
+  }
+  dst[g] = v;
+}
+
+// Zero-vertex operator sums: dst[mu][t] = sum_kap h(t, mu, kap) F_kap, h = Delta(t)[mu][kap] for a
+// forward line, -Delta_refl(t)[kap][mu] otherwise (block_sparse_backbone.cpp:216-221)
+__global__ void wfh_kernel(double *__restrict__ dst, const double *__restrict__ F, int qs, int b1, int b0, int r, int nflav, int forward,
+                           int o0, int o1, int o2, int o3, int o4, int o5, int o6, int o7, const double *__restrict__ delta,
+                           const double *__restrict__ delta_refl) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t w = (int64_t)b1 * b0;
+  if (g >= (int64_t)qs * r * w) return;
+  const int mu = (int)(g / (r * w));
+  const int t = (int)((g / w) % r);
+  const int64_t x = g % w;
+  const int orb[8] = {o0, o1, o2, o3, o4, o5, o6, o7};
+  double s = 0.0;
+  for (int k = 0; k < qs; ++k) {
+    const double h = forward ? delta[((int64_t)t * nflav + orb[mu]) * nflav + orb[k]] : -delta_refl[((int64_t)t * nflav + orb[k]) * nflav + orb[mu]];
+    s += h * F[(int64_t)k * w + x];
+  }
+  dst[g] = s;
+}
+
+// vmax[u] = max |src[u][x]|, one workgroup per item
+__global__ void __launch_bounds__(256) wmax_kernel(const double *__restrict__ src, int64_t n, double *vmax) {
+  __shared__ double red[256];
+  const int u = blockIdx.x;
+  double m = 0.0;
+  for (int64_t x = threadIdx.x; x < n; x += 256) m = fmax(m, fabs(src[(int64_t)u * n + x]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) vmax[u] = red[0];
+}
+
+// Sigma block += scale * sum over items, in order, of the kept items' pref * T
+__global__ void wfinal_kernel(const double *__restrict__ src, int64_t n, int nitem, const double *__restrict__ vmax,
+                              const double *__restrict__ pref, double *dst, double scale) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= n) return;
+  double s = 0.0;
+  for (int u = 0; u < nitem; ++u)
+    if (vmax[u] * fabs(pref[u]) > 1e-16) s += pref[u] * src[(int64_t)u * n + x];
+  dst[x] += scale * s;
+}
+
+hipError_t launch_wide(const WLaunch &L, const WArgs &A, double scale, hipStream_t s) {
+  switch (L.kind) {
+    case W_GEMM: {
+      const WGemm &G = L.g;
+      if (G.nitem * G.nt <= 0 || G.M <= 0 || G.N <= 0) return hipSuccess;
+      dim3 grid(((G.M + 63) / 64) * ((G.N + 63) / 64), G.nitem * G.nt);
+      hipLaunchKernelGGL(wgemm_kernel, grid, dim3(256), 0, s, G);
+      break;
+    }
+    case W_INIT: {
+      const int64_t tot = L.n * L.nitem;
+      hipLaunchKernelGGL(winit_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, L.src, L.dst, L.n, L.nitem);
+      break;
+    }
+    case W_USOLVE: {
+      const int cblocks = (L.ncol + 127) / 128;
+      hipLaunchKernelGGL(wusolve_kernel, dim3(cblocks * L.nitem), dim3(128), 0, s, A.U, L.dst, A.r, L.ncol, L.ld, L.si, L.nitem);
+      break;
+    }
+    case W_GK: {
+      const int64_t tot = (int64_t)A.r * L.n * L.nitem;
+      hipLaunchKernelGGL(wgk_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, L.src, L.dst, A.r, L.n, L.nitem, L.nlines, L.ls[0],
+                         L.ls[1], L.ls[2], L.p, A.kp, A.km);
+      break;
+    }
+    case W_FH: {
+      const int64_t tot = (int64_t)L.qs * A.r * L.b1 * L.b0;
+      hipLaunchKernelGGL(wfh_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, L.dst, L.src, L.qs, L.b1, L.b0, A.r, L.nflav,
+                         L.forward, L.orb[0], L.orb[1], L.orb[2], L.orb[3], L.orb[4], L.orb[5], L.orb[6], L.orb[7], A.delta, A.delta_refl);
+      break;
+    }
+    case W_MAX:
+      hipLaunchKernelGGL(wmax_kernel, dim3(L.nitem), dim3(256), 0, s, L.src, L.n, L.vmax);
+      break;
+    case W_FINAL:
+      hipLaunchKernelGGL(wfinal_kernel, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, L.src, L.n, L.nitem, L.vmax, L.pref, L.dst,
+                         scale);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace xca::dev
