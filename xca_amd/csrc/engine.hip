@@ -328,6 +328,59 @@ __global__ void __launch_bounds__(64 * NW) refl_kernel(ReflArgs A) {
   }
 }
 
+// Large-block engine, node-space part of an edge (or of an edge table): for columns of X (item u:
+// r rows of ld doubles at X + u xsi), G = vals2coefs(X) = U^{-1} (L^{-1} P) X and WG = W G, with the
+// same fragments and substitution as tab_kernel. 16 NW columns per workgroup.
+template <int RB, int NW>
+__global__ void __launch_bounds__(64 * NW) wv2c_kernel(WV2C A) {
+  constexpr int RP = 16 * RB, NT = 64 * NW, LDN = 16 * NW + 8;
+  extern __shared__ double lds[];
+  double *SA = lds, *SB = lds + RP * LDN;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const DlrFrag D = A.D;
+  const int r = D.r;
+  const int cblocks = (A.ncol + 16 * NW - 1) / (16 * NW);
+  const int u = blockIdx.x / cblocks, cb0 = (blockIdx.x % cblocks) * 16 * NW;
+  const int nc = min(16 * NW, A.ncol - cb0);
+  for (int x = tid; x < RP * LDN; x += NT) { SA[x] = 0.0; SB[x] = 0.0; }
+  __syncthreads();
+  const double *X = A.X + (int64_t)u * A.xsi + cb0;
+  for (int x = tid; x < r * nc; x += NT) {
+    const int t = x / nc, c = x % nc;
+    SA[(size_t)t * LDN + c] = X[(int64_t)t * A.ld + c];
+  }
+  __syncthreads();
+  v4d acc[RB];
+  const int c0 = 16 * wv;
+  vals2coefs_block<RB>(D, SA, LDN, c0, acc, lane);
+  store_acc<RB>(acc, SB, LDN, c0, lane, r);
+  const int col = c0 + (lane & 15);
+  if (col < nc) {
+    double *g = A.G + (int64_t)u * A.gsi + cb0 + col;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int t = acc_row(rb, v, lane);
+        if (t < r) g[(int64_t)t * A.ld] = acc[rb][v];
+      }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  zero_acc(acc);
+  gemm_acc<RB>(D.W, 4 * RB, 0, (r + 3) >> 2, SB, LDN, c0, acc, lane);
+  if (col < nc) {
+    double *wg = A.WG + (int64_t)u * A.wsi + cb0 + col;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int t = acc_row(rb, v, lane);
+        if (t < r) wg[(int64_t)t * A.ld] = acc[rb][v];
+      }
+  }
+}
+
 // Wave-uniform copy of a descriptor read from global memory: the compiler treats values loaded
 // from writable global memory as per-lane, which would turn every address derived from them into
 // vector arithmetic; every 32-bit word is moved to a scalar register instead.
@@ -940,6 +993,26 @@ int refl_columns() { return 16 * 8; }
 
 hipError_t launch_refl(int RB, const ReflArgs &A, hipStream_t s) {
   return RB == 2 ? launch_refl_t<2, 8>(A, s) : launch_refl_t<4, 8>(A, s);
+}
+
+template <int RB, int NW>
+static hipError_t launch_wv2c_t(const WV2C &A, hipStream_t s) {
+  constexpr int RP = 16 * RB, LDN = 16 * NW + 8;
+  const size_t sh = sizeof(double) * ((size_t)2 * RP * LDN);
+  static bool ready = false;
+  if (!ready) {
+    hipError_t e = hipFuncSetAttribute((const void *)wv2c_kernel<RB, NW>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
+    if (e != hipSuccess) return e;
+    ready = true;
+  }
+  if (A.nitem <= 0 || A.ncol <= 0) return hipSuccess;
+  const int cblocks = (A.ncol + 16 * NW - 1) / (16 * NW);
+  hipLaunchKernelGGL((wv2c_kernel<RB, NW>), dim3(cblocks * A.nitem), dim3(64 * NW), sh, s, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_wv2c(int RB, const WV2C &A, hipStream_t s) {
+  return RB == 2 ? launch_wv2c_t<2, 8>(A, s) : launch_wv2c_t<4, 8>(A, s);
 }
 
 hipError_t launch_corr(const CorrArgs &A, hipStream_t s) {

@@ -167,6 +167,16 @@ struct ReflArgs {
   int njobs, pad0;
 };
 
+// Large-block engine, node-space part of an edge: G = vals2coefs(X), WG = W G for columns
+// [0, ncol) of every item (r rows of ld doubles; item strides xsi, gsi, wsi).
+struct WV2C {
+  DlrFrag D;
+  const double *X;
+  double *G, *WG;
+  int64_t xsi, gsi, wsi;
+  int ld, ncol, nitem, pad0;
+};
+
 struct CorrArgs {
   const CTerm *terms;
   const CRun *runs;

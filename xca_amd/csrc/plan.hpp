@@ -123,10 +123,9 @@ struct WidePlan {
   int64_t device_bytes = 0;
 };
 
-// Problem-level DLR matrices of the large-block engine (plain row-major): L^{-1}P, U, W and
-// beta [cf2it | tcf2it].
+// Problem-level DLR matrix of the large-block engine (plain row-major): beta [cf2it | tcf2it].
 struct WideCtx {
-  DevBuf<double> LiP, U, W, M2;
+  DevBuf<double> M2;
 };
 
 DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t nfix);

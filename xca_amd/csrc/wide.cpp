@@ -2,7 +2,7 @@
 // enumeration as the tile engine (build_programs, following block_sparse_backbone.cpp:229-367); each
 // program runs over all its pole tuples at once as batched MFMA GEMMs with T resident in HBM:
 //   vertex      T[u](t) <- K(t, pole) F T[u](t)                 (batched over items and nodes)
-//   edge        y = (L^{-1}P) T, g = U^{-1} y, Wg = W g          (along the node index)
+//   edge        g = U^{-1} (L^{-1}P) T, Wg = W g                 (along the node index, wv2c_kernel)
 //               c_k = -(Wf_k g_k + f_k Wg_k), d_k = f_k g_k      (batched over items and k)
 //               T <- beta [cf2it | tcf2it] [c; d]
 //   zero pair   T <- sum_mu F_mu T Fh_mu(t), Fh_mu(t) = sum_kap h(t, mu, kap) F_kap
@@ -36,28 +36,13 @@ void init_wide_ctx(xca_ctx *c) {
   if (c->widectx) return;
   const auto &o = c->M.ops;
   const int r = o.r;
-  std::vector<double> LiP((size_t)r * r, 0.0), U((size_t)r * r, 0.0), M2((size_t)r * 2 * r, 0.0);
-  for (int i = 0; i < r; ++i) LiP[(size_t)i * r + i] = 1.0;
-  for (int k = 0; k < r; ++k)
-    if (o.piv[k] != k)
-      for (int j = 0; j < r; ++j) std::swap(LiP[(size_t)k * r + j], LiP[(size_t)o.piv[k] * r + j]);
-  for (int i = 1; i < r; ++i)
-    for (int k = 0; k < i; ++k) {
-      const double l = o.lu[(size_t)i * r + k];
-      if (l != 0.0)
-        for (int j = 0; j < r; ++j) LiP[(size_t)i * r + j] -= l * LiP[(size_t)k * r + j];
-    }
-  for (int i = 0; i < r; ++i)
-    for (int k = i; k < r; ++k) U[(size_t)i * r + k] = o.lu[(size_t)i * r + k];
+  std::vector<double> M2((size_t)r * 2 * r, 0.0);
   for (int i = 0; i < r; ++i)
     for (int k = 0; k < r; ++k) {
       M2[(size_t)i * 2 * r + k] = c->M.P.beta * o.cf2it[(size_t)i * r + k];
       M2[(size_t)i * 2 * r + r + k] = c->M.P.beta * o.tcf2it[(size_t)i * r + k];
     }
   auto w = std::make_unique<WideCtx>();
-  w->LiP.upload(LiP, c->stream);
-  w->U.upload(U, c->stream);
-  w->W.upload(o.W, c->stream);
   w->M2.upload(M2, c->stream);
   HIP_CHECK(hipStreamSynchronize(c->stream));
   c->widectx = std::move(w);
@@ -198,21 +183,25 @@ static WidePlan *get_wide_plan(xca_ctx *c, int m, const int *topo, const int32_t
     for (int q = 0; q < 3; ++q) L.ls[q] = q < tb.nlines ? tb.lsign[q] : 0;
     L.p = p;
     wp->launches.push_back(L);
-    WGemm g = gemm(W.LiP.ptr, Tab, ftab + tb.offset, r, (int)bb, r, r, (int)bb, (int)bb, ne, 1);
-    g.b_si = r * bb;
-    g.c_si = 2 * r * bb;
-    wp->launches.push_back(of(g));
-    WLaunch S{};
-    S.kind = W_USOLVE;
-    S.dst = ftab + tb.offset;
-    S.ncol = (int)bb;
-    S.ld = (int)bb;
-    S.si = 2 * r * bb;
-    S.nitem = ne;
-    wp->launches.push_back(S);
-    WGemm gw = gemm(W.W.ptr, ftab + tb.offset, ftab + tb.offset + r * bb, r, (int)bb, r, r, (int)bb, (int)bb, ne, 1);
-    gw.b_si = gw.c_si = 2 * r * bb;
-    wp->launches.push_back(of(gw));
+    WLaunch V{};
+    V.kind = W_V2C;
+    V.v.D = c->D;
+    V.v.X = Tab;
+    V.v.G = ftab + tb.offset;
+    V.v.WG = ftab + tb.offset + r * bb;
+    V.v.xsi = r * bb;
+    V.v.gsi = V.v.wsi = 2 * r * bb;
+    V.v.ld = (int)bb;
+    V.v.ncol = (int)bb;
+    V.v.nitem = ne;
+    wp->launches.push_back(V);
+  }
+  // the per-item maxima start at zero
+  const size_t zero_launch = wp->launches.size();
+  {
+    WLaunch Z{};
+    Z.kind = W_ZERO;
+    wp->launches.push_back(Z);
   }
 
   // per-item offsets and prefactors
@@ -221,7 +210,7 @@ static WidePlan *get_wide_plan(xca_ctx *c, int m, const int *topo, const int32_t
   int64_t nvmax = 0;
   struct Pending {  // launches that take device pointers into offs / pref / vmax, fixed after upload
     size_t launch;
-    int64_t a_off, s_off, pref, vmax;
+    int64_t a_off, s_off, pref, vmax, f_off = -1;
   };
   std::vector<Pending> pend;
   for (const ProgItems &pi : progs) {
@@ -267,20 +256,17 @@ static WidePlan *get_wide_plan(xca_ctx *c, int m, const int *topo, const int32_t
       } else if (op.type == OP_E) {
         const int b = op.bin;
         const int64_t w = (int64_t)b * cols;
-        WGemm y = gemm(W.LiP.ptr, cur, Y, r, (int)w, r, r, (int)w, (int)w, ni, 1);
-        y.b_si = y.c_si = r * w;
-        wp->launches.push_back(of(y));
-        WLaunch S{};
-        S.kind = W_USOLVE;
-        S.dst = Y;
-        S.ncol = (int)w;
-        S.ld = (int)w;
-        S.si = r * w;
-        S.nitem = ni;
-        wp->launches.push_back(S);
-        WGemm wg = gemm(W.W.ptr, Y, WG, r, (int)w, r, r, (int)w, (int)w, ni, 1);
-        wg.b_si = wg.c_si = r * w;
-        wp->launches.push_back(of(wg));
+        WLaunch V{};
+        V.kind = W_V2C;
+        V.v.D = c->D;
+        V.v.X = cur;
+        V.v.G = Y;
+        V.v.WG = WG;
+        V.v.xsi = V.v.gsi = V.v.wsi = r * w;
+        V.v.ld = (int)w;
+        V.v.ncol = (int)w;
+        V.v.nitem = ni;
+        wp->launches.push_back(V);
         // table entry (f part) of every item
         const int64_t eoff = (int64_t)offs.size();
         const auto &tb = plan.tables[op.etab];
@@ -291,21 +277,24 @@ static WidePlan *get_wide_plan(xca_ctx *c, int m, const int *topo, const int32_t
           offs.push_back(e);
         }
         // c = -(Wf g + f Wg), d = f g, per item and coefficient index k
-        auto local = [&](int64_t a_extra, const double *B, double *C, double alpha, double beta) {
-          WGemm g = gemm(ftab + a_extra, B, C, b, cols, b, b, cols, cols, ni, r);
-          g.a_st = (int64_t)b * b;
-          g.b_si = r * w;
-          g.b_st = w;
-          g.c_si = 2 * r * w;
-          g.c_st = w;
-          g.alpha = alpha;
-          g.beta = beta;
-          pend.push_back(Pending{wp->launches.size(), eoff, -1, -1, -1});
-          wp->launches.push_back(of(g));
-        };
-        local((int64_t)r * b * b, Y, CD, -1.0, 0.0);   // -Wf g
-        local(0, WG, CD, -1.0, 1.0);                     // -f Wg
-        local(0, Y, CD + r * w, 1.0, 0.0);               // f g
+        WLaunch Lc{};
+        Lc.kind = W_LOCAL;
+        Lc.loc.F = ftab;
+        Lc.loc.Gk = Y;
+        Lc.loc.WGk = WG;
+        Lc.loc.C = CD;
+        Lc.loc.D = CD + r * w;
+        Lc.loc.g_si = r * w;
+        Lc.loc.c_si = 2 * r * w;
+        Lc.loc.wf_shift = (int64_t)r * b * b;
+        Lc.loc.b = b;
+        Lc.loc.cols = cols;
+        Lc.loc.r = r;
+        Lc.loc.nitem = ni;
+        Pending pl{wp->launches.size(), -1, -1, -1, -1};
+        pl.f_off = eoff;
+        pend.push_back(pl);
+        wp->launches.push_back(Lc);
         WGemm h = gemm(W.M2.ptr, CD, oth, r, (int)w, 2 * r, 2 * r, (int)w, (int)w, ni, 1);
         h.b_si = 2 * r * w;
         h.c_si = r * w;
@@ -374,8 +363,7 @@ static WidePlan *get_wide_plan(xca_ctx *c, int m, const int *topo, const int32_t
     };
     const Ext ext[] = {{c->G.ptr, (int64_t)c->G.n}, {c->sigma.ptr, (int64_t)c->sigma.n}, {c->opmats.ptr, (int64_t)c->opmats.n},
                        {c->kp.ptr, (int64_t)c->kp.n}, {c->km.ptr, (int64_t)c->km.n}, {wp->work.ptr, (int64_t)wp->work.n},
-                       {wp->ftab.ptr, (int64_t)wp->ftab.n}, {W.LiP.ptr, (int64_t)W.LiP.n}, {W.W.ptr, (int64_t)W.W.n},
-                       {W.M2.ptr, (int64_t)W.M2.n}};
+                       {wp->ftab.ptr, (int64_t)wp->ftab.n}, {W.M2.ptr, (int64_t)W.M2.n}};
     auto in = [&](const double *ptr, int64_t lo, int64_t hi, const char *what) {
       for (const Ext &e : ext)
         if (ptr >= e.base && ptr < e.base + e.n) {
@@ -387,7 +375,11 @@ static WidePlan *get_wide_plan(xca_ctx *c, int m, const int *topo, const int32_t
     std::vector<std::pair<int64_t, int64_t>> aoff(wp->launches.size(), {0, 0}), soff(wp->launches.size(), {-1, -1});
     for (const Pending &pd : pend) {
       const WLaunch &L = wp->launches[pd.launch];
-      const int ni = L.kind == W_GEMM ? L.g.nitem : L.nitem;
+      const int ni = L.kind == W_GEMM ? L.g.nitem : (L.kind == W_LOCAL ? L.loc.nitem : L.nitem);
+      if (pd.f_off >= 0) {
+        auto mm = std::minmax_element(offs.begin() + pd.f_off, offs.begin() + pd.f_off + ni);
+        aoff[pd.launch] = {*mm.first, *mm.second};
+      }
       if (pd.a_off >= 0) {
         auto mm = std::minmax_element(offs.begin() + pd.a_off, offs.begin() + pd.a_off + ni);
         aoff[pd.launch] = {*mm.first, *mm.second};
@@ -416,8 +408,27 @@ static WidePlan *get_wide_plan(xca_ctx *c, int m, const int *topo, const int32_t
           in(L.src, 0, L.n - 1, "init src");
           in(L.dst, 0, L.n * L.nitem - 1, "init dst");
           break;
-        case W_USOLVE:
-          in(L.dst, 0, (int64_t)(L.nitem - 1) * L.si + (int64_t)(r - 1) * L.ld + L.ncol - 1, "usolve");
+        case W_V2C: {
+          const WV2C &v = L.v;
+          const int64_t span = (int64_t)(r - 1) * v.ld + v.ncol - 1;
+          in(v.X, 0, (int64_t)(v.nitem - 1) * v.xsi + span, "v2c X");
+          in(v.G, 0, (int64_t)(v.nitem - 1) * v.gsi + span, "v2c G");
+          in(v.WG, 0, (int64_t)(v.nitem - 1) * v.wsi + span, "v2c WG");
+          if (v.ncol > v.ld) throw std::logic_error("wide plan: leading dimension");
+          break;
+        }
+        case W_LOCAL: {
+          const WLocal &q = L.loc;
+          const int64_t bb = (int64_t)q.b * q.b, bc = (int64_t)q.b * q.cols;
+          in(q.F, aoff[i].first, aoff[i].second + (int64_t)(q.r - 1) * bb + q.wf_shift + bb - 1, "local f");
+          in(q.Gk, 0, (int64_t)(q.nitem - 1) * q.g_si + (int64_t)q.r * bc - 1, "local g");
+          in(q.WGk, 0, (int64_t)(q.nitem - 1) * q.g_si + (int64_t)q.r * bc - 1, "local Wg");
+          in(q.C, 0, (int64_t)(q.nitem - 1) * q.c_si + (int64_t)q.r * bc - 1, "local c");
+          in(q.D, 0, (int64_t)(q.nitem - 1) * q.c_si + (int64_t)q.r * bc - 1, "local d");
+          if (q.r != r) throw std::logic_error("wide plan: node count");
+          break;
+        }
+        case W_ZERO:
           break;
         case W_GK:
           in(L.src, 0, (int64_t)r * L.n - 1, "gk src");
@@ -442,8 +453,11 @@ static WidePlan *get_wide_plan(xca_ctx *c, int m, const int *topo, const int32_t
   wp->offs.upload(offs, s);
   wp->pref.upload(prefs, s);
   wp->vmax.alloc(std::max<int64_t>(nvmax, 1));
+  wp->launches[zero_launch].dst = wp->vmax.ptr;
+  wp->launches[zero_launch].n = std::max<int64_t>(nvmax, 1);
   for (const Pending &pd : pend) {
     WLaunch &L = wp->launches[pd.launch];
+    if (pd.f_off >= 0) L.loc.f_off = wp->offs.ptr + pd.f_off;
     if (pd.a_off >= 0) L.g.a_off = wp->offs.ptr + pd.a_off;
     if (pd.s_off >= 0) L.g.s_off = wp->offs.ptr + pd.s_off;
     if (pd.pref >= 0) L.pref = wp->pref.ptr + pd.pref;
@@ -459,7 +473,7 @@ static WidePlan *get_wide_plan(xca_ctx *c, int m, const int *topo, const int32_t
 void wide_accumulate(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t nfix, double scale) {
   WidePlan *wp = get_wide_plan(c, m, topo, fix, nfix);
   const WideCtx &W = *c->widectx;
-  WArgs A{W.U.ptr, c->kp.ptr, c->km.ptr, c->delta.ptr, c->delta_refl.ptr, c->M.P.r, 0};
+  WArgs A{c->kp.ptr, c->km.ptr, c->delta.ptr, c->delta_refl.ptr, c->M.P.r, c->RB};
   hipStream_t s = c->stream;
   c->last_items = wp->items;
   c->last_macs = wp->macs;

@@ -1,7 +1,10 @@
 // Kernels of the large-block Σ engine: a batched FP64 MFMA GEMM with per-item operand offsets and a
-// per-(item, node) scale, the LU back substitution along the node index, edge-table inputs, the
-// zero-vertex operator sums, and the per-item skip test and Σ accumulation.
+// per-(item, node) scale, the fused node-local products of an edge, edge-table inputs, the
+// zero-vertex operator sums, and the per-item skip test and Σ accumulation. The node-space part of an
+// edge (vals2coefs and W) runs in engine.hip's wv2c_kernel.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "wide.hpp"
 
@@ -84,37 +87,6 @@ __global__ void winit_kernel(const double *__restrict__ src, double *__restrict_
   dst[g] = src[g % n];
 }
 
-// U x = y per column, columns of 128 staged in LDS: X[u] holds r rows of ld doubles, columns
-// [0, ncol); U is r x r upper triangular (row-major).
-__global__ void __launch_bounds__(128) wusolve_kernel(const double *__restrict__ U, double *X, int r, int ncol, int ld, int64_t si, int nitem) {
-  __shared__ double Us[64 * 64];
-  __shared__ double Xs[64][129];
-  const int tid = threadIdx.x;
-  const int cblocks = (ncol + 127) / 128;
-  const int u = blockIdx.x / cblocks, c0 = (blockIdx.x % cblocks) * 128;
-  if (u >= nitem) return;
-  for (int x = tid; x < r * r; x += 128) Us[x] = U[x];
-  double *Xu = X + (int64_t)u * si;
-  const int nc = min(128, ncol - c0);
-  for (int x = tid; x < r * nc; x += 128) {
-    const int i = x / nc, j = x % nc;
-    Xs[i][j] = Xu[(int64_t)i * ld + c0 + j];
-  }
-  __syncthreads();
-  if (tid < nc) {
-    for (int i = r - 1; i >= 0; --i) {
-      double s = Xs[i][tid];
-      for (int k = i + 1; k < r; ++k) s -= Us[i * r + k] * Xs[k][tid];
-      Xs[i][tid] = s / Us[i * r + i];
-    }
-  }
-  __syncthreads();
-  for (int x = tid; x < r * nc; x += 128) {
-    const int i = x / nc, j = x % nc;
-    Xu[(int64_t)i * ld + c0 + j] = Xs[i][j];
-  }
-}
-
 // Edge-table inputs: dst[e] = G (r x n) times the K factors of the table's lines for entry e
 // (line q's pole = (e / p^q) mod p, sign ls[q])
 __global__ void wgk_kernel(const double *__restrict__ G, double *__restrict__ dst, int r, int64_t n, int nent, int nlines, int ls0, int ls1,
@@ -156,19 +128,93 @@ __global__ void wfh_kernel(double *__restrict__ dst, const double *__restrict__ 
   dst[g] = s;
 }
 
-// vmax[u] = max |src[u][x]|, one workgroup per item
+// vmax[u] = max(vmax[u], max |src[u][x]|): slices of one item per workgroup (blockIdx.y = item),
+// combined with an unsigned max on the bit patterns, which orders non-negative doubles
 __global__ void __launch_bounds__(256) wmax_kernel(const double *__restrict__ src, int64_t n, double *vmax) {
   __shared__ double red[256];
-  const int u = blockIdx.x;
+  const int u = blockIdx.y;
   double m = 0.0;
-  for (int64_t x = threadIdx.x; x < n; x += 256) m = fmax(m, fabs(src[(int64_t)u * n + x]));
+  for (int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x; x < n; x += (int64_t)gridDim.x * 256) m = fmax(m, fabs(src[(int64_t)u * n + x]));
   red[threadIdx.x] = m;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) vmax[u] = red[0];
+  if (threadIdx.x == 0) atomicMax((unsigned long long *)(vmax + u), (unsigned long long)__double_as_longlong(red[0]));
+}
+
+__global__ void wzero_kernel(double *dst, int64_t n) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n) dst[g] = 0.0;
+}
+
+// Node-local products of an edge, fused: one 64 x 64 tile of C[u][k] and D[u][k] per workgroup,
+// f / Wf tiles and g / Wg tiles staged together through LDS.
+__global__ void __launch_bounds__(256) wlocal_kernel(WLocal L) {
+  constexpr int TM = 64, TN = 64, TK = 16;
+  __shared__ double Af[TK][TM + 1], Aw[TK][TM + 1], Bg[TK][TN + 1], Bw[TK][TN + 1];
+  const int b = L.b, cols = L.cols;
+  const int tilesN = (cols + TN - 1) / TN;
+  const int tm = blockIdx.x / tilesN, tn = blockIdx.x % tilesN;
+  const int q = blockIdx.y, u = q / L.r, k = q % L.r;
+  const double *f = L.F + L.f_off[u] + (int64_t)k * b * b, *wf = f + L.wf_shift;
+  const int64_t gofs = (int64_t)u * L.g_si + (int64_t)k * b * cols;
+  const double *g = L.Gk + gofs, *wg = L.WGk + gofs;
+  const int64_t cofs = (int64_t)u * L.c_si + (int64_t)k * b * cols;
+  double *C = L.C + cofs, *D = L.D + cofs;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int m0 = tm * TM, n0 = tn * TN;
+  const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
+  v4d cc[2][2], dd[2][2];
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) cc[bi][bj] = dd[bi][bj] = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < b; k0 += TK) {
+    for (int x = tid; x < TM * TK; x += 256) {
+      const int i = x / TK, kk = x % TK, gi = m0 + i, gk = k0 + kk;
+      const bool ok = gi < b && gk < b;
+      Af[kk][i] = ok ? f[(int64_t)gi * b + gk] : 0.0;
+      Aw[kk][i] = ok ? wf[(int64_t)gi * b + gk] : 0.0;
+    }
+    for (int x = tid; x < TK * TN; x += 256) {
+      const int kk = x / TN, j = x % TN, gk = k0 + kk, gj = n0 + j;
+      const bool ok = gk < b && gj < cols;
+      Bg[kk][j] = ok ? g[(int64_t)gk * cols + gj] : 0.0;
+      Bw[kk][j] = ok ? wg[(int64_t)gk * cols + gj] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < TK; kk += 4) {
+      const int kr = kk + (lane >> 4), c = lane & 15;
+      const double f0 = Af[kr][wm + c], f1 = Af[kr][wm + 16 + c];
+      const double w0 = Aw[kr][wm + c], w1 = Aw[kr][wm + 16 + c];
+      const double g0 = Bg[kr][wn + c], g1 = Bg[kr][wn + 16 + c];
+      const double h0 = Bw[kr][wn + c], h1 = Bw[kr][wn + 16 + c];
+      cc[0][0] = mfma(w0, g0, mfma(f0, h0, cc[0][0]));
+      cc[0][1] = mfma(w0, g1, mfma(f0, h1, cc[0][1]));
+      cc[1][0] = mfma(w1, g0, mfma(f1, h0, cc[1][0]));
+      cc[1][1] = mfma(w1, g1, mfma(f1, h1, cc[1][1]));
+      dd[0][0] = mfma(f0, g0, dd[0][0]);
+      dd[0][1] = mfma(f0, g1, dd[0][1]);
+      dd[1][0] = mfma(f1, g0, dd[1][0]);
+      dd[1][1] = mfma(f1, g1, dd[1][1]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = m0 + wm + 16 * bi + 4 * v + (lane >> 4), j = n0 + wn + 16 * bj + (lane & 15);
+        if (i < b && j < cols) {
+          C[(int64_t)i * cols + j] = -cc[bi][bj][v];
+          D[(int64_t)i * cols + j] = dd[bi][bj][v];
+        }
+      }
 }
 
 // Sigma block += scale * sum over items, in order, of the kept items' pref * T
@@ -196,11 +242,17 @@ hipError_t launch_wide(const WLaunch &L, const WArgs &A, double scale, hipStream
       hipLaunchKernelGGL(winit_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, L.src, L.dst, L.n, L.nitem);
       break;
     }
-    case W_USOLVE: {
-      const int cblocks = (L.ncol + 127) / 128;
-      hipLaunchKernelGGL(wusolve_kernel, dim3(cblocks * L.nitem), dim3(128), 0, s, A.U, L.dst, A.r, L.ncol, L.ld, L.si, L.nitem);
+    case W_V2C:
+      return launch_wv2c(A.RB, L.v, s);
+    case W_LOCAL: {
+      const WLocal &W = L.loc;
+      dim3 grid(((W.b + 63) / 64) * ((W.cols + 63) / 64), W.nitem * W.r);
+      hipLaunchKernelGGL(wlocal_kernel, grid, dim3(256), 0, s, W);
       break;
     }
+    case W_ZERO:
+      hipLaunchKernelGGL(wzero_kernel, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, L.dst, L.n);
+      break;
     case W_GK: {
       const int64_t tot = (int64_t)A.r * L.n * L.nitem;
       hipLaunchKernelGGL(wgk_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, L.src, L.dst, A.r, L.n, L.nitem, L.nlines, L.ls[0],
@@ -213,9 +265,11 @@ hipError_t launch_wide(const WLaunch &L, const WArgs &A, double scale, hipStream
                          L.forward, L.orb[0], L.orb[1], L.orb[2], L.orb[3], L.orb[4], L.orb[5], L.orb[6], L.orb[7], A.delta, A.delta_refl);
       break;
     }
-    case W_MAX:
-      hipLaunchKernelGGL(wmax_kernel, dim3(L.nitem), dim3(256), 0, s, L.src, L.n, L.vmax);
+    case W_MAX: {
+      const int slices = (int)std::min<int64_t>(64, (L.n + 4095) / 4096);
+      hipLaunchKernelGGL(wmax_kernel, dim3(slices, L.nitem), dim3(256), 0, s, L.src, L.n, L.vmax);
       break;
+    }
     case W_FINAL:
       hipLaunchKernelGGL(wfinal_kernel, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, L.src, L.n, L.nitem, L.vmax, L.pref, L.dst,
                          scale);

@@ -6,6 +6,8 @@
 
 #include <cstdint>
 
+#include "engine.hpp"
+
 namespace xca::dev {
 
 // Batched row-major GEMM: for item u < nitem and t < nt,
@@ -23,14 +25,28 @@ struct WGemm {
   double alpha, beta;
 };
 
-enum WKind : int { W_GEMM = 0, W_INIT, W_USOLVE, W_GK, W_FH, W_MAX, W_FINAL };
+// Node-local products of an edge for item u and node index k (f_k, Wf_k: b x b table rows of the
+// item's entry at F + f_off[u] + k b^2 and + r b^2; g_k, Wg_k: b x cols):
+//   C[u][k] = -(Wf_k g_k + f_k Wg_k),  D[u][k] = f_k g_k.
+struct WLocal {
+  const double *F, *Gk, *WGk;
+  double *C, *D;
+  const int64_t *f_off;
+  int64_t g_si, c_si;   // item strides of g/Wg and of C/D (node stride = b cols)
+  int64_t wf_shift;     // r b^2
+  int b, cols, r, nitem;
+};
+
+enum WKind : int { W_GEMM = 0, W_INIT, W_V2C, W_GK, W_FH, W_MAX, W_FINAL, W_LOCAL, W_ZERO };
 
 // One launch of a wide plan. Fields by kind:
-//   W_INIT:   dst[u] = src (n doubles), optionally transposed (tr, b x b blocks of r nodes)
-//   W_USOLVE: X[u] (r rows, ncol columns, row stride ld, item stride si) <- U^{-1} X[u]
+//   W_INIT:   dst[u] = src (n doubles)
+//   W_V2C:    v.G = vals2coefs(v.X), v.WG = W v.G (launch_wv2c)
+//   W_LOCAL:  the node-local products of an edge
+//   W_ZERO:   dst[0, n) = 0
 //   W_GK:     dst[e] (r x n) = G block src (r x n) times prod_q K^{ls_q}(t, pole of line q of entry e)
 //   W_FH:     dst[mu][t] (b1 x b0) = sum_kap h(t, mu, kap) F_kap (F_kap at src + kap b1 b0)
-//   W_MAX:    vmax[u] = max |src[u]| over n doubles
+//   W_MAX:    vmax[u] = max(vmax[u], max |src[u]|) over n doubles (vmax zeroed by W_ZERO first)
 //   W_FINAL:  dst (r b0^2, the Sigma block) += scale sum_u [vmax[u] |pref[u]| > 1e-16] pref[u] src[u]
 struct WLaunch {
   int kind;
@@ -46,13 +62,16 @@ struct WLaunch {
   // W_MAX / W_FINAL
   double *vmax;
   const double *pref;
+  WV2C v;
+  WLocal loc;
 };
 
 struct WArgs {
-  const double *U;      // r x r upper factor of cf2it (row-major)
   const double *kp, *km, *delta, *delta_refl;
-  int r, pad0;
+  int r, RB;
 };
+
+hipError_t launch_wv2c(int RB, const WV2C &A, hipStream_t s);
 
 hipError_t launch_wide(const WLaunch &L, const WArgs &A, double scale, hipStream_t s);
 
