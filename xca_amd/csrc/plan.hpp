@@ -116,7 +116,7 @@ struct CorrDevPlan {
 // accumulation), the per-item operand offsets and prefactors, and the workspace.
 struct WidePlan {
   std::vector<WLaunch> launches;
-  DevBuf<double> work, ftab, pref, vmax;
+  DevBuf<double> work, ftab, pref, vmax, zperm;
   DevBuf<int64_t> offs;
   int64_t items = 0;
   double macs = 0;

@@ -5,7 +5,8 @@
 //   edge        g = U^{-1} (L^{-1}P) T, Wg = W g                 (along the node index, wv2c_kernel)
 //               c_k = -(Wf_k g_k + f_k Wg_k), d_k = f_k g_k      (batched over items and k)
 //               T <- beta [cf2it | tcf2it] [c; d]
-//   zero pair   T <- sum_mu F_mu T Fh_mu(t), Fh_mu(t) = sum_kap h(t, mu, kap) F_kap
+//   zero pair   T <- sum_mu F_mu T Fh_mu(t), Fh_mu(t) = sum_kap h(t, mu, kap) F_kap, as two GEMMs:
+//               Y = T [Fh_0 | .. | Fh_q-1] and T <- [F_mu columns interleaved] Y
 //   final       per-item max |T| (the reference's 1e-16 skip) and Sigma[b0] += scale sum pref T.
 // The launch list of a (topology, zero-block pattern, flat-index subset) is built once and replayed.
 #include <hip/hip_runtime_api.h>
@@ -103,7 +104,7 @@ static WidePlan *get_wide_plan(xca_ctx *c, int m, const int *topo, const int32_t
     std::vector<std::vector<int>> poles;  // [item][line]
   };
   std::vector<ProgItems> progs;
-  int64_t max_ws = 0, max_fh = 0;  // workspace doubles per buffer, zero-pair operator sums
+  int64_t max_ws = 0, max_fh = 0, max_yz = 0;  // workspace doubles per buffer, zero-pair sums and products
   for (const Program &pg : plan.progs) {
     int fb[kMaxLines + 1];
     for (int i = 0, f = pg.fb_ix; i < m; ++i, f /= 2) fb[i] = f % 2;
@@ -146,8 +147,9 @@ static WidePlan *get_wide_plan(xca_ctx *c, int m, const int *topo, const int32_t
     for (const Op &op : pg.ops) {
       w = std::max(w, std::max((int64_t)op.bin * cols, (int64_t)op.bout * cols));
       if (op.type == OP_Z) {
-        w = std::max(w, std::max((int64_t)op.bin, (int64_t)op.bout) * op.cols);  // T Fh_mu and the result
+        w = std::max(w, (int64_t)op.bout * op.cols);
         max_fh = std::max(max_fh, (int64_t)op.qs * r * op.b1 * op.cols);
+        max_yz = std::max(max_yz, ni * r * op.bin * op.qs * op.cols);  // T [Fh_0 | Fh_1 | ...]
         cols = op.cols;
       }
       rows = op.bout;
@@ -160,10 +162,27 @@ static WidePlan *get_wide_plan(xca_ctx *c, int m, const int *topo, const int32_t
   // buffers: T0, T1, Y, WG (max_ws each), CD (2 max_ws), Fh, table scratch
   int64_t max_tab = 0;
   for (const auto &tb : plan.tables) max_tab = std::max(max_tab, tb.nentries * r * (int64_t)P.dims[tb.block] * P.dims[tb.block]);
-  const int64_t nwork = 6 * max_ws + max_fh + max_tab;
+  const int64_t nwork = 6 * max_ws + max_fh + max_tab + max_yz;
   wp->work.alloc(std::max<int64_t>(nwork, 1));
   double *T0 = wp->work.ptr, *T1 = T0 + max_ws, *Y = T1 + max_ws, *WG = Y + max_ws, *CD = WG + max_ws, *Fh = CD + 2 * max_ws,
-         *Tab = Fh + max_fh;
+         *Tab = Fh + max_fh, *Yz = Tab + max_tab;
+  // mu operators of every zero-vertex pair with their columns reordered to (a qs + mu), so that the
+  // sum over mu is one GEMM against T [Fh_0 | Fh_1 | ...] read as (b_v qs) x b0
+  std::map<int64_t, int64_t> zperm_of;
+  std::vector<double> zperm;
+  for (const ProgItems &pi : progs)
+    for (const Op &op : pi.pg->ops)
+      if (op.type == OP_Z && !zperm_of.count(op.mubase)) {
+        zperm_of[op.mubase] = (int64_t)zperm.size();
+        const int bv = op.bin, bvp = op.bout, qs = op.qs;
+        const size_t base = zperm.size();
+        zperm.resize(base + (size_t)bvp * bv * qs);
+        for (int mu = 0; mu < qs; ++mu)
+          for (int i = 0; i < bvp; ++i)
+            for (int a = 0; a < bv; ++a)
+              zperm[base + (size_t)i * bv * qs + (size_t)a * qs + mu] = c->M.opmats[op.mubase + ((size_t)mu * bvp + i) * bv + a];
+      }
+  wp->zperm.upload(zperm, c->stream);
   wp->ftab.alloc(std::max<int64_t>(plan.ftab_size, 1));
   const WideCtx &W = *c->widectx;
   double *G = c->G.ptr, *ftab = wp->ftab.ptr;
@@ -313,22 +332,20 @@ static WidePlan *get_wide_plan(xca_ctx *c, int m, const int *topo, const int32_t
         F.nflav = n;
         for (int k = 0; k < qs; ++k) F.orb[k] = c->M.S.to_orb[op.set][k];
         wp->launches.push_back(F);
-        for (int mu = 0; mu < qs; ++mu) {
-          WGemm g1 = gemm(cur, Fh + (int64_t)mu * r * b1 * b0, Y, bv, b0, b1, b1, b0, b0, ni, r);
-          g1.a_si = (int64_t)r * bv * b1;
-          g1.a_st = (int64_t)bv * b1;
-          g1.b_st = (int64_t)b1 * b0;
-          g1.c_si = (int64_t)r * bv * b0;
-          g1.c_st = (int64_t)bv * b0;
-          wp->launches.push_back(of(g1));
-          WGemm g2 = gemm(c->opmats.ptr + op.mubase + (int64_t)mu * bvp * bv, Y, oth, bvp, b0, bv, bv, b0, b0, ni, r);
-          g2.b_si = (int64_t)r * bv * b0;
-          g2.b_st = (int64_t)bv * b0;
-          g2.c_si = (int64_t)r * bvp * b0;
-          g2.c_st = (int64_t)bvp * b0;
-          g2.beta = mu == 0 ? 0.0 : 1.0;
-          wp->launches.push_back(of(g2));
-        }
+        const int qb0 = qs * b0;
+        WGemm g1 = gemm(cur, Fh, Yz, bv, qb0, b1, b1, qb0, qb0, ni, r);  // T(t) [Fh_0(t) | ... ]
+        g1.a_si = (int64_t)r * bv * b1;
+        g1.a_st = (int64_t)bv * b1;
+        g1.b_st = (int64_t)b1 * qb0;
+        g1.c_si = (int64_t)r * bv * qb0;
+        g1.c_st = (int64_t)bv * qb0;
+        wp->launches.push_back(of(g1));
+        WGemm g2 = gemm(wp->zperm.ptr + zperm_of[op.mubase], Yz, oth, bvp, b0, bv * qs, bv * qs, b0, b0, ni, r);
+        g2.b_si = (int64_t)r * bv * qb0;
+        g2.b_st = (int64_t)bv * qb0;
+        g2.c_si = (int64_t)r * bvp * b0;
+        g2.c_st = (int64_t)bvp * b0;
+        wp->launches.push_back(of(g2));
         cols = b0;
         std::swap(cur, oth);
       }
@@ -363,7 +380,7 @@ static WidePlan *get_wide_plan(xca_ctx *c, int m, const int *topo, const int32_t
     };
     const Ext ext[] = {{c->G.ptr, (int64_t)c->G.n}, {c->sigma.ptr, (int64_t)c->sigma.n}, {c->opmats.ptr, (int64_t)c->opmats.n},
                        {c->kp.ptr, (int64_t)c->kp.n}, {c->km.ptr, (int64_t)c->km.n}, {wp->work.ptr, (int64_t)wp->work.n},
-                       {wp->ftab.ptr, (int64_t)wp->ftab.n}, {W.M2.ptr, (int64_t)W.M2.n}};
+                       {wp->ftab.ptr, (int64_t)wp->ftab.n}, {W.M2.ptr, (int64_t)W.M2.n}, {wp->zperm.ptr, (int64_t)wp->zperm.n}};
     auto in = [&](const double *ptr, int64_t lo, int64_t hi, const char *what) {
       for (const Ext &e : ext)
         if (ptr >= e.base && ptr < e.base + e.n) {
@@ -463,7 +480,7 @@ static WidePlan *get_wide_plan(xca_ctx *c, int m, const int *topo, const int32_t
     if (pd.pref >= 0) L.pref = wp->pref.ptr + pd.pref;
     if (pd.vmax >= 0) L.vmax = wp->vmax.ptr + pd.vmax;
   }
-  wp->device_bytes = 8 * (int64_t)(wp->work.n + wp->ftab.n + wp->pref.n + wp->vmax.n + wp->offs.n);
+  wp->device_bytes = 8 * (int64_t)(wp->work.n + wp->ftab.n + wp->pref.n + wp->vmax.n + wp->offs.n + wp->zperm.n);
   HIP_CHECK(hipStreamSynchronize(s));
   WidePlan *raw = wp.get();
   c->wide_plans[key] = std::move(wp);

@@ -21,10 +21,11 @@ __device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
 }  // namespace
 
 // 64 x 64 output tile per workgroup, 4 waves of 32 x 32 (2 x 2 MFMA blocks of 16 x 16), K staged
-// through LDS 16 at a time. MFMA f64 16x16x4 lane layout: A[i][k] in lane i + 16k, B[k][j] in lane
-// j + 16k, D(i, j) in lane j + 16 (i mod 4), register i / 4.
+// through LDS 32 at a time with the next K tile prefetched into registers while the current one is
+// multiplied. MFMA f64 16x16x4 lane layout: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
+// D(i, j) in lane j + 16 (i mod 4), register i / 4.
 __global__ void __launch_bounds__(256) wgemm_kernel(WGemm G) {
-  constexpr int TM = 64, TN = 64, TK = 16;
+  constexpr int TM = 64, TN = 64, TK = 32, PER = TM * TK / 256;  // 8 A and 8 B elements per thread
   __shared__ double As[TK][TM + 1];
   __shared__ double Bs[TK][TN + 1];
   const int tilesN = (G.N + TN - 1) / TN;
@@ -41,25 +42,39 @@ __global__ void __launch_bounds__(256) wgemm_kernel(WGemm G) {
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
     for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = v4d{0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < G.K; k0 += TK) {
-    for (int x = tid; x < TM * TK; x += 256) {
+  double ra[PER], rb[PER];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int x = tid + 256 * e;
       const int i = x / TK, kk = x % TK, gi = m0 + i, gk = k0 + kk;
-      As[kk][i] = (gi < G.M && gk < G.K) ? A[(int64_t)gi * G.lda + gk] : 0.0;
+      ra[e] = (gi < G.M && gk < G.K) ? A[(int64_t)gi * G.lda + gk] : 0.0;
+      const int kb = x / TN, j = x % TN, gkb = k0 + kb, gj = n0 + j;
+      rb[e] = (gkb < G.K && gj < G.N) ? B[(int64_t)gkb * G.ldb + gj] : 0.0;
     }
-    for (int x = tid; x < TK * TN; x += 256) {
-      const int kk = x / TN, j = x % TN, gk = k0 + kk, gj = n0 + j;
-      Bs[kk][j] = (gk < G.K && gj < G.N) ? B[(int64_t)gk * G.ldb + gj] : 0.0;
+  };
+  fetch(0);
+  for (int k0 = 0; k0 < G.K; k0 += TK) {
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int x = tid + 256 * e;
+      As[x % TK][x / TK] = ra[e];
+      Bs[x / TN][x % TN] = rb[e];
     }
     __syncthreads();
+    if (k0 + TK < G.K) fetch(k0 + TK);
+    const int kend = min(TK, G.K - k0);
 #pragma unroll
     for (int kk = 0; kk < TK; kk += 4) {
-      const int kr = kk + (lane >> 4), c = lane & 15;
-      const double a0 = As[kr][wm + c], a1 = As[kr][wm + 16 + c];
-      const double b0 = Bs[kr][wn + c], b1 = Bs[kr][wn + 16 + c];
-      acc[0][0] = mfma(a0, b0, acc[0][0]);
-      acc[0][1] = mfma(a0, b1, acc[0][1]);
-      acc[1][0] = mfma(a1, b0, acc[1][0]);
-      acc[1][1] = mfma(a1, b1, acc[1][1]);
+      if (kk < kend) {
+        const int kr = kk + (lane >> 4), c = lane & 15;
+        const double a0 = As[kr][wm + c], a1 = As[kr][wm + 16 + c];
+        const double b0 = Bs[kr][wn + c], b1 = Bs[kr][wn + 16 + c];
+        acc[0][0] = mfma(a0, b0, acc[0][0]);
+        acc[0][1] = mfma(a0, b1, acc[0][1]);
+        acc[1][0] = mfma(a1, b0, acc[1][0]);
+        acc[1][1] = mfma(a1, b1, acc[1][1]);
+      }
     }
     __syncthreads();
   }
@@ -108,8 +123,9 @@ __global__ void wgk_kernel(const double *__restrict__ G, double *__restrict__ ds
   dst[g] = v;
 }
 
-// Zero-vertex operator sums: dst[mu][t] = sum_kap h(t, mu, kap) F_kap, h = Delta(t)[mu][kap] for a
-// forward line, -Delta_refl(t)[kap][mu] otherwise (block_sparse_backbone.cpp:216-221)
+// Zero-vertex operator sums, side by side per node: dst[t] is b1 x (qs b0) with columns mu b0 + j,
+// Fh_mu(t) = sum_kap h(t, mu, kap) F_kap, h = Delta(t)[mu][kap] for a forward line,
+// -Delta_refl(t)[kap][mu] otherwise (block_sparse_backbone.cpp:216-221)
 __global__ void wfh_kernel(double *__restrict__ dst, const double *__restrict__ F, int qs, int b1, int b0, int r, int nflav, int forward,
                            int o0, int o1, int o2, int o3, int o4, int o5, int o6, int o7, const double *__restrict__ delta,
                            const double *__restrict__ delta_refl) {
@@ -119,13 +135,14 @@ __global__ void wfh_kernel(double *__restrict__ dst, const double *__restrict__ 
   const int mu = (int)(g / (r * w));
   const int t = (int)((g / w) % r);
   const int64_t x = g % w;
+  const int c = (int)(x / b0), j = (int)(x % b0);
   const int orb[8] = {o0, o1, o2, o3, o4, o5, o6, o7};
   double s = 0.0;
   for (int k = 0; k < qs; ++k) {
     const double h = forward ? delta[((int64_t)t * nflav + orb[mu]) * nflav + orb[k]] : -delta_refl[((int64_t)t * nflav + orb[k]) * nflav + orb[mu]];
     s += h * F[(int64_t)k * w + x];
   }
-  dst[g] = s;
+  dst[(int64_t)t * qs * w + (int64_t)c * qs * b0 + (int64_t)mu * b0 + j] = s;
 }
 
 // vmax[u] = max(vmax[u], max |src[u][x]|): slices of one item per workgroup (blockIdx.y = item),
@@ -150,9 +167,10 @@ __global__ void wzero_kernel(double *dst, int64_t n) {
 }
 
 // Node-local products of an edge, fused: one 64 x 64 tile of C[u][k] and D[u][k] per workgroup,
-// f / Wf tiles and g / Wg tiles staged together through LDS.
+// f / Wf tiles and g / Wg tiles staged together through LDS, 32 of K at a time, the next K tile
+// prefetched into registers.
 __global__ void __launch_bounds__(256) wlocal_kernel(WLocal L) {
-  constexpr int TM = 64, TN = 64, TK = 16;
+  constexpr int TM = 64, TN = 64, TK = 32, PER = TM * TK / 256;
   __shared__ double Af[TK][TM + 1], Aw[TK][TM + 1], Bg[TK][TN + 1], Bw[TK][TN + 1];
   const int b = L.b, cols = L.cols;
   const int tilesN = (cols + TN - 1) / TN;
@@ -171,35 +189,51 @@ __global__ void __launch_bounds__(256) wlocal_kernel(WLocal L) {
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
     for (int bj = 0; bj < 2; ++bj) cc[bi][bj] = dd[bi][bj] = v4d{0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < b; k0 += TK) {
-    for (int x = tid; x < TM * TK; x += 256) {
+  double rf[PER], rw[PER], rg[PER], rh[PER];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int x = tid + 256 * e;
       const int i = x / TK, kk = x % TK, gi = m0 + i, gk = k0 + kk;
-      const bool ok = gi < b && gk < b;
-      Af[kk][i] = ok ? f[(int64_t)gi * b + gk] : 0.0;
-      Aw[kk][i] = ok ? wf[(int64_t)gi * b + gk] : 0.0;
+      const bool oka = gi < b && gk < b;
+      rf[e] = oka ? f[(int64_t)gi * b + gk] : 0.0;
+      rw[e] = oka ? wf[(int64_t)gi * b + gk] : 0.0;
+      const int kb = x / TN, j = x % TN, gkb = k0 + kb, gj = n0 + j;
+      const bool okb = gkb < b && gj < cols;
+      rg[e] = okb ? g[(int64_t)gkb * cols + gj] : 0.0;
+      rh[e] = okb ? wg[(int64_t)gkb * cols + gj] : 0.0;
     }
-    for (int x = tid; x < TK * TN; x += 256) {
-      const int kk = x / TN, j = x % TN, gk = k0 + kk, gj = n0 + j;
-      const bool ok = gk < b && gj < cols;
-      Bg[kk][j] = ok ? g[(int64_t)gk * cols + gj] : 0.0;
-      Bw[kk][j] = ok ? wg[(int64_t)gk * cols + gj] : 0.0;
+  };
+  fetch(0);
+  for (int k0 = 0; k0 < b; k0 += TK) {
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int x = tid + 256 * e;
+      Af[x % TK][x / TK] = rf[e];
+      Aw[x % TK][x / TK] = rw[e];
+      Bg[x / TN][x % TN] = rg[e];
+      Bw[x / TN][x % TN] = rh[e];
     }
     __syncthreads();
+    if (k0 + TK < b) fetch(k0 + TK);
+    const int kend = min(TK, b - k0);
 #pragma unroll
     for (int kk = 0; kk < TK; kk += 4) {
-      const int kr = kk + (lane >> 4), c = lane & 15;
-      const double f0 = Af[kr][wm + c], f1 = Af[kr][wm + 16 + c];
-      const double w0 = Aw[kr][wm + c], w1 = Aw[kr][wm + 16 + c];
-      const double g0 = Bg[kr][wn + c], g1 = Bg[kr][wn + 16 + c];
-      const double h0 = Bw[kr][wn + c], h1 = Bw[kr][wn + 16 + c];
-      cc[0][0] = mfma(w0, g0, mfma(f0, h0, cc[0][0]));
-      cc[0][1] = mfma(w0, g1, mfma(f0, h1, cc[0][1]));
-      cc[1][0] = mfma(w1, g0, mfma(f1, h0, cc[1][0]));
-      cc[1][1] = mfma(w1, g1, mfma(f1, h1, cc[1][1]));
-      dd[0][0] = mfma(f0, g0, dd[0][0]);
-      dd[0][1] = mfma(f0, g1, dd[0][1]);
-      dd[1][0] = mfma(f1, g0, dd[1][0]);
-      dd[1][1] = mfma(f1, g1, dd[1][1]);
+      if (kk < kend) {
+        const int kr = kk + (lane >> 4), c = lane & 15;
+        const double f0 = Af[kr][wm + c], f1 = Af[kr][wm + 16 + c];
+        const double w0 = Aw[kr][wm + c], w1 = Aw[kr][wm + 16 + c];
+        const double g0 = Bg[kr][wn + c], g1 = Bg[kr][wn + 16 + c];
+        const double h0 = Bw[kr][wn + c], h1 = Bw[kr][wn + 16 + c];
+        cc[0][0] = mfma(w0, g0, mfma(f0, h0, cc[0][0]));
+        cc[0][1] = mfma(w0, g1, mfma(f0, h1, cc[0][1]));
+        cc[1][0] = mfma(w1, g0, mfma(f1, h0, cc[1][0]));
+        cc[1][1] = mfma(w1, g1, mfma(f1, h1, cc[1][1]));
+        dd[0][0] = mfma(f0, g0, dd[0][0]);
+        dd[0][1] = mfma(f0, g1, dd[0][1]);
+        dd[1][0] = mfma(f1, g0, dd[1][0]);
+        dd[1][1] = mfma(f1, g1, dd[1][1]);
+      }
     }
     __syncthreads();
   }

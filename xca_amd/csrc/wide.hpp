@@ -45,7 +45,7 @@ enum WKind : int { W_GEMM = 0, W_INIT, W_V2C, W_GK, W_FH, W_MAX, W_FINAL, W_LOCA
 //   W_LOCAL:  the node-local products of an edge
 //   W_ZERO:   dst[0, n) = 0
 //   W_GK:     dst[e] (r x n) = G block src (r x n) times prod_q K^{ls_q}(t, pole of line q of entry e)
-//   W_FH:     dst[mu][t] (b1 x b0) = sum_kap h(t, mu, kap) F_kap (F_kap at src + kap b1 b0)
+//   W_FH:     dst[t] (b1 x qs b0), block mu = sum_kap h(t, mu, kap) F_kap (F_kap at src + kap b1 b0)
 //   W_MAX:    vmax[u] = max(vmax[u], max |src[u]|) over n doubles (vmax zeroed by W_ZERO first)
 //   W_FINAL:  dst (r b0^2, the Sigma block) += scale sum_u [vmax[u] |pref[u]| > 1e-16] pref[u] src[u]
 struct WLaunch {
