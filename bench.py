@@ -243,16 +243,11 @@ def main():
         print(json.dumps({"spgf": spgf_leg(ev, P, G, max_order, max(args.spgf, 1), cpu=False)}))
         return
     ev.set_G(G)
-    topos = [(np.array(t, dtype=np.int32), (-1) ** o * s * (-1) ** (o + 1)) for o in range(1, max_order + 1)
-             for s, t in all_connected_pairings(o)]
 
-    def step(record=None):
-        ev.reset()
-        for t, sc in topos:
-            ev.accumulate(t, sc)
-            if record is not None:
-                ev.synchronize()
-                record.append(ev.last_stats())
+    def step():
+        # Sigma over orders 1..max_order with the solver's signs: every topology's edge tables in one
+        # launch, all chunks in one persistent chain launch, one reduction; replayed as a HIP graph
+        ev.sigma_order(max_order)
 
     for _ in range(args.warmup):
         step()
@@ -264,19 +259,18 @@ def main():
     el = time.perf_counter() - t0
     ms_per_step = 1e3 * el / args.steps
 
-    # kernel-level measurement of the dominant launch (chain kernel of the highest-order topology)
+    # kernel-level measurement of the dominant launch: the chain launch of one evaluation (all
+    # topologies), bracketed by HIP events on the launch stream (eager replays of the same sequence)
     recs = []
-    nrec = min(args.steps, 50)
-    for _ in range(nrec):
-        step(recs)
-    per_topo = {}
-    for i, (items, macs, kms) in enumerate(recs):
-        per_topo.setdefault(i % len(topos), []).append((items, macs, kms))
-    dom = max(per_topo, key=lambda k: np.mean([x[2] for x in per_topo[k]]))
-    items, macs, _ = per_topo[dom][0]
-    kms = float(np.mean([x[2] for x in per_topo[dom]]))
+    for _ in range(min(args.steps, 50)):
+        ev.sigma_order(max_order, timed=True)
+        ev.synchronize()
+        recs.append(ev.last_stats())
+    items, macs, _ = recs[0]
+    kms = float(np.mean([x[2] for x in recs]))
     achieved = 2.0 * macs / (kms * 1e-3) / 1e12
-    total_macs = sum(per_topo[k][0][1] for k in per_topo)
+    total_macs = macs
+    topo_list = [t for o in range(1, max_order + 1) for _, t in all_connected_pairings(o)]
 
     sig = ev.get_sigma()
     out = {
@@ -300,13 +294,14 @@ def main():
                                         "Lambda=800, eps=1e-10, 36 (N,Sz) blocks up to 100, orders 1-2, one Sigma evaluation "
                                         "per step (large-block engine)"}[args.workload],
                    "dlr_rank": P.grid.r, "poles": len(P.poles), "blocks": [int(d) for d in P.dims],
-                   "items_per_eval": int(sum(per_topo[k][0][0] for k in per_topo)),
+                   "items_per_eval": int(items),
                    "canonical_macs_per_eval": total_macs},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
                      "traffic": measured_traffic(args.workload),
-                     "kernel": "chain_kernel" if args.workload != "oca5" else "large-block launch sequence (wgemm_kernel dominant)",
+                     "kernel": ("chain_kernel, one launch for all topologies of the evaluation" if args.workload != "oca5"
+                                else "large-block launch sequence of the evaluation (wgemm_kernel dominant)"),
                      "kernel_ms": kms,
-                     "flops_per_launch": 2.0 * macs, "topology": topos[dom][0].tolist()},
+                     "flops_per_launch": 2.0 * macs, "topologies": [[list(p) for p in t] for t in topo_list]},
     }
     if args.spgf > 0:
         out["spgf"] = spgf_leg(ev, P, G, max_order, args.spgf, cpu=not args.no_cpu and args.workload == "oca2")

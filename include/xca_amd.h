@@ -109,6 +109,17 @@ int xca_sigma_accumulate(xca_ctx *ctx, int m, const int *topo, double scale);
 int xca_get_sigma(xca_ctx *ctx, double *sigma_out);
 int xca_synchronize(xca_ctx *ctx);
 
+/* The device Sigma accumulator <- Sigma of the uploaded G summed over orders 1..max_order and their
+ * connected topologies with the solver's net factor -s_topo
+ * (python/triqs_xca/block_sparse_solver.py:597-640, the loop of eval_pseudo_particle_self_energy),
+ * in one launch sequence: the edge tables of every topology in one launch, every topology's chunks in
+ * one persistent chain launch, one per-block reduction. The sequence is captured once as a HIP graph
+ * and replayed while the zero-block pattern is unchanged; timed != 0 launches it eagerly with the
+ * chain launches bracketed by events for xca_last_stats. Asynchronous: xca_get_sigma or
+ * xca_synchronize waits for it. Equal, bit for bit, to xca_sigma_zero followed by
+ * xca_sigma_accumulate over the same topologies with the same factors. */
+int xca_sigma_order(xca_ctx *ctx, int max_order, int timed);
+
 /* Pseudo-particle Dyson solver: one dyson_it_ppsc(beta, itops, G0_block) per block
  * (c++/triqs_xca/dlr_dyson_ppsc.hpp:92-117, constructed at python/triqs_xca/block_sparse_solver.py:140).
  * G0: concatenated (r, b, b) blocks of the atomic pseudo-particle propagator. */

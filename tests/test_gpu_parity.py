@@ -206,3 +206,24 @@ def test_tca_3orb_orders_1_2_full():
             floor = oracle_noise_floor(orc, G, t, None)
             print(f"  order {order} rel_err={e:.2e} oracle_floor={floor:.2e}", flush=True)
             assert e <= max(TOL, 4 * floor)
+
+
+@pytest.mark.parametrize("case,order", [("oca_2orb", 2), ("two_band_discrete_bath", 3)])
+def test_sigma_order_one_launch_equals_topology_loop(case, order):
+    """xca_sigma_order (all topologies in one launch sequence, graph-replayed) equals the topology
+    loop bit for bit, on every replay and eagerly."""
+    P = CASES[case]()
+    G = P.perturbed_G()
+    ev = _evaluator(P)
+    ref = ev.pseudo_particle_self_energy(G, order)
+    ev.set_G(G)
+    for rep in range(3):
+        ev.sigma_order(order)
+        got = ev.get_sigma()
+        assert all(np.array_equal(a, b) for a, b in zip(got, ref)), rep
+    ev.sigma_order(order, timed=True)
+    ev.synchronize()
+    assert all(np.array_equal(a, b) for a, b in zip(ev.get_sigma(), ref))
+    items, macs, ms = ev.last_stats()
+    print(f"[gpu] sigma_order {case} items={items} chain_ms={ms:.3f}", flush=True)
+    assert items > 0 and ms > 0

@@ -36,6 +36,7 @@ EXPORTS = {
     "xca_sigma_accumulate": (C.c_int, [C.c_void_p, C.c_int, i_p, C.c_double]),
     "xca_get_sigma": (C.c_int, [C.c_void_p, d_p]),
     "xca_synchronize": (C.c_int, [C.c_void_p]),
+    "xca_sigma_order": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "xca_last_stats": (C.c_int, [C.c_void_p, i64_p, d_p, d_p]),
     "xca_server_create": (C.c_void_p, [C.c_int, C.c_int64, C.c_int, C.c_int]),
     "xca_server_destroy": (None, [C.c_void_p]),

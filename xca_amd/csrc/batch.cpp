@@ -413,6 +413,169 @@ extern "C" int xca_ppsc_solve(xca_ctx *c, double eta0, int max_order, double tol
   }
 }
 
+// ======================================================================= full-order Sigma
+
+namespace xca {
+
+// Build (or reuse) the combined launch descriptors of Sigma over orders 1..max_order for the
+// context's current zero-block pattern.
+static SigmaAll *sigma_all(xca_ctx *c, int max_order) {
+  std::vector<int> key(c->G_zero.begin(), c->G_zero.end());
+  key.push_back(max_order);
+  key.push_back(c->tile_variant);
+  if (c->sigall && c->sigall->key == key) return c->sigall.get();
+  c->sigall.reset();
+  auto S = std::make_unique<SigmaAll>();
+  S->key = key;
+  hipStream_t s = c->stream;
+  for (int order = 1; order <= max_order; ++order)
+    for (auto &st : connected_topologies(order)) {
+      DevPlan *dp = get_plan(c, order, st.second.data(), nullptr, 0);
+      if (dp->items == 0) continue;
+      if (dp->chosen < 0) {  // autotune on first use, with this G's tables
+        if (dp->ntabjobs > 0) {
+          std::vector<TabSeg> ts(1, tab_seg(c, dp));
+          std::vector<int> tp{0, dp->ntabjobs};
+          DevBuf<TabSeg> dts;
+          DevBuf<int> dtp;
+          dts.upload(ts, s);
+          dtp.upload(tp, s);
+          HIP_CHECK(launch_tab(c->RB, TabBatch{dts.ptr, dtp.ptr, 1, dp->ntabjobs}, s));
+        }
+        autotune(c, dp);
+      }
+      S->plans.push_back(dp);
+      S->scales.push_back(-(double)st.first);
+      S->items += dp->items;
+      S->macs += dp->macs;
+    }
+  S->h_tab_prefix.assign(1, 0);
+  std::map<int, size_t> group_of;  // tile variant -> group
+  for (size_t q = 0; q < S->plans.size(); ++q) {
+    DevPlan *dp = S->plans[q];
+    if (dp->ntabjobs > 0) {
+      S->h_tab.push_back(tab_seg(c, dp));
+      S->h_tab_prefix.push_back(S->h_tab_prefix.back() + dp->ntabjobs);
+    }
+    DevVariant *V = dp->var[dp->chosen].get();
+    auto g = group_of.find(dp->chosen);
+    if (g == group_of.end()) {
+      group_of[dp->chosen] = S->groups.size();
+      S->groups.push_back(std::make_unique<SigmaAll::Group>());
+      S->groups.back()->RB = c->RB;
+      S->groups.back()->variant = dp->chosen;
+      S->groups.back()->h_prefix.assign(1, 0);
+      g = group_of.find(dp->chosen);
+    }
+    SigmaAll::Group &G = *S->groups[g->second];
+    G.h_segs.push_back(chain_args(c, dp, V));
+    G.h_prefix.push_back(G.h_prefix.back() + V->nchunks);
+    S->h_red.push_back(RedPlan{V->slot.ptr, V->red_chunks.ptr, V->red_range.ptr, V->partial.ptr, S->scales[q]});
+  }
+  S->tab.upload(S->h_tab, s);
+  S->tab_prefix.upload(S->h_tab_prefix, s);
+  for (auto &gp : S->groups) {
+    SigmaAll::Group &G = *gp;
+    G.segs.upload(G.h_segs, s);
+    G.prefix.upload(G.h_prefix, s);
+    G.queue.alloc(1);
+    const int64_t stride = 16LL * G.RB * tile_columns(G.variant);
+    G.scratch.alloc((size_t)chain_resident(G.RB, G.variant) * stride);
+    G.B = ChainBatch{G.segs.ptr, G.prefix.ptr, (int)G.h_segs.size(), G.h_prefix.back(), G.queue.ptr, G.scratch.ptr, stride};
+  }
+  S->red.upload(S->h_red, s);
+  const Problem &P = c->M.P;
+  RedProb rq{};
+  rq.plans = S->red.ptr;
+  rq.nplans = (int)S->plans.size();
+  rq.sig_off = c->sig_off.ptr;
+  rq.sig_size = c->sig_size.ptr;
+  rq.sigma = c->sigma.ptr;
+  rq.nb = P.nb;
+  S->prob.upload(std::vector<RedProb>(1, rq), s);
+  S->prob_prefix.upload(std::vector<int>{0, P.nb}, s);
+  S->R = RedBatch{S->prob.ptr, S->prob_prefix.ptr, 1, P.nb};
+  for (int b = 0; b < P.nb; ++b) S->max_sig = std::max(S->max_sig, P.r * P.dims[b] * P.dims[b]);
+  HIP_CHECK(hipStreamSynchronize(s));
+  c->sigall = std::move(S);
+  return c->sigall.get();
+}
+
+// the launch sequence; events (when given) bracket the chain launches
+static hipError_t sigma_all_launch(xca_ctx *c, SigmaAll &S, hipEvent_t e0, hipEvent_t e1) {
+  hipError_t e;
+  hipStream_t s = c->stream;
+  if (S.plans.empty()) return launch_fill(c->M.G_size, 0.0, c->sigma.ptr, s);
+  if (!S.h_tab.empty() &&
+      (e = launch_tab(c->RB, TabBatch{S.tab.ptr, S.tab_prefix.ptr, (int)S.h_tab.size(), S.h_tab_prefix.back()}, s)) != hipSuccess)
+    return e;
+  if (e0 && (e = hipEventRecord(e0, s)) != hipSuccess) return e;
+  for (auto &g : S.groups)
+    if ((e = launch_chain(g->RB, g->variant, g->B, s)) != hipSuccess) return e;
+  if (e1 && (e = hipEventRecord(e1, s)) != hipSuccess) return e;
+  return launch_reduce_assign(S.R, S.max_sig, s);
+}
+
+}  // namespace xca
+
+extern "C" int xca_sigma_order(xca_ctx *c, int max_order, int timed) {
+  try {
+    if (!c) throw std::invalid_argument("xca: null context");
+    if (max_order < 1 || max_order > 4) throw std::invalid_argument("xca_sigma_order: max_order must be between 1 and 4");
+    if (!c->have_G) throw std::runtime_error("xca: G has not been set");
+    HIP_CHECK(hipSetDevice(c->device));
+    if (c->wide) {  // large blocks: topology by topology on the large-block engine, timed as a whole
+      HIP_CHECK(launch_fill(c->M.G_size, 0.0, c->sigma.ptr, c->stream));
+      int64_t items = 0;
+      double macs = 0;
+      c->outer_events = true;
+      HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+      try {
+        for (int order = 1; order <= max_order; ++order)
+          for (auto &st : connected_topologies(order)) {
+            accumulate(c, order, st.second.data(), nullptr, 0, -(double)st.first);
+            items += c->last_items;
+            macs += c->last_macs;
+          }
+      } catch (...) {
+        c->outer_events = false;
+        throw;
+      }
+      c->outer_events = false;
+      HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+      c->last_items = items;
+      c->last_macs = macs;
+      c->last_ms = 0;
+      return 0;
+    }
+    SigmaAll *S = sigma_all(c, max_order);
+    c->last_items = S->items;
+    c->last_macs = S->macs;
+    c->last_ms = 0;
+    if (timed || std::getenv("XCA_NO_GRAPH")) {
+      HIP_CHECK(sigma_all_launch(c, *S, timed ? c->ev0 : nullptr, timed ? c->ev1 : nullptr));
+      return 0;
+    }
+    if (!S->exec) {
+      HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+      hipError_t e = sigma_all_launch(c, *S, nullptr, nullptr);
+      hipGraph_t g = nullptr;
+      hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+      if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        throw std::runtime_error(std::string("xca: graph capture failed: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+      }
+      S->graph = g;
+      HIP_CHECK(hipGraphInstantiate(&S->exec, S->graph, nullptr, nullptr, 0));
+    }
+    HIP_CHECK(hipGraphLaunch(S->exec, c->stream));
+    return 0;
+  } catch (std::exception &e) {
+    xca::set_error(e.what());
+    return -1;
+  }
+}
+
 // ================================================================================== server
 
 struct XcaRequest {

@@ -778,6 +778,7 @@ extern "C" void xca_ctx_destroy(xca_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->plans.clear();
   c->corr_plans.clear();
+  c->sigall.reset();
   c->wide_plans.clear();
   c->widectx.reset();
   xca_dyson_release(c);
@@ -860,6 +861,7 @@ extern "C" int xca_set_tile_variant(xca_ctx *c, int variant) {
     return -1;
   }
   c->tile_variant = variant;
+  c->sigall.reset();
   c->plans.clear();
   c->corr_plans.clear();
   return 0;

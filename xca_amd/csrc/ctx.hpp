@@ -48,6 +48,7 @@ struct DevPlan;
 struct CorrDevPlan;
 struct WidePlan;
 struct WideCtx;
+struct SigmaAll;
 
 }  // namespace xca
 
@@ -76,8 +77,10 @@ struct xca_ctx {
   Plans plans;
   std::map<std::vector<int>, std::unique_ptr<xca::CorrDevPlan>> corr_plans;
   bool wide = false;  // blocks above the tile engine's 12 rows: the large-block engine evaluates Sigma
+  bool outer_events = false;  // ev0/ev1 bracket a sequence of evaluations: inner launches leave them
   std::map<std::vector<int>, std::unique_ptr<xca::WidePlan>> wide_plans;
   std::unique_ptr<xca::WideCtx> widectx;
+  std::unique_ptr<xca::SigmaAll> sigall;
   xca::DevBuf<double> corr_out;
   XcaDyson *dyson = nullptr;
   int64_t last_items = 0;

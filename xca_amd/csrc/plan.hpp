@@ -128,7 +128,45 @@ struct WideCtx {
   DevBuf<double> M2;
 };
 
+// Σ over orders 1..max_order of one context in one launch sequence: the edge tables of every plan
+// in one launch, the chunks of every plan in one persistent chain launch per tile width, and one
+// per-block reduction with the topology factors. Replayed as a HIP graph while the plan set holds.
+struct SigmaAll {
+  struct Group {
+    int RB = 0, variant = 0;
+    std::vector<ChainArgs> h_segs;
+    std::vector<int> h_prefix;
+    DevBuf<ChainArgs> segs;
+    DevBuf<int> prefix, queue;
+    DevBuf<double> scratch;
+    ChainBatch B{};
+  };
+  std::vector<int> key;
+  std::vector<DevPlan *> plans;
+  std::vector<double> scales;
+  std::vector<TabSeg> h_tab;
+  std::vector<int> h_tab_prefix;
+  DevBuf<TabSeg> tab;
+  DevBuf<int> tab_prefix;
+  std::vector<std::unique_ptr<Group>> groups;
+  std::vector<RedPlan> h_red;
+  DevBuf<RedPlan> red;
+  DevBuf<RedProb> prob;
+  DevBuf<int> prob_prefix;
+  RedBatch R{};
+  int max_sig = 0;
+  int64_t items = 0;
+  double macs = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~SigmaAll() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
 DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t nfix);
+void accumulate(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t nfix, double scale);
 void wide_accumulate(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t nfix, double scale);
 int chunk_tiles(int64_t total_tiles);
 double canonical_macs(const Program &pg, int r);

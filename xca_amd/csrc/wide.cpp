@@ -497,9 +497,9 @@ void wide_accumulate(xca_ctx *c, int m, const int *topo, const int32_t *fix, int
   c->last_ms = 0;
   c->last_variant = -1;
   if (wp->items == 0) return;
-  HIP_CHECK(hipEventRecord(c->ev0, s));
+  if (!c->outer_events) HIP_CHECK(hipEventRecord(c->ev0, s));
   for (const WLaunch &L : wp->launches) HIP_CHECK(launch_wide(L, A, scale, s));
-  HIP_CHECK(hipEventRecord(c->ev1, s));
+  if (!c->outer_events) HIP_CHECK(hipEventRecord(c->ev1, s));
 }
 
 }  // namespace xca

@@ -206,6 +206,11 @@ class DiagramEvaluator:
         check(lib().xca_last_tile(self._h, C.byref(v), C.byref(cols)))
         return v.value, cols.value
 
+    def sigma_order(self, max_order, timed=False):
+        """Device Sigma <- Sigma of the uploaded G over orders 1..max_order with the solver's signs, in
+        one graph-replayed launch sequence (asynchronous; get_sigma or synchronize waits)."""
+        check(lib().xca_sigma_order(self._h, int(max_order), 1 if timed else 0))
+
     def pseudo_particle_self_energy(self, G, max_order):
         """Sigma summed over orders 1..max_order and connected topologies with the solver's signs
         (block_sparse_solver.py:597-638: net factor -s_topo per topology)."""
