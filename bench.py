@@ -164,6 +164,55 @@ def serving(n, seed=7):
             "iterations_run": out["stats"]["ticks"], "max_active_slots": 64}
 
 
+def plain_gpu_leg(ev, G, max_order, batched_per_s, wide, budget_s=6.0, sample=3000):
+    """The plain one-problem-at-a-time GPU forms of the same Sigma evaluation, beside the batched value:
+    - per_topology_calls: the solver's topology loop through the drop-in call
+      compute_self_energy(G, topology) with host arrays, one topology after another
+      (block_sparse_solver.py:629-640);
+    - per_item_launches: one chain launch per (path, pole tuple) item (launch mode 1 on explicit
+      flat-index subsets), timed on a seeded sample of flat indices per topology and scaled by the
+      topology's item count, since path cost does not depend on the poles. Not applicable to the
+      large-block engine, whose launches are per program already."""
+    from xca_amd.diag import all_connected_pairings
+    topos = [(s, np.array(t, dtype=np.int32)) for o in range(1, max_order + 1) for s, t in all_connected_pairings(o)]
+    items = []
+    for _, t in topos:  # plans built, item counts
+        ev.compute_self_energy(G, t)
+        items.append(ev.last_stats()[0])
+    reps, t0 = 0, time.perf_counter()
+    while reps == 0 or (time.perf_counter() - t0 < budget_s and reps < 50):
+        for _, t in topos:
+            ev.compute_self_energy(G, t)
+        reps += 1
+    per_topo = reps / (time.perf_counter() - t0)
+    out = {"per_topology_calls": {"evals_per_s": per_topo, "reps": reps, "batched_over_plain": batched_per_s / per_topo}}
+    if wide:
+        out["per_item_launches"] = None
+        return out
+    rng = np.random.default_rng(13)
+    ev.set_launch_mode(1)
+    s_per_eval, sampled, launches_s = 0.0, 0, 0.0
+    try:
+        for (_, t), n_items in zip(topos, items):
+            nf = ev.get_num_self_energy_backbones(t)
+            fix = np.arange(nf, dtype=np.int32) if nf <= sample else np.sort(rng.choice(nf, size=sample, replace=False)).astype(np.int32)
+            ev.compute_self_energy(G, t, fix)  # builds the subset plan
+            t1 = time.perf_counter()
+            ev.compute_self_energy(G, t, fix)
+            dt = time.perf_counter() - t1
+            k = ev.last_stats()[0]
+            if k > 0:
+                s_per_eval += dt / k * n_items
+                sampled += k
+                launches_s += dt
+    finally:
+        ev.set_launch_mode(0)
+    out["per_item_launches"] = {"evals_per_s": 1.0 / s_per_eval, "s_per_eval": s_per_eval, "items_timed": sampled,
+                                "timed_s": launches_s, "batched_over_plain": batched_per_s * s_per_eval,
+                                "sample": f"up to {sample} seeded flat indices per topology, scaled by its item count"}
+    return out
+
+
 def spgf_leg(ev, P, G, max_order, n, cpu):
     """Single-particle Green's function g(tau) of the same workload and G: every connected topology of
     orders 1..max_order with the solver's signs (block_sparse_solver.py:657-677), timed over n
@@ -221,6 +270,7 @@ def main():
     ap.add_argument("--solves", type=int, default=50, help="NCA self-consistent solves timed for the latency figures (0: skip)")
     ap.add_argument("--serving", type=int, default=512, help="requests in the served stream (0: skip)")
     ap.add_argument("--spgf", type=int, default=5, help="timed single-particle Green's function evaluations (0: skip)")
+    ap.add_argument("--plain", type=int, default=1, help="time the plain one-problem-at-a-time GPU forms (0: skip)")
     ap.add_argument("--spgf-only", action="store_true", help="run only the single-particle Green's function leg (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (reference dispatches for the HBM byte counters)")
@@ -303,6 +353,12 @@ def main():
                      "kernel_ms": kms,
                      "flops_per_launch": 2.0 * macs, "topologies": [[list(p) for p in t] for t in topo_list]},
     }
+    if args.plain:
+        out["plain_gpu"] = plain_gpu_leg(ev, G, max_order, out["value"], wide=args.workload == "oca5")
+        pg = out["plain_gpu"]
+        log(f"plain GPU forms: per-topology calls {pg['per_topology_calls']['evals_per_s']:.2f} evals/s"
+            + (f", per-item launches {pg['per_item_launches']['evals_per_s']:.4g} evals/s" if pg["per_item_launches"] else ""))
+        ev.set_G(G)
     if args.spgf > 0:
         out["spgf"] = spgf_leg(ev, P, G, max_order, args.spgf, cpu=not args.no_cpu and args.workload == "oca2")
         log(f"spgf {out['spgf']['evals_per_s']:.1f} evals/s, dominant chain {out['spgf']['dominant']['frac']:.3f} of peak")

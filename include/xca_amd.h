@@ -177,6 +177,13 @@ int xca_server_stats(xca_server *sv, int64_t *ticks, int64_t *bytes_in_use, int6
  * environment variable XCA_TILE_VARIANT sets the initial value. Changing it drops cached plans.
  * No reference counterpart (the reference has no device tiles). */
 int xca_set_tile_variant(xca_ctx *ctx, int variant);
+/* Launch form of xca_sigma / xca_sigma_accumulate on blocks of at most 12 rows: 0 (the default)
+ * drains every chunk of a plan in one persistent chain launch; 1 launches the chain kernel once per
+ * chunk, and a call with an explicit flat-index subset has one chunk per (path, pole tuple) item, so
+ * mode 1 with a subset is the one-backbone-term-per-launch form. Results are identical in both
+ * modes. xca_sigma_order always uses the combined launch. No reference counterpart (a GPU launch
+ * form; the reference evaluates backbone terms one after another on the CPU). */
+int xca_set_launch_mode(xca_ctx *ctx, int mode);
 /* Variant and columns per tile used by the last accumulate call. */
 int xca_last_tile(xca_ctx *ctx, int *variant, int *columns);
 

@@ -227,3 +227,28 @@ def test_sigma_order_one_launch_equals_topology_loop(case, order):
     items, macs, ms = ev.last_stats()
     print(f"[gpu] sigma_order {case} items={items} chain_ms={ms:.3f}", flush=True)
     assert items > 0 and ms > 0
+
+
+@pytest.mark.parametrize("case,order", [("oca_2orb", 2), ("two_band_discrete_bath", 3)])
+def test_per_item_launch_form_equals_batched(case, order):
+    """Launch mode 1 (one chain launch per chunk; with an explicit flat-index subset, one launch per
+    (path, pole tuple) item) gives the batched launch's Sigma bit for bit."""
+    P = CASES[case]()
+    G = P.perturbed_G()
+    ev = _evaluator(P)
+    rng = np.random.default_rng(3)
+    for _, topo in all_connected_pairings(order):
+        t = np.array(topo, dtype=np.int32)
+        nf = ev.get_num_self_energy_backbones(t)
+        sub = np.sort(rng.choice(nf, size=min(nf, 400), replace=False)).astype(np.int32)
+        ev.set_launch_mode(0)
+        full0, sub0 = ev.compute_self_energy(G, t), ev.compute_self_energy(G, t, sub)
+        ev.set_launch_mode(1)
+        full1, sub1 = ev.compute_self_energy(G, t), ev.compute_self_energy(G, t, sub)
+        items, _, ms = ev.last_stats()
+        print(f"[gpu] per-item launches {case} topo={topo} items={items} ms={ms:.3f}", flush=True)
+        assert all(np.array_equal(a, b) for a, b in zip(full0, full1))
+        assert all(np.array_equal(a, b) for a, b in zip(sub0, sub1))
+    ev.set_launch_mode(0)
+    with pytest.raises(Exception, match="mode must be 0 or 1"):
+        ev.set_launch_mode(2)

@@ -47,6 +47,7 @@ EXPORTS = {
     "xca_server_request_bytes": (C.c_int64, [C.c_void_p, C.c_int64]),
     "xca_server_stats": (C.c_int, [C.c_void_p, i64_p, i64_p, i64_p]),
     "xca_set_tile_variant": (C.c_int, [C.c_void_p, C.c_int]),
+    "xca_set_launch_mode": (C.c_int, [C.c_void_p, C.c_int]),
     "xca_last_tile": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "xca_dyson_init": (C.c_int, [C.c_void_p, d_p]),
     "xca_dyson_solve": (C.c_int, [C.c_void_p, d_p, C.c_double, d_p]),

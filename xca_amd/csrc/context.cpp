@@ -257,6 +257,15 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
   }
   auto it = c->plans.find(key);
   if (it != c->plans.end()) return it->second.get();
+  if (fix)  // one explicit-subset plan at a time: a caller sweeping flat indices does not accumulate plans
+    for (auto e = c->plans.begin(); e != c->plans.end();) {
+      if (std::find(e->first.begin(), e->first.end(), -1) == e->first.end()) {
+        ++e;
+        continue;
+      }
+      HIP_CHECK(hipStreamSynchronize(c->stream));  // its launches may still be in flight
+      e = c->plans.erase(e);
+    }
 
   const Problem &P = c->M.P;
   const int r = P.r, p = P.p, n = P.n;
@@ -569,10 +578,25 @@ void accumulate(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t 
     V->seg.upload(V->h_seg, s);
     V->seg_prefix.upload(V->h_seg_prefix, s);
   }
-  HIP_CHECK(hipEventRecord(c->ev0, s));
   const int64_t stride = (int64_t)c->RP * V->NT;
   if (!V->scratch.ptr) V->scratch.alloc((size_t)chain_resident(c->RB, dp->chosen) * stride);
-  HIP_CHECK(launch_chain(c->RB, dp->chosen, ChainBatch{V->seg.ptr, V->seg_prefix.ptr, 1, V->nchunks, dp->queue.ptr, V->scratch.ptr, stride}, s));
+  if (c->launch_mode == 1 && !V->item_seg.ptr) {
+    std::vector<ChainArgs> a(V->nchunks, V->h_seg[0]);
+    for (int k = 0; k < V->nchunks; ++k) {
+      a[k].chunks = V->chunks.ptr + k;
+      a[k].nchunks = 1;
+    }
+    V->item_seg.upload(a, s);
+    V->item_prefix.upload(std::vector<int>{0, 1}, s);
+  }
+  HIP_CHECK(hipEventRecord(c->ev0, s));
+  if (c->launch_mode == 1) {
+    // one launch per chunk; an explicit flat-index subset has one chunk per (path, pole tuple) item
+    for (int k = 0; k < V->nchunks; ++k)
+      HIP_CHECK(launch_chain(c->RB, dp->chosen, ChainBatch{V->item_seg.ptr + k, V->item_prefix.ptr, 1, 1, dp->queue.ptr, V->scratch.ptr, stride}, s));
+  } else {
+    HIP_CHECK(launch_chain(c->RB, dp->chosen, ChainBatch{V->seg.ptr, V->seg_prefix.ptr, 1, V->nchunks, dp->queue.ptr, V->scratch.ptr, stride}, s));
+  }
   HIP_CHECK(hipEventRecord(c->ev1, s));
   int maxsz = 0;
   for (int b = 0; b < c->M.P.nb; ++b) maxsz = std::max(maxsz, c->M.P.r * c->M.P.dims[b] * c->M.P.dims[b]);
@@ -864,6 +888,16 @@ extern "C" int xca_set_tile_variant(xca_ctx *c, int variant) {
   c->sigall.reset();
   c->plans.clear();
   c->corr_plans.clear();
+  return 0;
+}
+
+extern "C" int xca_set_launch_mode(xca_ctx *c, int mode) {
+  if (!c) return -1;
+  if (mode < 0 || mode > 1) {
+    xca::set_error("xca_set_launch_mode: mode must be 0 or 1");
+    return -1;
+  }
+  c->launch_mode = mode;
   return 0;
 }
 

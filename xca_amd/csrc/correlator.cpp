@@ -131,6 +131,15 @@ static CorrDevPlan *get_corr_plan(xca_ctx *c, int m, const int *topo, const int3
   }
   auto found = c->corr_plans.find(key);
   if (found != c->corr_plans.end()) return found->second.get();
+  if (fix)  // one explicit-subset plan at a time
+    for (auto e = c->corr_plans.begin(); e != c->corr_plans.end();) {
+      if (std::find(e->first.begin(), e->first.end(), -1) == e->first.end()) {
+        ++e;
+        continue;
+      }
+      HIP_CHECK(hipStreamSynchronize(c->stream));  // its launches may still be in flight
+      e = c->corr_plans.erase(e);
+    }
 
   if (m < 1 || m > kMaxLines + 1) throw std::invalid_argument("xca: diagram order must be between 1 and 4");
   CorrPlanHost hp;

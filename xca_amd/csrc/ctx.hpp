@@ -64,6 +64,7 @@ struct xca_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int RB = 0, RP = 0, maxexp = 8;
   int tile_variant = -1;  // forced tile variant, -1: autotuned per plan
+  int launch_mode = 0;    // 0: one persistent chain launch per plan; 1: one chain launch per chunk
   int last_variant = -1;
   std::vector<int> plist_h;
   int plist_off[4];  // fwd_a, fwd_b, bwd_a, bwd_b

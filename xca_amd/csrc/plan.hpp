@@ -53,6 +53,9 @@ struct DevVariant {
   DevBuf<int> seg_prefix;
   std::vector<ChainArgs> h_seg;
   std::vector<int> h_seg_prefix;
+  // per-chunk launch descriptors (launch mode 1): entry k covers chunk k alone
+  DevBuf<ChainArgs> item_seg;
+  DevBuf<int> item_prefix;
 };
 
 // Everything one (topology, zero-block pattern, flat-index subset) needs on the device: the op list

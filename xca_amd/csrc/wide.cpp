@@ -80,6 +80,15 @@ static WidePlan *get_wide_plan(xca_ctx *c, int m, const int *topo, const int32_t
   }
   auto found = c->wide_plans.find(key);
   if (found != c->wide_plans.end()) return found->second.get();
+  if (fix)  // one explicit-subset plan at a time
+    for (auto e = c->wide_plans.begin(); e != c->wide_plans.end();) {
+      if (std::find(e->first.begin(), e->first.end(), -1) == e->first.end()) {
+        ++e;
+        continue;
+      }
+      HIP_CHECK(hipStreamSynchronize(c->stream));  // its launches may still be in flight
+      e = c->wide_plans.erase(e);
+    }
   if (m < 1 || m > kMaxLines + 1) throw std::invalid_argument("xca: diagram order must be between 1 and 4");
   init_wide_ctx(c);
   const Problem &P = c->M.P;

@@ -200,6 +200,11 @@ class DiagramEvaluator:
         """Force the chain kernel's tile width (0, 1, 2 = 128, 64, 32 columns) or -1 for the autotuner."""
         check(lib().xca_set_tile_variant(self._h, int(variant)))
 
+    def set_launch_mode(self, mode):
+        """0: one persistent chain launch per plan (default); 1: one chain launch per chunk, which with
+        an explicit flat-index subset is one launch per (path, pole tuple) item. Same results."""
+        check(lib().xca_set_launch_mode(self._h, int(mode)))
+
     def last_tile(self):
         v = C.c_int(0)
         cols = C.c_int(0)
