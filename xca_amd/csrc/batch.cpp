@@ -450,27 +450,42 @@ static SigmaAll *sigma_all(xca_ctx *c, int max_order) {
       S->macs += dp->macs;
     }
   S->h_tab_prefix.assign(1, 0);
-  std::map<int, size_t> group_of;  // tile variant -> group
+  // Every plan takes the tile width of the costliest one (by the autotuner's estimate) where it has
+  // that variant, so the small plans' chunks fill the idle workgroups of the costliest plan's last
+  // round instead of running as a launch of their own; the costliest plans are queued first.
+  std::vector<size_t> order(S->plans.size());
+  for (size_t q = 0; q < order.size(); ++q) order[q] = q;
+  auto est = [&](size_t q) { return S->plans[q]->var[S->plans[q]->chosen]->ms; };
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return est(a) > est(b); });
+  const int vdom = order.empty() ? -1 : S->plans[order[0]]->chosen;
+  std::vector<DevVariant *> pv(S->plans.size());
   for (size_t q = 0; q < S->plans.size(); ++q) {
     DevPlan *dp = S->plans[q];
+    pv[q] = (vdom >= 0 && dp->var[vdom]) ? dp->var[vdom].get() : dp->var[dp->chosen].get();
     if (dp->ntabjobs > 0) {
       S->h_tab.push_back(tab_seg(c, dp));
       S->h_tab_prefix.push_back(S->h_tab_prefix.back() + dp->ntabjobs);
     }
-    DevVariant *V = dp->var[dp->chosen].get();
-    auto g = group_of.find(dp->chosen);
+    DevVariant *V = pv[q];
+    S->h_red.push_back(RedPlan{V->slot.ptr, V->red_chunks.ptr, V->red_range.ptr, V->partial.ptr, S->scales[q]});
+  }
+  std::map<int, size_t> group_of;  // tile variant -> group
+  for (size_t q : order) {
+    DevPlan *dp = S->plans[q];
+    DevVariant *V = pv[q];
+    const int v = (int)(std::find_if(dp->var.begin(), dp->var.end(), [&](const auto &x) { return x.get() == V; }) - dp->var.begin());
+    auto g = group_of.find(v);
     if (g == group_of.end()) {
-      group_of[dp->chosen] = S->groups.size();
+      group_of[v] = S->groups.size();
       S->groups.push_back(std::make_unique<SigmaAll::Group>());
       S->groups.back()->RB = c->RB;
-      S->groups.back()->variant = dp->chosen;
+      S->groups.back()->variant = v;
       S->groups.back()->h_prefix.assign(1, 0);
-      g = group_of.find(dp->chosen);
+      g = group_of.find(v);
     }
     SigmaAll::Group &G = *S->groups[g->second];
     G.h_segs.push_back(chain_args(c, dp, V));
     G.h_prefix.push_back(G.h_prefix.back() + V->nchunks);
-    S->h_red.push_back(RedPlan{V->slot.ptr, V->red_chunks.ptr, V->red_range.ptr, V->partial.ptr, S->scales[q]});
   }
   S->tab.upload(S->h_tab, s);
   S->tab_prefix.upload(S->h_tab_prefix, s);

@@ -528,9 +528,10 @@ void autotune(xca_ctx *c, DevPlan *dp) {
   if (best < 0) throw std::runtime_error("xca: no tile variant applies to this plan");
   HIP_CHECK(hipStreamSynchronize(s));
   dp->chosen = best;
-  // release the other variants' buffers
+  // release the other variants' buffers, except small ones (a combined launch of several plans
+  // may run a small plan at the tile width of a larger one)
   for (int v = 0; v < (int)dp->var.size(); ++v)
-    if (v != best && dp->var[v]) {
+    if (v != best && dp->var[v] && variant_bytes(*dp->var[v]) > (16 << 20)) {
       dp->device_bytes -= variant_bytes(*dp->var[v]);
       dp->var[v].reset();
     }
