@@ -215,8 +215,12 @@ DOp device_op(const Op &op, const std::vector<EdgeTable> &tables, const xca_ctx 
     for (int q = 0; q < op.nlines; ++q) o.estride[q] = ipow(P.p, q) * esz;
   }
   o.mubase = op.mubase; o.qs = op.qs; o.forward = op.forward; o.b1 = op.b1; o.set = op.set;
-  if (op.type == OP_Z)
+  if (op.type == OP_Z) {
+    // the device reads the tabulated kappa sums Fh (r, qs, b1, b0) in place of the kappa operators
+    if (op.fhbase < 0) throw std::logic_error("xca: zero-vertex kappa sums are not tabulated for this problem");
+    o.fbase = op.fhbase;
     for (int k = 0; k < op.qs; ++k) o.orb[k] = c->M.S.to_orb[op.set][k];
+  }
   return o;
 }
 

@@ -631,29 +631,21 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
         __syncthreads();
       } else {  // zero-vertex pair
         const int bv = op.bin, bvp = op.bout, b1 = op.b1, b0 = op.cols, qs = op.qs, win = bv * b1, wout = bvp * b0;
-        const int n = A.n;
-        const int *orbs = A.ops[P.op0 + oi].orb;
-        // out(t)[i][j] = sum_mu sum_a Fm_mu[i][a] sum_c T(t)[a][c] Fh_mu(t)[c][j] with
-        // Fh_mu(t) = sum_kk Delta(t)[mu][kk] F_kk: the kappa sum folded into one column of F per mu
-        // (registers), the mu operators read with wave-uniform addresses
+        // out(t)[i][j] = sum_mu sum_a Fm_mu[i][a] sum_c T(t)[a][c] Fh_mu(t)[c][j] with the kappa
+        // sums Fh_mu(t) = sum_kk h(t)[mu][kk] F_kk tabulated per (set, direction, block) at fbase,
+        // (r, qs, b1, b0); one column of Fh_mu(t) in registers, the mu operators read with
+        // wave-uniform addresses
         for (int x = tid; x < r * nit * b0; x += NT) {
           const int t = x / (nit * b0), rem = x % (nit * b0), it = rem / b0, j = rem % b0;
           const double *Tt = Scur + (size_t)t * LDN + it * win;
+          const double *Fht = A.opmats + op.fbase + (size_t)t * qs * b1 * b0 + j;
           double out[kMaxB];
 #pragma unroll
           for (int i = 0; i < kMaxB; ++i) out[i] = 0.0;
           for (int mu = 0; mu < qs; ++mu) {
             double fh[kMaxB];
 #pragma unroll
-            for (int c = 0; c < kMaxB; ++c) fh[c] = 0.0;
-            for (int kk = 0; kk < qs; ++kk) {
-              const double hv = op.forward ? A.delta[((size_t)t * n + orbs[mu]) * n + orbs[kk]]
-                                           : -A.delta_refl[((size_t)t * n + orbs[kk]) * n + orbs[mu]];
-              const double *Fk = A.opmats + op.fbase + (int64_t)kk * b1 * b0 + j;
-#pragma unroll
-              for (int c = 0; c < kMaxB; ++c)
-                if (c < b1) fh[c] += hv * Fk[c * b0];
-            }
+            for (int c = 0; c < kMaxB; ++c) fh[c] = (c < b1) ? Fht[(size_t)(mu * b1 + c) * b0] : 0.0;
             const double *Fm = A.opmats + op.mubase + (int64_t)mu * bvp * bv;
             for (int a = 0; a < bv; ++a) {
               double y = 0.0;

@@ -190,6 +190,33 @@ void Model::init(const Problem &Pin) {
         for (int x = 0; x < n * n; ++x) delta_refl[(size_t)t * n * n + x] += kk * c[(size_t)k * n * n + x];
       }
   }
+  // zero-vertex kappa sums (the Delta contraction of the pair's two operators, independent of G
+  // and of the pole tuple), accumulated in kappa order with fused multiply-adds
+  off_Fh.assign(off_F.size(), -1);
+  if (*std::max_element(P.dims.begin(), P.dims.end()) <= 12)
+    for (int s = 0; s < S.q; ++s)
+      for (int dag = 0; dag < 2; ++dag)
+        for (int j = 0; j < nb; ++j) {
+          const size_t key = ((size_t)s * 2 + dag) * nb + j;
+          const int tb = conn_set[key];
+          if (tb < 0) continue;
+          const int qs = S.size[s], b1 = P.dims[tb], b0 = P.dims[j];
+          const size_t sz = (size_t)b1 * b0;
+          off_Fh[key] = (int64_t)opmats.size();
+          const size_t base = opmats.size();
+          opmats.resize(base + (size_t)r * qs * sz, 0.0);
+          for (int t = 0; t < r; ++t)
+            for (int mu = 0; mu < qs; ++mu)
+              for (size_t x = 0; x < sz; ++x) {
+                double acc = 0.0;
+                for (int kk = 0; kk < qs; ++kk) {
+                  const int om = S.to_orb[s][mu], ok = S.to_orb[s][kk];
+                  const double h = dag == 0 ? delta[((size_t)t * n + om) * n + ok] : -delta_refl[((size_t)t * n + ok) * n + om];
+                  acc = std::fma(h, opmats[off_F[key] + kk * sz + x], acc);
+                }
+                opmats[base + ((size_t)t * qs + mu) * sz + x] = acc;
+              }
+        }
   G_off.assign(nb, 0);
   G_size = 0;
   Nmax = 0;
@@ -377,6 +404,7 @@ void build_programs(const Model &M, const PoleSets &ps, const int *topo, int m, 
               z.cols = bd[0];
               // kappa operators: (forward ? F : F^dag)[s] at block b_ix, (b1 x b0) each
               z.fbase = M.F_offset(s, z.forward ? 0 : 1, b_ix);
+              z.fhbase = M.Fh_offset(s, z.forward ? 0 : 1, b_ix);
               // mu operators: (forward ? F^dag : F)[s] at block ind[vct0-1], (bvp x bv) each
               z.mubase = M.F_offset(s, z.forward ? 1 : 0, ind[vct0 - 1]);
               if (z.fbase < 0 || z.mubase < 0) throw std::logic_error("missing zero-vertex operator block");

@@ -55,8 +55,9 @@ struct Op {
   int lines[3] = {-1, -1, -1};
   int lsign[3] = {0, 0, 0};
   int etab = -1;       // index into the table list
-  // Z: kappa operators (b1 x b0) at fbase, mu operators (bvp x bv) at mubase, set size qs
-  int64_t mubase = 0;
+  // Z: kappa operators (b1 x b0) at fbase, mu operators (bvp x bv) at mubase, set size qs;
+  // fhbase: the kappa sums Fh (r, qs, b1, b0) of the same (set, dag, block) (-1: not tabulated)
+  int64_t mubase = 0, fhbase = -1;
   int qs = 0, forward = 0, set = -1, b1 = 0;
   int tr = 0;  // V, E: transposed operator / edge function (the [beta, tau] side of a correlator)
 };
@@ -100,6 +101,10 @@ struct Model {
                                 // then the same blocks transposed ((q_s, bin, bout) and (q_s, p, bin, bout))
   // offsets into opmats: [set][dag][block], -1 if absent
   std::vector<int64_t> off_F, off_bar, off_FT, off_barT;
+  // zero-vertex kappa sums Fh(t, mu) = sum_kappa h(t, mu, kappa) F_kappa, (r, q_s, bout, bin) per
+  // [set][dag][block] (dag 0: h = Delta(t)[mu][kappa]; dag 1: h = -Delta_refl(t)[kappa][mu]);
+  // tabulated only when every block has at most 12 rows (the tile engine's operand)
+  std::vector<int64_t> off_Fh;
   std::vector<int> conn_set;    // [set*2+dag][block] target block or -1
   std::vector<int> G_off;       // offsets (in doubles) of G blocks in a packed (r,b,b) array
   int64_t G_size = 0;
@@ -110,6 +115,7 @@ struct Model {
   int64_t bar_offset(int set, int dag, int block) const { return off_bar[((size_t)set * 2 + dag) * P.nb + block]; }
   int64_t F_offset_T(int set, int dag, int block) const { return off_FT[((size_t)set * 2 + dag) * P.nb + block]; }
   int64_t bar_offset_T(int set, int dag, int block) const { return off_barT[((size_t)set * 2 + dag) * P.nb + block]; }
+  int64_t Fh_offset(int set, int dag, int block) const { return off_Fh[((size_t)set * 2 + dag) * P.nb + block]; }
   int conn(int set, int dag, int block) const { return conn_set[((size_t)set * 2 + dag) * P.nb + block]; }
   int64_t num_backbones(int m) const;
 };
