@@ -20,21 +20,25 @@ __device__ __forceinline__ static int find_seg(const int *prefix, int nsegs, int
   return lo;
 }
 
-// in-place LU solve of one column (stride between nodes) with the DLR interpolation factors
-__device__ void lu_solve_col(const double *__restrict__ lu, const int *__restrict__ piv, int r, double *x, int stride) {
+// in-place LU solve of one column (stride between nodes) with the DLR interpolation factors; the
+// kernels call it on LDS copies of the column and the factors
+__device__ __forceinline__ void lu_solve_col(const double *__restrict__ lu, const int *__restrict__ piv, int r, double *x, int stride) {
   for (int k = 0; k < r; ++k)
     if (piv[k] != k) {
       double a = x[(size_t)k * stride], b = x[(size_t)piv[k] * stride];
       x[(size_t)k * stride] = b;
       x[(size_t)piv[k] * stride] = a;
     }
+  // the products are accumulated in index order; unrolling lets the loads of 8 terms issue together
   for (int i = 1; i < r; ++i) {
     double s = x[(size_t)i * stride];
+#pragma unroll 8
     for (int k = 0; k < i; ++k) s -= lu[i * r + k] * x[(size_t)k * stride];
     x[(size_t)i * stride] = s;
   }
   for (int i = r - 1; i >= 0; --i) {
     double s = x[(size_t)i * stride];
+#pragma unroll 8
     for (int k = i + 1; k < r; ++k) s -= lu[i * r + k] * x[(size_t)k * stride];
     x[(size_t)i * stride] = s / lu[i * r + i];
   }
@@ -46,6 +50,14 @@ __device__ __forceinline__ v4d mfma16x16x4(double a, double b, v4d c) { return _
 
 constexpr int kDysonThreads = 1024;
 constexpr int kDysonMaxNp = 784;  // padded r n: DLR rank <= 64, blocks <= 12
+constexpr int kMaxRank = 64;      // DLR rank limit of the device path (checked at context creation)
+
+// Orders the LDS accesses of one wave around this point (a wave executes its LDS operations in
+// issue order; this keeps the compiler from moving them across it).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
 
 // Dyson solve of one block per workgroup (1024 threads), in the coefficient form of
 // dyson_it_ppsc::solve (dlr_dyson_ppsc.hpp:135-168):
@@ -57,13 +69,18 @@ constexpr int kDysonMaxNp = 784;  // padded r n: DLR rank <= 64, blocks <= 12
 // maps DLR coefficients to values. The same equation without the inverse of cf2it: its rounding
 // sensitivity is 2e-16 against 6e-16 for the transposed-solve form and 4e-9 for an explicit inverse.
 // The N x N system (N = r n, padded to a multiple of 16 with an identity block) is factorised by a
-// right-looking blocked LU with partial pivoting: 16-column panels, then the trailing update
-// A22 -= L21 U12 on FP64 MFMA tiles. The n right-hand sides (columns of G0) are solved by blocked
-// substitution, G = (cf2it x I) g is formed and hermitised (block_sparse_solver.py:579-587).
+// right-looking blocked LU with partial pivoting: 16-column panels factorised by one wave in LDS,
+// then the trailing update A22 -= L21 U12 on FP64 MFMA tiles. The n right-hand sides (columns of
+// G0) are solved in LDS by blocked substitution, G = (cf2it x I) g is formed and hermitised
+// (block_sparse_solver.py:579-587). Sequential work (column solves with the DLR factors, panel
+// pivoting, diagonal-block substitution) runs on LDS copies; the arithmetic is the same as on the
+// global arrays.
 __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
-  __shared__ double red_v[kDysonThreads / 64];
-  __shared__ int red_i[kDysonThreads / 64];
-  extern __shared__ double Pn[];  // LU panel: (Np - k0) rows x 16 columns
+  __shared__ double s_lu[kMaxRank * kMaxRank];  // LU factors of cf2it
+  __shared__ int s_piv[kMaxRank];
+  __shared__ double s_dg[16 * 16];               // diagonal block of the substitution
+  extern __shared__ double Pn[];  // np_max x 16 doubles: LU panel ((Np - k0) rows x 16 columns), or
+                                  // the r x n^2 columns of a vals2coefs, or the Np x n right-hand sides
   const int pidx = find_seg(Bt.prefix, Bt.nprobs, blockIdx.x);
   const DysonArgs &A = Bt.probs[pidx];
   const int b = blockIdx.x - Bt.prefix[pidx], n = A.dims[b], r = A.r, nn = n * n, N = r * n;
@@ -76,10 +93,15 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
   double *S = A.sys + A.soff[b];
   double *B = A.rhs + A.roff[b];
   int *ip = A.ipiv + A.poff[b];
-  // ---- sigc = vals2coefs(Sigma_b)
-  for (int x = tid; x < r * nn; x += NT) sigc[x] = A.sigma[go + x];
+  for (int x = tid; x < r * r; x += NT) s_lu[x] = A.lu[x];
+  for (int x = tid; x < r; x += NT) s_piv[x] = A.piv[x];
+  // ---- sigc = vals2coefs(Sigma_b), the columns solved in LDS (r n^2 <= r n 16 <= np_max 16)
+  double *X = Pn;
+  for (int x = tid; x < r * nn; x += NT) X[x] = A.sigma[go + x];
   __syncthreads();
-  for (int c = tid; c < nn; c += NT) lu_solve_col(A.lu, A.piv, r, sigc + c, nn);
+  for (int c = tid; c < nn; c += NT) lu_solve_col(s_lu, s_piv, r, X + c, nn);
+  __syncthreads();
+  for (int x = tid; x < r * nn; x += NT) sigc[x] = X[x];
   __syncthreads();
   // ---- W sigc -> wf (scratch)
   for (int x = tid; x < r * nn; x += NT) {
@@ -115,14 +137,17 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
   __syncthreads();
   for (int x = tid; x < r * nn; x += NT) ggs[x] = wf[x];
   __syncthreads();
-  // ---- f = vals2coefs(ggs) in place, W f -> wf
-  for (int c = tid; c < nn; c += NT) lu_solve_col(A.lu, A.piv, r, ggs + c, nn);
+  // ---- f = vals2coefs(ggs) (solved in LDS), W f -> wf
+  for (int x = tid; x < r * nn; x += NT) X[x] = ggs[x];
+  __syncthreads();
+  for (int c = tid; c < nn; c += NT) lu_solve_col(s_lu, s_piv, r, X + c, nn);
   __syncthreads();
   for (int x = tid; x < r * nn; x += NT) {
     const int t = x / nn, c = x % nn;
     double s = 0.0;
-    for (int k = 0; k < r; ++k) s += A.W[t * r + k] * ggs[(size_t)k * nn + c];
+    for (int k = 0; k < r; ++k) s += A.W[t * r + k] * X[(size_t)k * nn + c];
     wf[x] = s;
+    ggs[x] = X[x];
   }
   __syncthreads();
   // ---- system cf2it x I + Mc, padded with identity to Np x Np. The sum over k of Mc is, for each
@@ -162,43 +187,44 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
     const int m = Np - k0;
     for (int x = tid; x < m * 16; x += NT) Pn[x] = S[(size_t)(k0 + (x >> 4)) * Np + k0 + (x & 15)];
     __syncthreads();
-    for (int jj = 0; jj < 16; ++jj) {
-      // pivot: first row of maximal |P[i][jj]|, i >= jj (thread scan, wave shuffles, then the wave
-      // results in a fixed order)
-      double best = -1.0;
-      int bi = Np;
-      for (int i = jj + tid; i < m; i += NT) {
-        const double v = fabs(Pn[i * 16 + jj]);
-        if (v > best) { best = v; bi = i; }
-      }
+    if (wv == 0) {  // the panel on one wave: no workgroup barrier per column
+      for (int jj = 0; jj < 16; ++jj) {
+        // pivot: first row of maximal |P[i][jj]|, i >= jj (lane scan, then wave shuffles)
+        double best = -1.0;
+        int bi = Np;
+        for (int i = jj + lane; i < m; i += 64) {
+          const double v = fabs(Pn[i * 16 + jj]);
+          if (v > best) { best = v; bi = i; }
+        }
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const double v2 = __shfl_xor(best, off, 64);
-        const int i2 = __shfl_xor(bi, off, 64);
-        if (v2 > best || (v2 == best && i2 < bi)) { best = v2; bi = i2; }
+        for (int off = 32; off > 0; off >>= 1) {
+          const double v2 = __shfl_xor(best, off, 64);
+          const int i2 = __shfl_xor(bi, off, 64);
+          if (v2 > best || (v2 == best && i2 < bi)) { best = v2; bi = i2; }
+        }
+        const int p = bi;
+        if (lane == 0) ip[k0 + jj] = k0 + p;
+        if (p != jj && lane < 16) {
+          const double a = Pn[jj * 16 + lane];
+          Pn[jj * 16 + lane] = Pn[p * 16 + lane];
+          Pn[p * 16 + lane] = a;
+        }
+        wave_lds_sync();
+        const double pv = Pn[jj * 16 + jj];
+        double prow[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) prow[c] = Pn[jj * 16 + c];
+        for (int i = jj + 1 + lane; i < m; i += 64) {
+          const double l = Pn[i * 16 + jj] / pv;
+          Pn[i * 16 + jj] = l;
+#pragma unroll
+          for (int c = 0; c < 16; ++c)
+            if (c > jj) Pn[i * 16 + c] -= l * prow[c];
+        }
+        wave_lds_sync();
       }
-      if (lane == 0) { red_v[wv] = best; red_i[wv] = bi; }
-      __syncthreads();
-      best = red_v[0];
-      bi = red_i[0];
-      for (int w = 1; w < NWV; ++w)
-        if (red_v[w] > best || (red_v[w] == best && red_i[w] < bi)) { best = red_v[w]; bi = red_i[w]; }
-      const int p = bi;
-      if (tid == 0) ip[k0 + jj] = k0 + p;
-      if (p != jj && tid < 16) {
-        const double a = Pn[jj * 16 + tid];
-        Pn[jj * 16 + tid] = Pn[p * 16 + tid];
-        Pn[p * 16 + tid] = a;
-      }
-      __syncthreads();
-      const double pv = Pn[jj * 16 + jj];
-      for (int i = jj + 1 + tid; i < m; i += NT) {
-        const double l = Pn[i * 16 + jj] / pv;
-        Pn[i * 16 + jj] = l;
-        for (int c = jj + 1; c < 16; ++c) Pn[i * 16 + c] -= l * Pn[jj * 16 + c];
-      }
-      __syncthreads();
     }
+    __syncthreads();
     for (int x = tid; x < m * 16; x += NT) S[(size_t)(k0 + (x >> 4)) * Np + k0 + (x & 15)] = Pn[x];
     // the panel's row interchanges on every other column (in order, per column), and U12
     const int c1 = k0 + 16;
@@ -249,48 +275,60 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
     }
     __syncthreads();
   }
-  // ---- right-hand sides: B[(t,i)][j] = G0[t][i][j], padding rows zero; row interchanges in order
-  for (int x = tid; x < Np * n; x += NT) B[x] = (x < N * n) ? G0[x] : 0.0;
+  // ---- right-hand sides in LDS (Np n <= np_max 16): Y[(t,i)][j] = G0[t][i][j], padding rows zero;
+  // row interchanges in order
+  double *Y = Pn;
+  for (int x = tid; x < Np * n; x += NT) Y[x] = (x < N * n) ? G0[x] : 0.0;
   __syncthreads();
   for (int col = tid; col < n; col += NT)
     for (int k = 0; k < Np; ++k)
       if (ip[k] != k) {
-        const double a = B[(size_t)k * n + col];
-        B[(size_t)k * n + col] = B[(size_t)ip[k] * n + col];
-        B[(size_t)ip[k] * n + col] = a;
+        const double a = Y[(size_t)k * n + col];
+        Y[(size_t)k * n + col] = Y[(size_t)ip[k] * n + col];
+        Y[(size_t)ip[k] * n + col] = a;
       }
   __syncthreads();
-  // forward substitution with the unit lower factor, 16 rows at a time
+  // forward substitution with the unit lower factor, 16 rows at a time (diagonal block in LDS)
   for (int k0 = 0; k0 < Np; k0 += 16) {
+    for (int x = tid; x < 256; x += NT) s_dg[x] = S[(size_t)(k0 + (x >> 4)) * Np + k0 + (x & 15)];
+    __syncthreads();
     for (int col = tid; col < n; col += NT)
+#pragma unroll 1
       for (int q = 1; q < 16; ++q) {
-        double s = B[(size_t)(k0 + q) * n + col];
-        for (int q0 = 0; q0 < q; ++q0) s -= S[(size_t)(k0 + q) * Np + k0 + q0] * B[(size_t)(k0 + q0) * n + col];
-        B[(size_t)(k0 + q) * n + col] = s;
+        double s = Y[(size_t)(k0 + q) * n + col];
+#pragma unroll
+        for (int q0 = 0; q0 < 16; ++q0)
+          if (q0 < q) s -= s_dg[q * 16 + q0] * Y[(size_t)(k0 + q0) * n + col];
+        Y[(size_t)(k0 + q) * n + col] = s;
       }
     __syncthreads();
     for (int x = tid; x < (Np - k0 - 16) * n; x += NT) {
       const int i = k0 + 16 + x / n, col = x % n;
-      double s = B[(size_t)i * n + col];
-      for (int q = 0; q < 16; ++q) s -= S[(size_t)i * Np + k0 + q] * B[(size_t)(k0 + q) * n + col];
-      B[(size_t)i * n + col] = s;
+      double s = Y[(size_t)i * n + col];
+      for (int q = 0; q < 16; ++q) s -= S[(size_t)i * Np + k0 + q] * Y[(size_t)(k0 + q) * n + col];
+      Y[(size_t)i * n + col] = s;
     }
     __syncthreads();
   }
   // backward substitution with the upper factor
   for (int k0 = Np - 16; k0 >= 0; k0 -= 16) {
+    for (int x = tid; x < 256; x += NT) s_dg[x] = S[(size_t)(k0 + (x >> 4)) * Np + k0 + (x & 15)];
+    __syncthreads();
     for (int col = tid; col < n; col += NT)
+#pragma unroll 1
       for (int q = 15; q >= 0; --q) {
-        double s = B[(size_t)(k0 + q) * n + col];
-        for (int q1 = q + 1; q1 < 16; ++q1) s -= S[(size_t)(k0 + q) * Np + k0 + q1] * B[(size_t)(k0 + q1) * n + col];
-        B[(size_t)(k0 + q) * n + col] = s / S[(size_t)(k0 + q) * Np + k0 + q];
+        double s = Y[(size_t)(k0 + q) * n + col];
+#pragma unroll
+        for (int q1 = 0; q1 < 16; ++q1)
+          if (q1 > q) s -= s_dg[q * 16 + q1] * Y[(size_t)(k0 + q1) * n + col];
+        Y[(size_t)(k0 + q) * n + col] = s / s_dg[q * 16 + q];
       }
     __syncthreads();
     for (int x = tid; x < k0 * n; x += NT) {
       const int i = x / n, col = x % n;
-      double s = B[(size_t)i * n + col];
-      for (int q = 0; q < 16; ++q) s -= S[(size_t)i * Np + k0 + q] * B[(size_t)(k0 + q) * n + col];
-      B[(size_t)i * n + col] = s;
+      double s = Y[(size_t)i * n + col];
+      for (int q = 0; q < 16; ++q) s -= S[(size_t)i * Np + k0 + q] * Y[(size_t)(k0 + q) * n + col];
+      Y[(size_t)i * n + col] = s;
     }
     __syncthreads();
   }
@@ -299,7 +337,7 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
   for (int x = tid; x < r * nn; x += NT) {
     const int t = x / nn, ij = x % nn;
     double s = 0.0;
-    for (int k = 0; k < r; ++k) s += A.cf2it[t * r + k] * B[(size_t)k * nn + ij];
+    for (int k = 0; k < r; ++k) s += A.cf2it[t * r + k] * Y[(size_t)k * nn + ij];
     Gout[x] = s;
   }
   __syncthreads();
@@ -318,18 +356,28 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
 // entries (LU solve, summed in a fixed order by one thread), deta = ln|Z| / beta, eta += deta; the
 // difference accumulator is cleared for the update kernel that follows.
 __global__ void __launch_bounds__(256) trace_kernel(IterBatch Bt) {
+  constexpr int kChunk = 48;  // diagonal entries solved at once, one thread each, in LDS (< 64 KB)
+  __shared__ double s_lu[kMaxRank * kMaxRank];
+  __shared__ int s_piv[kMaxRank];
+  __shared__ double xs[kChunk * kMaxRank];
   const IterArgs &A = Bt.probs[blockIdx.x];
   const int tid = threadIdx.x, r = A.r;
-  for (int d = tid; d < A.ndiag; d += blockDim.x) {
-    int b = 0, i = d;
-    while (i >= A.dims[b]) { i -= A.dims[b]; ++b; }
-    const int n = A.dims[b];
-    double *x = A.trs + (size_t)d * r;
-    for (int t = 0; t < r; ++t) x[t] = A.Gnew[A.goff[b] + (size_t)t * n * n + i * n + i];
-    lu_solve_col(A.lu, A.piv, r, x, 1);
-    double acc = 0.0;
-    for (int k = 0; k < r; ++k) acc += A.kbeta[k] * x[k];
-    x[0] = acc;
+  for (int x = tid; x < r * r; x += blockDim.x) s_lu[x] = A.lu[x];
+  for (int x = tid; x < r; x += blockDim.x) s_piv[x] = A.piv[x];
+  __syncthreads();
+  for (int d0 = 0; d0 < A.ndiag; d0 += kChunk) {
+    const int d = d0 + tid;
+    if (tid < kChunk && d < A.ndiag) {
+      int b = 0, i = d;
+      while (i >= A.dims[b]) { i -= A.dims[b]; ++b; }
+      const int n = A.dims[b];
+      double *x = xs + tid * kMaxRank;
+      for (int t = 0; t < r; ++t) x[t] = A.Gnew[A.goff[b] + (size_t)t * n * n + i * n + i];
+      lu_solve_col(s_lu, s_piv, r, x, 1);
+      double acc = 0.0;
+      for (int k = 0; k < r; ++k) acc += A.kbeta[k] * x[k];
+      A.trs[(size_t)d * r] = acc;
+    }
   }
   __syncthreads();
   if (tid == 0) {
