@@ -3,6 +3,7 @@
 // BlockSparseSolver.solve, python/triqs_xca/block_sparse_solver.py:295-346,427-449,572-590).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "dyson.hpp"
@@ -51,6 +52,7 @@ __device__ __forceinline__ v4d mfma16x16x4(double a, double b, v4d c) { return _
 constexpr int kDysonThreads = 1024;
 constexpr int kDysonMaxNp = 784;  // padded r n: DLR rank <= 64, blocks <= 12
 constexpr int kMaxRank = 64;      // DLR rank limit of the device path (checked at context creation)
+constexpr int kLdsSysMax = 112;   // launches whose largest padded system fits keep the systems in LDS
 
 // Orders the LDS accesses of one wave around this point (a wave executes its LDS operations in
 // issue order; this keeps the compiler from moving them across it).
@@ -79,8 +81,12 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
   __shared__ double s_lu[kMaxRank * kMaxRank];  // LU factors of cf2it
   __shared__ int s_piv[kMaxRank];
   __shared__ double s_dg[16 * 16];               // diagonal block of the substitution
-  extern __shared__ double Pn[];  // np_max x 16 doubles: LU panel ((Np - k0) rows x 16 columns), or
-                                  // the r x n^2 columns of a vals2coefs, or the Np x n right-hand sides
+  // dynamic LDS: [the system (np_max^2 doubles), when np_max <= kLdsSysMax] then np_max x 16 doubles
+  // for the LU panel ((Np - k0) rows x 16 columns), the r x n^2 columns of a vals2coefs, or the
+  // Np x n right-hand sides
+  extern __shared__ double lds_dyn[];
+  const bool sys_lds = Bt.np_max <= kLdsSysMax;
+  double *Pn = lds_dyn + (sys_lds ? (size_t)Bt.np_max * Bt.np_max : 0);
   const int pidx = find_seg(Bt.prefix, Bt.nprobs, blockIdx.x);
   const DysonArgs &A = Bt.probs[pidx];
   const int b = blockIdx.x - Bt.prefix[pidx], n = A.dims[b], r = A.r, nn = n * n, N = r * n;
@@ -90,7 +96,7 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
   const size_t go = A.goff[b];
   const double *G0 = A.G0 + go, *g0c = A.g0c + go, *Wg0c = A.Wg0c + go;
   double *sigc = A.sigc + go, *ggs = A.ggs + go, *wf = A.wf + go;
-  double *S = A.sys + A.soff[b];
+  double *S = sys_lds ? lds_dyn : A.sys + A.soff[b];  // Np x Np system, then its LU factors
   double *B = A.rhs + A.roff[b];
   int *ip = A.ipiv + A.poff[b];
   for (int x = tid; x < r * r; x += NT) s_lu[x] = A.lu[x];
@@ -445,14 +451,17 @@ __global__ void fill_kernel(int64_t n, double v, double *p) {
 hipError_t launch_dyson(const DysonBatch &B, hipStream_t s) {
   static bool ready = false;  // attribute set once, outside any stream capture after the first launch
   if (!ready) {
+    const size_t most = std::max<size_t>((size_t)kDysonMaxNp * 16, (size_t)kLdsSysMax * (kLdsSysMax + 16));
     hipError_t e = hipFuncSetAttribute((const void *)dyson_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)(sizeof(double) * kDysonMaxNp * 16));
+                                       (int)(sizeof(double) * most));
     if (e != hipSuccess) return e;
     ready = true;
   }
   if (B.total <= 0) return hipSuccess;
   if (B.np_max > kDysonMaxNp) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(dyson_kernel, dim3(B.total), dim3(kDysonThreads), sizeof(double) * B.np_max * 16, s, B);
+  const size_t np = (size_t)B.np_max;
+  const size_t sh = sizeof(double) * ((np <= (size_t)kLdsSysMax ? np * np : 0) + np * 16);
+  hipLaunchKernelGGL(dyson_kernel, dim3(B.total), dim3(kDysonThreads), sh, s, B);
   return hipGetLastError();
 }
 
