@@ -471,9 +471,9 @@ ChainArgs chain_args(xca_ctx *c, DevPlan *dp, DevVariant *V) {
   A.D = c->D;
   A.ops = dp->ops.ptr; A.progs = V->progs.ptr; A.chunks = V->chunks.ptr;
   A.plist = c->plist.ptr; A.G = c->G.ptr; A.opmats = c->opmats.ptr; A.ftab = dp->ftab.ptr; A.kp = c->kp.ptr; A.km = c->km.ptr;
-  A.delta = c->delta.ptr; A.delta_refl = c->delta_refl.ptr; A.kpow = c->kpow.ptr; A.partial = V->partial.ptr;
+  A.kpow = c->kpow.ptr; A.partial = V->partial.ptr;
   A.nchunks = V->nchunks;
-  A.r = c->M.P.r; A.p = c->M.P.p; A.n = c->M.P.n; A.maxexp = c->maxexp;
+  A.r = c->M.P.r; A.p = c->M.P.p; A.maxexp = c->maxexp;
   return A;
 }
 

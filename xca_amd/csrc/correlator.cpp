@@ -412,9 +412,9 @@ static CorrDevPlan *get_corr_plan(xca_ctx *c, int m, const int *topo, const int3
     A.D = c->D;
     A.ops = dp->ops.ptr; A.progs = B->progs.ptr; A.chunks = B->chunks.ptr;
     A.plist = c->plist.ptr; A.G = c->G.ptr; A.opmats = c->opmats.ptr; A.ftab = dp->ftab.ptr; A.kp = c->kp.ptr; A.km = c->km.ptr;
-    A.delta = c->delta.ptr; A.delta_refl = c->delta_refl.ptr; A.kpow = c->kpow.ptr; A.partial = nullptr; A.store = dp->store.ptr;
+    A.kpow = c->kpow.ptr; A.partial = nullptr; A.store = dp->store.ptr;
     A.nchunks = B->nchunks;
-    A.r = r; A.p = p; A.n = n; A.maxexp = c->maxexp;
+    A.r = r; A.p = p; A.maxexp = c->maxexp;
     B->seg.upload(std::vector<ChainArgs>(1, A), s);
     B->seg_prefix.upload(std::vector<int>{0, B->nchunks}, s);
   }

@@ -566,10 +566,14 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
         // groups and 4 columns, whose reads all happen before any of them is overwritten.
         {
           const int ngroups = r * nit, gpr = NT / b;
+          // groups past the last full round are split into single elements when they fit one round
+          const int full = ngroups <= gpr ? ngroups : (ngroups / gpr) * gpr, tail = ngroups - full;
+          const bool split_tail = tail > 0 && tail * b * cols <= NT;
+          const int gend = split_tail ? full : ngroups;
           const int gl = tid / b, i = tid % b;
-          for (int g0 = 0; g0 < ngroups; g0 += gpr) {
+          for (int g0 = 0; g0 < gend; g0 += gpr) {
             const int gi = g0 + gl;
-            const bool act = gl < gpr && gi < ngroups;
+            const bool act = gl < gpr && gi < gend;
             const int t = act ? gi / nit : 0, it = act ? gi % nit : 0;
             double fr[kMaxB], wfr[kMaxB];
             if (act) {
@@ -616,6 +620,36 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
               __syncthreads();
             }
           }
+          if (split_tail) {  // one element (group, row, column) per thread
+            const int e = tid, bc = b * cols;
+            const bool act = e < tail * bc;
+            const int gi = full + (act ? e / bc : 0), ie = act ? (e % bc) / cols : 0, je = act ? e % cols : 0;
+            const int t = gi / nit, it = gi % nit;
+            double s1 = 0.0, s2 = 0.0;
+            if (act) {
+              int64_t en = op.ebase;
+#pragma unroll
+              for (int q = 0; q < kMaxLines; ++q)
+                if (q < op.nlines) en += (int64_t)ipole[it * kMaxLines + op.lines[q]] * op.estride[q];
+              const double *f = A.ftab + en + (size_t)t * b * b + ie * b;
+              const double *wf = f + (size_t)r * b * b;
+              const double *gcol = Soth + (size_t)t * LDN + it * w, *wgcol = Scur + (size_t)t * LDN + it * w;
+#pragma unroll
+              for (int k = 0; k < kMaxB; ++k)
+                if (k < b) {
+                  const double fk = f[k], wfk = wf[k];
+                  const double g = gcol[k * cols + je], wg = wgcol[k * cols + je];
+                  s1 += wfk * g + fk * wg;
+                  s2 += fk * g;
+                }
+            }
+            __syncthreads();
+            if (act) {
+              Soth[(size_t)t * LDN + it * w + ie * cols + je] = -s1;
+              Scur[(size_t)t * LDN + it * w + ie * cols + je] = s2;
+            }
+            __syncthreads();
+          }
         }
         // h = beta (cf2it c + tcf2it d) -> Soth, stored after every wave has read both slabs
         {
@@ -635,7 +669,8 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
         // sums Fh_mu(t) = sum_kk h(t)[mu][kk] F_kk tabulated per (set, direction, block) at fbase,
         // (r, qs, b1, b0); one column of Fh_mu(t) in registers, the mu operators read with
         // wave-uniform addresses
-        for (int x = tid; x < r * nit * b0; x += NT) {
+        const int R = r * nit * b0, whole = R <= NT ? R : (R / NT) * NT;
+        for (int x = tid; x < whole; x += NT) {
           const int t = x / (nit * b0), rem = x % (nit * b0), it = rem / b0, j = rem % b0;
           const double *Tt = Scur + (size_t)t * LDN + it * win;
           const double *Fht = A.opmats + op.fbase + (size_t)t * qs * b1 * b0 + j;
@@ -660,6 +695,28 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
 #pragma unroll
           for (int i = 0; i < kMaxB; ++i)
             if (i < bvp) Soth[(size_t)t * LDN + it * wout + i * b0 + j] = out[i];
+        }
+        // the columns past the last full round: one output element (t, item, i, j) per thread
+        for (int e = tid; e < (R - whole) * bvp; e += NT) {
+          const int x = whole + e / bvp, i = e % bvp;
+          const int t = x / (nit * b0), rem = x % (nit * b0), it = rem / b0, j = rem % b0;
+          const double *Tt = Scur + (size_t)t * LDN + it * win;
+          const double *Fht = A.opmats + op.fbase + (size_t)t * qs * b1 * b0 + j;
+          double o = 0.0;
+          for (int mu = 0; mu < qs; ++mu) {
+            double fh[kMaxB];
+#pragma unroll
+            for (int c = 0; c < kMaxB; ++c) fh[c] = (c < b1) ? Fht[(size_t)(mu * b1 + c) * b0] : 0.0;
+            const double *Fmi = A.opmats + op.mubase + (int64_t)mu * bvp * bv + i * bv;
+            for (int a = 0; a < bv; ++a) {
+              double y = 0.0;
+#pragma unroll
+              for (int c = 0; c < kMaxB; ++c)
+                if (c < b1) y += Tt[a * b1 + c] * fh[c];
+              o += Fmi[a] * y;
+            }
+          }
+          Soth[(size_t)t * LDN + it * wout + i * b0 + j] = o;
         }
         rows = bvp;
         cols = b0;

@@ -10,7 +10,7 @@ constexpr int kMaxSet = 8;     // largest symmetry set handled by the zero-verte
 // One vertex/edge/zero-vertex operation (see host/model.hpp for the semantics).
 struct DOp {
   int type, bin, bout, cols;
-  int64_t fbase, fstride_l;      // V: operator matrix, bar stride per pole; Z: kappa operators
+  int64_t fbase, fstride_l;      // V: operator matrix, bar stride per pole; Z: kappa sums Fh (r, qs, b1, b0)
   int bar_line, kx, ks, nlines;  // V: bar line, K line and sign; E: number of K lines
   int lines[kMaxLines];
   int pad0;
@@ -73,11 +73,11 @@ struct ChainArgs {
   const DProg *progs;
   const DChunk *chunks;
   const int *plist;            // pole lists
-  const double *G, *opmats, *ftab, *kp, *km, *delta, *delta_refl, *kpow;
+  const double *G, *opmats, *ftab, *kp, *km, *kpow;
   double *partial;
   double *store;               // per-item outputs of store-mode programs
   int nchunks;
-  int r, p, n, maxexp;
+  int r, p, maxexp;
 };
 
 // One chain launch drains the chunks of several segments (plans, possibly of different problems
