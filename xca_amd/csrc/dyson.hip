@@ -21,28 +21,44 @@ __device__ __forceinline__ static int find_seg(const int *prefix, int nsegs, int
   return lo;
 }
 
-// in-place LU solve of one column (stride between nodes) with the DLR interpolation factors; the
-// kernels call it on LDS copies of the column and the factors
-__device__ __forceinline__ void lu_solve_col(const double *__restrict__ lu, const int *__restrict__ piv, int r, double *x, int stride) {
-  for (int k = 0; k < r; ++k)
-    if (piv[k] != k) {
-      double a = x[(size_t)k * stride], b = x[(size_t)piv[k] * stride];
-      x[(size_t)k * stride] = b;
-      x[(size_t)piv[k] * stride] = a;
+// value of lane k (k wave-uniform)
+__device__ __forceinline__ double lane_value(double v, int k) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), k), hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
+  return __hiloint2double(hi, lo);
+}
+
+// LU solve of one column by one wave, lane j holding x[j] (j < r <= 64): the row interchanges in
+// order, then column-oriented substitutions (the forward pass subtracts its terms in the same order
+// as the row-oriented one; the backward pass subtracts them from the last row up). The factors and
+// pivots are read from LDS.
+__device__ __forceinline__ double lu_solve_wave(const double *__restrict__ lu, const int *__restrict__ piv, int r, double v, int lane) {
+  for (int k = 0; k < r; ++k) {
+    const int p = __builtin_amdgcn_readfirstlane(piv[k]);
+    if (p != k) {
+      const double a = lane_value(v, k), b = lane_value(v, p);
+      if (lane == k) v = b;
+      else if (lane == p) v = a;
     }
-  // the products are accumulated in index order; unrolling lets the loads of 8 terms issue together
-  for (int i = 1; i < r; ++i) {
-    double s = x[(size_t)i * stride];
-#pragma unroll 8
-    for (int k = 0; k < i; ++k) s -= lu[i * r + k] * x[(size_t)k * stride];
-    x[(size_t)i * stride] = s;
   }
-  for (int i = r - 1; i >= 0; --i) {
-    double s = x[(size_t)i * stride];
-#pragma unroll 8
-    for (int k = i + 1; k < r; ++k) s -= lu[i * r + k] * x[(size_t)k * stride];
-    x[(size_t)i * stride] = s / lu[i * r + i];
+  // branch-free steps: every lane loads its factor entry (inside the kMaxRank^2 array for every
+  // lane) so that the loads of later steps can issue early; lanes outside the update keep v
+  const int row = lane < r ? lane : 0;
+#pragma unroll 4
+  for (int k = 0; k + 1 < r; ++k) {
+    const double l = lu[row * r + k];
+    const double xk = lane_value(v, k);
+    const double u = v - l * xk;
+    v = (lane > k && lane < r) ? u : v;
   }
+#pragma unroll 4
+  for (int k = r - 1; k >= 0; --k) {
+    const double l = lu[row * r + k], dk = lu[k * r + k];
+    if (lane == k) v = v / dk;
+    const double xk = lane_value(v, k);
+    const double u = v - l * xk;
+    v = lane < k ? u : v;
+  }
+  return v;
 }
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -95,7 +111,7 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
   const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wv = tid >> 6, NWV = NT >> 6;
   const size_t go = A.goff[b];
   const double *G0 = A.G0 + go, *g0c = A.g0c + go, *Wg0c = A.Wg0c + go;
-  double *sigc = A.sigc + go, *ggs = A.ggs + go, *wf = A.wf + go;
+  double *ggs = A.ggs + go, *wf = A.wf + go;
   double *S = sys_lds ? lds_dyn : A.sys + A.soff[b];  // Np x Np system, then its LU factors
   double *B = A.rhs + A.roff[b];
   int *ip = A.ipiv + A.poff[b];
@@ -105,22 +121,24 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
   double *X = Pn;
   for (int x = tid; x < r * nn; x += NT) X[x] = A.sigma[go + x];
   __syncthreads();
-  for (int c = tid; c < nn; c += NT) lu_solve_col(s_lu, s_piv, r, X + c, nn);
-  __syncthreads();
-  for (int x = tid; x < r * nn; x += NT) sigc[x] = X[x];
+  for (int c = wv; c < nn; c += NWV) {  // one wave per column
+    const double v = lu_solve_wave(s_lu, s_piv, r, lane < r ? X[(size_t)lane * nn + c] : 0.0, lane);
+    if (lane < r) X[(size_t)lane * nn + c] = v;
+  }
   __syncthreads();
   // ---- W sigc -> wf (scratch)
   for (int x = tid; x < r * nn; x += NT) {
     const int t = x / nn, c = x % nn;
     double s = 0.0;
-    for (int k = 0; k < r; ++k) s += A.W[t * r + k] * sigc[(size_t)k * nn + c];
+#pragma unroll 8
+    for (int k = 0; k < r; ++k) s += A.W[t * r + k] * X[(size_t)k * nn + c];  // X holds sigc
     wf[x] = s;
   }
   __syncthreads();
   // ---- node products: c = -(Wg0c sigc + g0c Wsig) -> ggs (temporarily), d = g0c sigc -> B (scratch)
   for (int x = tid; x < r * nn; x += NT) {
     const int t = x / nn, i = (x % nn) / n, j = x % n;
-    const double *f = g0c + (size_t)t * nn, *wfk = Wg0c + (size_t)t * nn, *g = sigc + (size_t)t * nn, *wg = wf + (size_t)t * nn;
+    const double *f = g0c + (size_t)t * nn, *wfk = Wg0c + (size_t)t * nn, *g = X + (size_t)t * nn, *wg = wf + (size_t)t * nn;
     double s1 = 0.0, s2 = 0.0;
     for (int l = 0; l < n; ++l) {
       s1 += wfk[i * n + l] * g[l * n + j] + f[i * n + l] * wg[l * n + j];
@@ -134,6 +152,7 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
   for (int x = tid; x < r * nn; x += NT) {
     const int t = x / nn, c = x % nn;
     double h1 = 0.0, h2 = 0.0;
+#pragma unroll 8
     for (int k = 0; k < r; ++k) {
       h1 += A.cf2it[t * r + k] * ggs[(size_t)k * nn + c];
       h2 += A.tcf2it[t * r + k] * B[(size_t)k * nn + c];
@@ -146,11 +165,15 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
   // ---- f = vals2coefs(ggs) (solved in LDS), W f -> wf
   for (int x = tid; x < r * nn; x += NT) X[x] = ggs[x];
   __syncthreads();
-  for (int c = tid; c < nn; c += NT) lu_solve_col(s_lu, s_piv, r, X + c, nn);
+  for (int c = wv; c < nn; c += NWV) {
+    const double v = lu_solve_wave(s_lu, s_piv, r, lane < r ? X[(size_t)lane * nn + c] : 0.0, lane);
+    if (lane < r) X[(size_t)lane * nn + c] = v;
+  }
   __syncthreads();
   for (int x = tid; x < r * nn; x += NT) {
     const int t = x / nn, c = x % nn;
     double s = 0.0;
+#pragma unroll 8
     for (int k = 0; k < r; ++k) s += A.W[t * r + k] * X[(size_t)k * nn + c];
     wf[x] = s;
     ggs[x] = X[x];
@@ -169,6 +192,7 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
       const int t0 = 16 * (tile / RBd), kp0 = 16 * (tile % RBd);
       const int ta = t0 + (lane & 15), kpb = kp0 + (lane & 15);
       v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
       for (int ks = 0; ks < nks; ++ks) {
         const int k = 4 * ks + (lane >> 4);
         const double a = (ta < r && k < r) ? A.cf2it[ta * r + k] * ggs[(size_t)k * nn + c] : 0.0;
@@ -343,6 +367,7 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
   for (int x = tid; x < r * nn; x += NT) {
     const int t = x / nn, ij = x % nn;
     double s = 0.0;
+#pragma unroll 8
     for (int k = 0; k < r; ++k) s += A.cf2it[t * r + k] * Y[(size_t)k * nn + ij];
     Gout[x] = s;
   }
@@ -359,31 +384,26 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
 }
 
 // Per problem (one workgroup each): Z = -Tr G_new(beta) from the DLR coefficients of the diagonal
-// entries (LU solve, summed in a fixed order by one thread), deta = ln|Z| / beta, eta += deta; the
-// difference accumulator is cleared for the update kernel that follows.
-__global__ void __launch_bounds__(256) trace_kernel(IterBatch Bt) {
-  constexpr int kChunk = 48;  // diagonal entries solved at once, one thread each, in LDS (< 64 KB)
+// entries (one wave per entry for the LU solve; the entries summed in a fixed order by one thread),
+// deta = ln|Z| / beta, eta += deta; the difference accumulator is cleared for the update kernel that
+// follows.
+__global__ void __launch_bounds__(1024) trace_kernel(IterBatch Bt) {
   __shared__ double s_lu[kMaxRank * kMaxRank];
   __shared__ int s_piv[kMaxRank];
-  __shared__ double xs[kChunk * kMaxRank];
   const IterArgs &A = Bt.probs[blockIdx.x];
-  const int tid = threadIdx.x, r = A.r;
+  const int tid = threadIdx.x, r = A.r, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
   for (int x = tid; x < r * r; x += blockDim.x) s_lu[x] = A.lu[x];
   for (int x = tid; x < r; x += blockDim.x) s_piv[x] = A.piv[x];
   __syncthreads();
-  for (int d0 = 0; d0 < A.ndiag; d0 += kChunk) {
-    const int d = d0 + tid;
-    if (tid < kChunk && d < A.ndiag) {
-      int b = 0, i = d;
-      while (i >= A.dims[b]) { i -= A.dims[b]; ++b; }
-      const int n = A.dims[b];
-      double *x = xs + tid * kMaxRank;
-      for (int t = 0; t < r; ++t) x[t] = A.Gnew[A.goff[b] + (size_t)t * n * n + i * n + i];
-      lu_solve_col(s_lu, s_piv, r, x, 1);
-      double acc = 0.0;
-      for (int k = 0; k < r; ++k) acc += A.kbeta[k] * x[k];
-      A.trs[(size_t)d * r] = acc;
-    }
+  for (int d = wv; d < A.ndiag; d += nwv) {  // one wave per diagonal entry
+    int b = 0, i = d;
+    while (i >= A.dims[b]) { i -= A.dims[b]; ++b; }
+    const int n = A.dims[b];
+    const double x = lu_solve_wave(s_lu, s_piv, r, lane < r ? A.Gnew[A.goff[b] + (size_t)lane * n * n + i * n + i] : 0.0, lane);
+    const double kb = lane < r ? A.kbeta[lane] : 0.0;
+    double acc = 0.0;
+    for (int k = 0; k < r; ++k) acc = fma(lane_value(kb, k), lane_value(x, k), acc);
+    if (lane == 0) A.trs[(size_t)d * r] = acc;
   }
   __syncthreads();
   if (tid == 0) {
@@ -467,7 +487,7 @@ hipError_t launch_dyson(const DysonBatch &B, hipStream_t s) {
 
 hipError_t launch_iterate(const IterBatch &B, hipStream_t s) {
   if (B.nprobs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(trace_kernel, dim3(B.nprobs), dim3(256), 0, s, B);
+  hipLaunchKernelGGL(trace_kernel, dim3(B.nprobs), dim3(1024), 0, s, B);
   const dim3 grid((unsigned)((B.max_total + 255) / 256), B.nprobs);
   hipLaunchKernelGGL(scale_diff_kernel, grid, dim3(256), 0, s, B);
   hipLaunchKernelGGL(mix_kernel, grid, dim3(256), 0, s, B);

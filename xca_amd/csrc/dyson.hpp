@@ -22,7 +22,7 @@ struct DysonArgs {
   const double *cf2it, *tcf2it, *W;           // (r, r)
   const double *G0, *g0c, *Wg0c;              // packed blocks
   const double *sigma;                        // packed blocks
-  double *sigc, *ggs, *wf, *sys, *rhs, *gout; // scratch / output
+  double *ggs, *wf, *sys, *rhs, *gout;       // scratch / output
   int *ipiv;                                  // pivots
 };
 

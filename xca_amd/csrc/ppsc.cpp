@@ -32,7 +32,7 @@ DysonArgs dyson_args(xca_ctx *c, const double *eta_dev, const double *sigma_dev)
   A.dims = d->dims.ptr; A.goff = d->goff.ptr; A.soff = d->soff.ptr; A.poff = d->poff.ptr; A.roff = d->roff.ptr; A.lu = d->lu.ptr; A.piv = d->piv.ptr;
   A.cf2it = d->cf2it.ptr; A.tcf2it = d->tcf2it.ptr; A.W = d->W.ptr;
   A.G0 = d->G0.ptr; A.g0c = d->g0c.ptr; A.Wg0c = d->Wg0c.ptr; A.sigma = sigma_dev;
-  A.sigc = d->sigc.ptr; A.ggs = d->ggs.ptr; A.wf = d->wf.ptr; A.sys = d->sys.ptr; A.rhs = d->rhs.ptr; A.gout = d->gnew.ptr; A.ipiv = d->ipiv.ptr;
+  A.ggs = d->ggs.ptr; A.wf = d->wf.ptr; A.sys = d->sys.ptr; A.rhs = d->rhs.ptr; A.gout = d->gnew.ptr; A.ipiv = d->ipiv.ptr;
   return A;
 }
 
@@ -62,7 +62,7 @@ int64_t dyson_bytes(const xca_ctx *c) {
     piv += Np;
     ndiag += P.dims[b];
   }
-  return 8 * (7 * c->M.G_size + sys + rhs + P.r * ndiag + 6 * (int64_t)P.r * P.r) + 4 * piv;
+  return 8 * (6 * c->M.G_size + sys + rhs + P.r * ndiag + 6 * (int64_t)P.r * P.r) + 4 * piv;
 }
 
 void require_dyson(xca_ctx *c) {
@@ -124,7 +124,6 @@ extern "C" int xca_dyson_init(xca_ctx *c, const double *G0) {
     d->G0.upload(d->G0_h, s);
     d->g0c.upload(g0c, s);
     d->Wg0c.upload(Wg0c, s);
-    d->sigc.alloc(c->M.G_size);
     d->ggs.alloc(c->M.G_size);
     d->wf.alloc(c->M.G_size);
     d->gnew.alloc(c->M.G_size);
