@@ -322,15 +322,16 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
   for (int k0 = 0; k0 < Np; k0 += 16) {
     for (int x = tid; x < 256; x += NT) s_dg[x] = S[(size_t)(k0 + (x >> 4)) * Np + k0 + (x & 15)];
     __syncthreads();
-    for (int col = tid; col < n; col += NT)
-#pragma unroll 1
-      for (int q = 1; q < 16; ++q) {
-        double s = Y[(size_t)(k0 + q) * n + col];
+    for (int col = wv; col < n; col += NWV) {  // one wave per column, lane q holding row k0 + q
+      double v = lane < 16 ? Y[(size_t)(k0 + lane) * n + col] : 0.0;
 #pragma unroll
-        for (int q0 = 0; q0 < 16; ++q0)
-          if (q0 < q) s -= s_dg[q * 16 + q0] * Y[(size_t)(k0 + q0) * n + col];
-        Y[(size_t)(k0 + q) * n + col] = s;
+      for (int q0 = 0; q0 < 15; ++q0) {
+        const double l = s_dg[(lane & 15) * 16 + q0];
+        const double u = v - l * lane_value(v, q0);
+        v = (lane > q0 && lane < 16) ? u : v;
       }
+      if (lane < 16) Y[(size_t)(k0 + lane) * n + col] = v;
+    }
     __syncthreads();
     for (int x = tid; x < (Np - k0 - 16) * n; x += NT) {
       const int i = k0 + 16 + x / n, col = x % n;
@@ -344,15 +345,17 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
   for (int k0 = Np - 16; k0 >= 0; k0 -= 16) {
     for (int x = tid; x < 256; x += NT) s_dg[x] = S[(size_t)(k0 + (x >> 4)) * Np + k0 + (x & 15)];
     __syncthreads();
-    for (int col = tid; col < n; col += NT)
-#pragma unroll 1
-      for (int q = 15; q >= 0; --q) {
-        double s = Y[(size_t)(k0 + q) * n + col];
+    for (int col = wv; col < n; col += NWV) {  // one wave per column, from the last row up
+      double v = lane < 16 ? Y[(size_t)(k0 + lane) * n + col] : 0.0;
 #pragma unroll
-        for (int q1 = 0; q1 < 16; ++q1)
-          if (q1 > q) s -= s_dg[q * 16 + q1] * Y[(size_t)(k0 + q1) * n + col];
-        Y[(size_t)(k0 + q) * n + col] = s / s_dg[q * 16 + q];
+      for (int q = 15; q >= 0; --q) {
+        const double dq = s_dg[q * 16 + q], l = s_dg[(lane & 15) * 16 + q];
+        if (lane == q) v = v / dq;
+        const double u = v - l * lane_value(v, q);
+        v = lane < q ? u : v;
       }
+      if (lane < 16) Y[(size_t)(k0 + lane) * n + col] = v;
+    }
     __syncthreads();
     for (int x = tid; x < k0 * n; x += NT) {
       const int i = x / n, col = x % n;
