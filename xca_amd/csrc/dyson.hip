@@ -27,6 +27,32 @@ __device__ __forceinline__ double lane_value(double v, int k) {
   return __hiloint2double(hi, lo);
 }
 
+// One step of the wave argmax (largest value, then smallest index) over the DPP pattern CTRL,
+// restricted to the rows in ROW_MASK; lanes without a source combine with (-1, idx0), which never wins.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void dpp_argmax(double &v, int &i, int idx0) {
+  const double none = -1.0;
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(none), __double2loint(v), CTRL, ROW_MASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(none), __double2hiint(v), CTRL, ROW_MASK, 0xF, false);
+  const int i2 = __builtin_amdgcn_update_dpp(idx0, i, CTRL, ROW_MASK, 0xF, false);
+  const double v2 = __hiloint2double(hi, lo);
+  if (v2 > v || (v2 == v && i2 < i)) { v = v2; i = i2; }
+}
+
+// (largest value, smallest index among equal values) of the wave, in every lane: a reduction to
+// lane 63 by row shifts 1, 2, 4, 8 and the row broadcasts of lanes 15 and 31 (no LDS traffic), then
+// read from lane 63.
+__device__ __forceinline__ void wave_argmax(double &v, int &i, int idx0) {
+  dpp_argmax<0x111, 0xF>(v, i, idx0);  // row_shr:1
+  dpp_argmax<0x112, 0xF>(v, i, idx0);  // row_shr:2
+  dpp_argmax<0x114, 0xF>(v, i, idx0);  // row_shr:4
+  dpp_argmax<0x118, 0xF>(v, i, idx0);  // row_shr:8
+  dpp_argmax<0x142, 0xA>(v, i, idx0);  // row_bcast:15 into rows 1 and 3
+  dpp_argmax<0x143, 0xC>(v, i, idx0);  // row_bcast:31 into rows 2 and 3
+  v = lane_value(v, 63);
+  i = __builtin_amdgcn_readlane(i, 63);
+}
+
 // LU solve of one column by one wave, lane j holding x[j] (j < r <= 64): the row interchanges in
 // order, then column-oriented substitutions (the forward pass subtracts its terms in the same order
 // as the row-oriented one; the backward pass subtracts them from the last row up). The factors and
@@ -219,19 +245,14 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
     __syncthreads();
     if (wv == 0) {  // the panel on one wave: no workgroup barrier per column
       for (int jj = 0; jj < 16; ++jj) {
-        // pivot: first row of maximal |P[i][jj]|, i >= jj (lane scan, then wave shuffles)
+        // pivot: first row of maximal |P[i][jj]|, i >= jj (lane scan, then a DPP reduction)
         double best = -1.0;
         int bi = Np;
         for (int i = jj + lane; i < m; i += 64) {
           const double v = fabs(Pn[i * 16 + jj]);
           if (v > best) { best = v; bi = i; }
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          const double v2 = __shfl_xor(best, off, 64);
-          const int i2 = __shfl_xor(bi, off, 64);
-          if (v2 > best || (v2 == best && i2 < bi)) { best = v2; bi = i2; }
-        }
+        wave_argmax(best, bi, Np);
         const int p = bi;
         if (lane == 0) ip[k0 + jj] = k0 + p;
         if (p != jj && lane < 16) {
