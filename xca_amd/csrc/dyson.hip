@@ -204,6 +204,10 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
     wf[x] = s;
     ggs[x] = X[x];
   }
+  // the LU factors are done with: cf2it takes their LDS copy for the system (X keeps f)
+  double *s_cf = s_lu;
+  __syncthreads();
+  for (int x = tid; x < r * r; x += NT) s_cf[x] = A.cf2it[x];
   __syncthreads();
   // ---- system cf2it x I + Mc, padded with identity to Np x Np. The sum over k of Mc is, for each
   // orbital pair (i, i'), the product (cf2it diag(f_{i i'})) W: 16 x 16 MFMA tiles over (t, k').
@@ -221,7 +225,7 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
 #pragma unroll 4
       for (int ks = 0; ks < nks; ++ks) {
         const int k = 4 * ks + (lane >> 4);
-        const double a = (ta < r && k < r) ? A.cf2it[ta * r + k] * ggs[(size_t)k * nn + c] : 0.0;
+        const double a = (ta < r && k < r) ? s_cf[ta * r + k] * X[(size_t)k * nn + c] : 0.0;
         const double bb = (k < r && kpb < r) ? A.W[k * r + kpb] : 0.0;
         acc = mfma16x16x4(a, bb, acc);
       }
@@ -229,8 +233,8 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
       for (int v = 0; v < 4; ++v) {
         const int t = t0 + 4 * v + (lane >> 4), kp = kpb;
         if (t < r && kp < r) {
-          const double sv = -A.cf2it[t * r + kp] * wf[(size_t)kp * nn + c] + A.tcf2it[t * r + kp] * ggs[(size_t)kp * nn + c] - acc[v];
-          S[(size_t)(t * n + i) * Np + kp * n + ipp] = A.beta * sv + (i == ipp ? A.cf2it[t * r + kp] : 0.0);
+          const double sv = -s_cf[t * r + kp] * wf[(size_t)kp * nn + c] + A.tcf2it[t * r + kp] * X[(size_t)kp * nn + c] - acc[v];
+          S[(size_t)(t * n + i) * Np + kp * n + ipp] = A.beta * sv + (i == ipp ? s_cf[t * r + kp] : 0.0);
         }
       }
     }
