@@ -41,10 +41,67 @@ class FockSpace:
         return (self.cdag(o) @ self.c(o)).tocsr()
 
 
+def auto_partition(fs, H):
+    """Finest partition of the Fock space into blocks that H leaves invariant and that every c_o and
+    c^dagger_o maps, block by block, into single blocks: the automatic partition of TRIQS's atom_diag
+    (used where the reference builds its AtomDiag, python/triqs_xca/block_sparse_solver.py), after
+    Seth et al., Comput. Phys. Commun. 200, 274 (2016), appendix on the automatic partition.
+    1. Blocks = connected components of the graph of the nonzero elements of H.
+    2. Repeat until nothing changes: for every operator and every block, merge the blocks that the
+       block's states are mapped into.
+    Returns a list of blocks (sorted Fock-state arrays), ordered by their smallest state."""
+    dim = fs.dim
+    parent = np.arange(dim)
+
+    def find(x):
+        while parent[x] != x:
+            parent[x] = parent[parent[x]]
+            x = parent[x]
+        return x
+
+    def union(a, b):
+        ra, rb = find(a), find(b)
+        if ra == rb:
+            return False
+        if ra < rb:
+            parent[rb] = ra
+        else:
+            parent[ra] = rb
+        return True
+
+    Hc = sp.coo_matrix(H)
+    for i, j, v in zip(Hc.row, Hc.col, Hc.data):
+        if v != 0.0:
+            union(int(i), int(j))
+    ops = []
+    for o in range(fs.n):
+        for M in (fs.c(o), fs.cdag(o)):
+            Mc = M.tocoo()
+            ops.append((Mc.col.astype(np.int64), Mc.row.astype(np.int64)))  # (source, target) pairs
+    changed = True
+    while changed:
+        changed = False
+        for src, dst in ops:
+            first_target = {}  # source block -> a target state
+            for a, t in zip(src, dst):
+                ra = find(int(a))
+                if ra in first_target:
+                    changed |= union(first_target[ra], int(t))
+                else:
+                    first_target[ra] = int(t)
+    roots = np.array([find(x) for x in range(dim)])
+    blocks = {}
+    for x in range(dim):
+        blocks.setdefault(int(roots[x]), []).append(x)
+    return [np.array(v, dtype=np.int64) for _, v in sorted(blocks.items(), key=lambda kv: min(kv[1]))]
+
+
 class AtomDiag:
     """Block-diagonalised local Hamiltonian with block-sparse c and c^dagger.
 
-    H: (dim, dim) real matrix (dense or sparse) in the Fock basis; spin[o] = +1/-1 for S_z blocks.
+    H: (dim, dim) real matrix (dense or sparse) in the Fock basis. qn: "NSz" (spin[o] = +1/-1 gives
+    S_z), "N", "none" (one block), or "auto" (the finest partition compatible with H and the
+    operators, auto_partition).
     """
 
     def __init__(self, fs, H, qn="NSz", spin=None):
@@ -53,7 +110,12 @@ class AtomDiag:
         H = sp.csr_matrix(H)
         states = np.arange(dim)
         N = np.array([bin(s).count("1") for s in states])
-        if qn == "NSz":
+        if qn == "auto":
+            keys = [0] * dim
+            for b, blk in enumerate(auto_partition(fs, H)):
+                for x in blk:
+                    keys[x] = b
+        elif qn == "NSz":
             spin = np.asarray(spin)
             sz = np.array([sum(spin[o] for o in range(n) if (s >> o) & 1) for s in states])
             keys = list(zip(N, sz))

@@ -134,13 +134,14 @@ def anderson_nca(beta=10.0, lam=100.0, eps=1e-10, U=2.0, mu=1.0, t2=0.25):
     return _problem(grid, ad, delta, "anderson_nca")
 
 
-def kanamori(norb, U, J, mu, beta, lam, eps=1e-10, t2=0.25, offdiag=0.0):
-    """Kanamori impurity with (N, S_z) blocks and a semicircular hybridization per flavour; offdiag
-    couples different orbitals of the same spin with relative weight `offdiag`."""
+def kanamori(norb, U, J, mu, beta, lam, eps=1e-10, t2=0.25, offdiag=0.0, qn="NSz"):
+    """Kanamori impurity with (N, S_z) blocks (qn="auto": the finest partition compatible with H and
+    the operators) and a semicircular hybridization per flavour; offdiag couples different orbitals
+    of the same spin with relative weight `offdiag`."""
     grid = DLRGrid(beta, lam, eps)
     fs = FockSpace(2 * norb)
     H = kanamori_hamiltonian(fs, norb, U, J, mu)
-    ad = AtomDiag(fs, H, qn="NSz", spin=kanamori_spins(norb))
+    ad = AtomDiag(fs, H, qn=qn, spin=kanamori_spins(norb))
     d = semicircle_delta_tau(grid.tau, beta, t2=t2)
     n = 2 * norb
     M = np.zeros((n, n))
