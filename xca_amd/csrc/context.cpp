@@ -441,6 +441,17 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
       rg.y = (int)V->h_red.size();
       V->h_red_range.push_back(rg);
     }
+    // queue order: the costliest chunks first (columns in use summed over the chunk's tiles), so the
+    // cheap ones, such as programs' last partly filled tiles, fill the last round. Each chunk keeps
+    // its slot and the reduction its order, so the result does not change.
+    {
+      auto cost = [&](const DChunk &ch) {
+        const DProg &d = V->h_progs[ch.prog];
+        const int64_t items = std::min<int64_t>((int64_t)ch.t1 * d.items_per_tile, d.ntup) - (int64_t)ch.t0 * d.items_per_tile;
+        return items * d.max_w;
+      };
+      std::stable_sort(V->h_chunks.begin(), V->h_chunks.end(), [&](const DChunk &a, const DChunk &b) { return cost(a) > cost(b); });
+    }
     V->nchunks = (int)V->h_chunks.size();
     V->progs.upload(V->h_progs, s);
     V->chunks.upload(V->h_chunks, s);
