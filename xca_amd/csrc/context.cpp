@@ -24,6 +24,18 @@ namespace xca {
 
 // tiles per chunk: enough chunks to cover the GPU several times, at most 64 tiles per chunk
 // (XCA_CHUNK_TILES overrides: longer chunks reuse a shared prefix across more tiles)
+// Queue order of a plan's chunks: the costliest first (columns in use summed over the chunk's
+// tiles), so the cheap ones, such as programs' last partly filled tiles, fill the last round of the
+// persistent launch instead of extending it.
+void costliest_first(std::vector<DChunk> &chunks, const std::vector<DProg> &progs) {
+  auto cost = [&](const DChunk &ch) {
+    const DProg &d = progs[ch.prog];
+    const int64_t items = std::min<int64_t>((int64_t)ch.t1 * d.items_per_tile, d.ntup) - (int64_t)ch.t0 * d.items_per_tile;
+    return items * d.max_w;
+  };
+  std::stable_sort(chunks.begin(), chunks.end(), [&](const DChunk &a, const DChunk &b) { return cost(a) > cost(b); });
+}
+
 int chunk_tiles(int64_t total_tiles) {
   if (const char *e = std::getenv("XCA_CHUNK_TILES")) {
     const int v = std::atoi(e);
@@ -441,17 +453,9 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
       rg.y = (int)V->h_red.size();
       V->h_red_range.push_back(rg);
     }
-    // queue order: the costliest chunks first (columns in use summed over the chunk's tiles), so the
-    // cheap ones, such as programs' last partly filled tiles, fill the last round. Each chunk keeps
-    // its slot and the reduction its order, so the result does not change.
-    {
-      auto cost = [&](const DChunk &ch) {
-        const DProg &d = V->h_progs[ch.prog];
-        const int64_t items = std::min<int64_t>((int64_t)ch.t1 * d.items_per_tile, d.ntup) - (int64_t)ch.t0 * d.items_per_tile;
-        return items * d.max_w;
-      };
-      std::stable_sort(V->h_chunks.begin(), V->h_chunks.end(), [&](const DChunk &a, const DChunk &b) { return cost(a) > cost(b); });
-    }
+    // each chunk keeps its slot and the reduction its order, so the queue order does not change
+    // the result
+    costliest_first(V->h_chunks, V->h_progs);
     V->nchunks = (int)V->h_chunks.size();
     V->progs.upload(V->h_progs, s);
     V->chunks.upload(V->h_chunks, s);

@@ -375,6 +375,7 @@ static CorrDevPlan *get_corr_plan(xca_ctx *c, int m, const int *topo, const int3
       const int ntiles = (d.ntup + d.items_per_tile - 1) / d.items_per_tile;
       for (int t0 = 0; t0 < ntiles; t0 += ct) B->h_chunks.push_back(DChunk{(int)pi, t0, std::min(ntiles, t0 + ct), 0, 0});
     }
+    costliest_first(B->h_chunks, B->h_progs);  // store mode: each item's output has its own place
     B->nchunks = (int)B->h_chunks.size();
     B->ncchunks = (int)cchunks.size();
     B->nrjobs = (int)rjobs.size();

@@ -172,6 +172,7 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
 void accumulate(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t nfix, double scale);
 void wide_accumulate(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t nfix, double scale);
 int chunk_tiles(int64_t total_tiles);
+void costliest_first(std::vector<DChunk> &chunks, const std::vector<DProg> &progs);
 double canonical_macs(const Program &pg, int r);
 DOp device_op(const Op &op, const std::vector<EdgeTable> &tables, const xca_ctx *c);
 std::vector<DTabJob> tab_jobs(const std::vector<EdgeTable> &tables, const xca_ctx *c);
