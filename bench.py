@@ -3,9 +3,15 @@
 One step = one full Sigma evaluation (all connected topologies of orders 1..max_order, with the
 solver's signs) of the OCA 2-orbital Kanamori workload (U = 4, J = 0.5, half filling, beta = 50,
 Lambda = 1000, eps = 1e-10, 16 states in 9 (N, S_z) blocks, semicircular hybridization, G = atomic
-propagator with seed-perturbed DLR coefficients). G stays resident in HBM across steps.
+propagator with seed-perturbed DLR coefficients). G stays resident in HBM across steps; each step
+is one graph-replayed launch sequence (edge tables, one persistent chain launch, one reduction).
 
-Prints one JSON line (see the contract in the repository's README of the benchmark harness).
+Prints one JSON line: metric, value (Sigma-evaluations/s), ms_per_step and the run parameters;
+roofline (the chain launch's canonical FP64 rate against the MI355X peak, its HIP-event time and
+the HBM bytes per launch from the committed rocprofv3 summary); the plain GPU forms beside the
+batched value (plain_gpu); g(tau) evaluations (spgf); NCA solve latency (solve); the served request
+stream (serving); and the oracle on the host's cores (cpu_baseline). --workload tca3 / oca5 measure
+the 3-orbital TCA and 5-orbital Slater-Condon configurations instead.
 """
 import argparse
 import json
