@@ -66,8 +66,8 @@ __device__ __forceinline__ double lu_solve_wave(const double *__restrict__ lu, c
       else if (lane == p) v = a;
     }
   }
-  // branch-free steps: every lane loads its factor entry (inside the kMaxRank^2 array for every
-  // lane) so that the loads of later steps can issue early; lanes outside the update keep v
+  // branch-free steps: every lane loads a factor entry (lanes past r read row 0, inside the r x r
+  // factors) so that the loads of later steps can issue early; lanes outside the update keep v
   const int row = lane < r ? lane : 0;
 #pragma unroll 4
   for (int k = 0; k + 1 < r; ++k) {
