@@ -126,3 +126,22 @@ def test_device_sigma_with_the_automatic_partition(norb, order):
             for sgn, t in all_connected_pairings(o):
                 ref = ref + (-1) ** o * sgn * (-1) ** (o + 1) * _fock(B, [x.real for x in ob.sigma(B.G0, np.array(t, dtype=np.int32))])
         assert np.abs(sb - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+def test_five_orbital_slater_automatic_partition():
+    """5-orbital Slater-Condon d shell: the automatic partition (blocks of at most 16 states, which
+    the tile engine or the large-block engine takes by size) gives the (N, S_z) partition's Sigma
+    (blocks up to 100, large-block engine) in the Fock basis, orders 1-2, on a grid that resolves
+    the atomic energies (beta = 2, Lambda = 800)."""
+    from xca_amd.evaluator import DiagramEvaluator
+    A = models.oca_5orb_slater(beta=2.0, lam=800.0, qn="NSz")
+    B = models.oca_5orb_slater(beta=2.0, lam=800.0, qn="auto")
+    assert max(A.dims) == 100 and max(B.dims) <= 16
+    ea = DiagramEvaluator(A.poles, A.coeffs, A.grid, A.ad)
+    eb = DiagramEvaluator(B.poles, B.coeffs, B.grid, B.ad)
+    sa = _fock(A, ea.pseudo_particle_self_energy(A.G0, 2))
+    sb = _fock(B, eb.pseudo_particle_self_energy(B.G0, 2))
+    err = np.abs(sa - sb).max() / np.abs(sa).max()
+    print(f"[gpu] 5-orbital automatic partition: {len(B.dims)} blocks <= {max(B.dims)}, rel={err:.2e}", flush=True)
+    assert err <= 1e-12

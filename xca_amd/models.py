@@ -167,20 +167,21 @@ def tca_3orb():
     return kanamori(3, 5.0, 0.7, 4.35, beta=100.0, lam=2000.0, offdiag=0.05)
 
 
-def oca_5orb_slater(beta=40.0, lam=800.0, eps=1e-10, F=(4.0, 6.0, 3.75)):
+def oca_5orb_slater(beta=40.0, lam=800.0, eps=1e-10, F=(4.0, 6.0, 3.75), qn="NSz"):
     """5-orbital d shell with the rotationally invariant Slater-Condon interaction, F0 = 4, F2 = 6,
     F4 = 0.625 F2, half filling mu = (45 F0 - 70/49 F2 - 630/441 F4)/10
     (benchmarks/slater_condon/slater_condon_interaction.py:29), 1024 states in 36 (N, S_z) blocks up to
-    100, diagonal semicircular hybridization, beta = 40, Lambda = 800."""
+    100, diagonal semicircular hybridization, beta = 40, Lambda = 800. qn="auto" partitions by the
+    automatic partition instead (276 blocks of at most 16 states)."""
     from .atom import slater_hamiltonian
     F0, F2, F4 = F
     mu = (45 * F0 - 70 / 49 * F2 - 630 / 441 * F4) / 10.0
     grid = DLRGrid(beta, lam, eps)
     fs = FockSpace(10)
     H = slater_hamiltonian(fs, 2, F, mu)
-    ad = AtomDiag(fs, H, qn="NSz", spin=kanamori_spins(5))
+    ad = AtomDiag(fs, H, qn=qn, spin=kanamori_spins(5))
     d = semicircle_delta_tau(grid.tau, beta)
-    return _problem(grid, ad, d[:, None, None] * np.eye(10)[None], "oca5_slater")
+    return _problem(grid, ad, d[:, None, None] * np.eye(10)[None], "oca5_slater" if qn == "NSz" else f"oca5_slater_{qn}")
 
 
 def serving_request(rng, eps=1e-10):
