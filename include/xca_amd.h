@@ -54,6 +54,11 @@ typedef struct xca_problem {
   const int *cdag_conn;  /* (n, nblocks) */
   const double *c_mats;
   const double *cdag_mats;
+  /* Optional (3): the extents of coeffs as the caller holds it. NULL means (p, n, n). When given,
+   * they are checked against (p, n, n) before coeffs is read, with the message of the reference's
+   * Fq-only constructor check (c++/triqs_xca/block_sparse_backbone.cpp:28-40): "hyb_coeffs must have
+   * shape (p, n, n) with p = hyb_poles.size() = .. and n = sum(Fq.sym_set_sizes) = .., got (..)". */
+  const int64_t *coeffs_shape;
 } xca_problem;
 
 typedef struct xca_ctx xca_ctx;

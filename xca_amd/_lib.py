@@ -19,6 +19,7 @@ class XcaProblem(C.Structure):
         ("p", C.c_int), ("poles", d_p), ("coeffs", d_p),
         ("n", C.c_int), ("nblocks", C.c_int), ("dims", i_p),
         ("c_conn", i_p), ("cdag_conn", i_p), ("c_mats", d_p), ("cdag_mats", d_p),
+        ("coeffs_shape", i64_p),
     ]
 
 

@@ -192,6 +192,67 @@ class AtomDiag:
         return Hd
 
 
+class _Flavours:
+    def __init__(self, n):
+        self.n = n
+
+
+class AtomDiagArrays:
+    """The block-sparse operator data an evaluator needs, without a Hamiltonian: flavour count, block
+    dimensions, c / c^dagger block connections (n, nblocks) and the Fock-basis blocks
+    {(o, j): (dims[conn[o, j]], dims[j])}. It is the array form of what get_operators builds from an
+    atom_diag (c++/triqs_xca/atom_diag_utils.cpp:39-185) and stands wherever an AtomDiag is accepted
+    for evaluation."""
+
+    def __init__(self, n, dims, c_conn, cdag_conn, c_blocks, cdag_blocks):
+        self.fs = _Flavours(int(n))
+        self.dims = np.asarray(dims, dtype=np.int32)
+        self.nblocks = len(self.dims)
+        self.c_conn = np.asarray(c_conn, dtype=np.int32).reshape(n, self.nblocks)
+        self.cdag_conn = np.asarray(cdag_conn, dtype=np.int32).reshape(n, self.nblocks)
+        self.c_blocks = dict(c_blocks)
+        self.cdag_blocks = dict(cdag_blocks)
+        for conn, blocks, what in ((self.c_conn, self.c_blocks, "c"), (self.cdag_conn, self.cdag_blocks, "c^dagger")):
+            for o in range(n):
+                for j in range(self.nblocks):
+                    t = conn[o, j]
+                    if t < 0:
+                        continue
+                    if t >= self.nblocks:
+                        raise ValueError(f"AtomDiagArrays: {what}_{o} connects block {j} to block {t}, beyond {self.nblocks} blocks")
+                    shp = np.shape(blocks.get((o, j)))
+                    if shp != (self.dims[t], self.dims[j]):
+                        raise ValueError(f"AtomDiagArrays: {what}_{o} block from {j} has shape {shp}, expected {(self.dims[t], self.dims[j])}")
+
+    packed_ops = AtomDiag.packed_ops
+
+    @classmethod
+    def from_atom_diag(cls, ad):
+        return cls(ad.fs.n, ad.dims, ad.c_conn, ad.cdag_conn, ad.c_blocks, ad.cdag_blocks)
+
+    @classmethod
+    def from_triqs(cls, ad):
+        """From a TRIQS atom_diag as get_operators_impl reads it (atom_diag_utils.cpp:39-148): the
+        fundamental operators (`fops`, flavour o = bit o of a Fock state), the subspaces' Fock states
+        (`fock_states`), and the connections `c_connection` / `cdag_connection` (n, n_subspaces).
+        Blocks are the Fock-basis operator matrices restricted to (final, initial) subspace states."""
+        n = len(ad.fops)
+        fock = [np.asarray(f, dtype=np.int64) for f in ad.fock_states]
+        nb = len(fock)
+        cc = np.asarray(ad.c_connection, dtype=np.int64).reshape(n, nb)
+        dc = np.asarray(ad.cdag_connection, dtype=np.int64).reshape(n, nb)
+        fs = FockSpace(n)
+        cb, db = {}, {}
+        for o in range(n):
+            C, Cd = fs.c(o).tocsr(), fs.cdag(o).tocsr()
+            for j in range(nb):
+                if cc[o, j] >= 0:
+                    cb[(o, j)] = C[fock[cc[o, j]], :][:, fock[j]].toarray()
+                if dc[o, j] >= 0:
+                    db[(o, j)] = Cd[fock[dc[o, j]], :][:, fock[j]].toarray()
+        return cls(n, [len(f) for f in fock], cc, dc, cb, db)
+
+
 # --------------------------------------------------------------------------- Hamiltonians
 
 def kanamori_hamiltonian(fs, norb, U, J, mu):

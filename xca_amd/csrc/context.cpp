@@ -54,6 +54,13 @@ namespace {
 Problem problem_from(const xca_problem *d) {
   if (!d) throw std::invalid_argument("xca_ctx_create: null problem");
   if (d->r <= 0 || d->p <= 0 || d->n <= 0 || d->nblocks <= 0) throw std::invalid_argument("xca_ctx_create: empty problem");
+  if (d->coeffs_shape) {  // the extent check of the Fq-only constructor (block_sparse_backbone.cpp:28-40), before coeffs is read
+    const int64_t *s = d->coeffs_shape;
+    if (s[0] != d->p || s[1] != d->n || s[2] != d->n)
+      throw std::invalid_argument("DiagramEvaluator: hyb_coeffs must have shape (p, n, n) with p = hyb_poles.size() = " + std::to_string(d->p) +
+                                  " and n = sum(Fq.sym_set_sizes) = " + std::to_string(d->n) + ", got (" + std::to_string(s[0]) + ", " +
+                                  std::to_string(s[1]) + ", " + std::to_string(s[2]) + ")");
+  }
   Problem P;
   P.beta = d->beta; P.r = d->r; P.p = d->p; P.n = d->n; P.nb = d->nblocks;
   P.it.assign(d->it, d->it + P.r);

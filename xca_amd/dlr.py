@@ -33,11 +33,35 @@ class DLRGrid:
         rf = np.zeros(max_r)
         it = np.zeros(max_r)
         r = check(lib().xca_dlr_build(self.lam, self.eps, max_r, dptr(rf), dptr(it)))
-        self.r = r
-        self.rf = rf[:r].copy()
-        self.it = it[:r].copy()
+        self._set_nodes(rf[:r].copy(), it[:r].copy())
+
+    def _set_nodes(self, rf, it):
+        self.r = len(rf)
+        self.rf = rf
+        self.it = it
         self.tau = rel2abs(self.it) * self.beta           # absolute times in [0, beta]
         self.cf2it = kernel(self.it[:, None], self.rf[None, :])
+
+    @classmethod
+    def from_nodes(cls, beta, it, rf, lam=None, eps=None):
+        """Grid on externally supplied nodes, e.g. those of the reference's MeshDLRImTime /
+        ImTimeOps (block_sparse_solver.py:105,136): it in relative format (t in [0, 1/2] as t, t in
+        (1/2, 1) as t - 1; absolute times in [0, beta] are accepted and converted), rf dimensionless
+        (beta * w); times tau / beta in [0, 1] are accepted and converted. lam and eps are
+        informational."""
+        it = np.array(it, dtype=np.float64).ravel()
+        rf = np.array(rf, dtype=np.float64).ravel()
+        if it.shape != rf.shape or it.size == 0:
+            raise ValueError(f"DLRGrid.from_nodes: it and rf must be nonempty and of equal length, got {it.size} and {rf.size}")
+        if np.any(np.abs(it) > 1.0):
+            raise ValueError("DLRGrid.from_nodes: times must be relative (tau / beta), not absolute")
+        it = np.where(it > 0.5, it - 1.0, it)
+        g = cls.__new__(cls)
+        g.beta = float(beta)
+        g.lam = float(lam) if lam is not None else float(np.abs(rf).max())
+        g.eps = float(eps) if eps is not None else float("nan")
+        g._set_nodes(rf, it)
+        return g
 
     def vals2coefs(self, g):
         g = np.asarray(g)

@@ -28,13 +28,12 @@ def problem_struct(hyb_poles, hyb_coeffs, grid, ad, keep):
     n = ad.fs.n
     poles = keep(as_f64(hyb_poles).ravel())
     coeffs = keep(_real(hyb_coeffs, "hyb_coeffs"))
-    if coeffs.ndim != 3 or coeffs.shape[0] != len(poles) or coeffs.shape[1] != n or coeffs.shape[2] != n:
-        raise XcaError(f"DiagramEvaluator: hyb_coeffs must have shape (p, n, n) with p = hyb_poles.size() = {len(poles)} "
-                       f"and n = {n}, got {coeffs.shape}")
+    # the library checks the extents against (p, n, n) before it reads the array
+    shape = keep(np.array((list(coeffs.shape) + [0, 0, 0])[:3] if coeffs.ndim <= 3 else [-1, -1, -1], dtype=np.int64))
     cm, dm = ad.packed_ops()
     return XcaProblem(grid.beta, grid.r, dptr(keep(as_f64(grid.it))), dptr(keep(as_f64(grid.rf))), len(poles), dptr(poles), dptr(coeffs),
                       n, ad.nblocks, iptr(keep(as_i32(ad.dims))), iptr(keep(as_i32(ad.c_conn))), iptr(keep(as_i32(ad.cdag_conn))),
-                      dptr(keep(_real(cm, "c"))), dptr(keep(_real(dm, "cdag"))))
+                      dptr(keep(_real(cm, "c"))), dptr(keep(_real(dm, "cdag"))), shape.ctypes.data_as(i64_p))
 
 
 class DiagramEvaluator:
