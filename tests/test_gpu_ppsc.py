@@ -38,7 +38,7 @@ def test_ppsc_solve_matches_oracle(case, order):
     from xca_amd.solver import BlockSparseSolver
     P = getattr(models, case)()
     tol, maxiter = 1e-10, 60
-    S = BlockSparseSolver(P).solve(order, tol=tol, maxiter=maxiter, mix=1.0)
+    S = BlockSparseSolver(P).solve(order, tol=tol, maxiter=maxiter, mix=1.0, hyb_comp=False)
     G_o, eta_o, it_o, diff_o = po.ppsc_solve(P, order, tol=tol, maxiter=maxiter, mix=1.0)
     e = rel_err(S.G, G_o)
     print(f"[gpu] ppsc {case}: iters {S.n_ppsc_iter} vs {it_o}, eta {S.eta:.15e} vs {eta_o:.15e}, rel_err={e:.2e}", flush=True)
@@ -47,3 +47,58 @@ def test_ppsc_solve_matches_oracle(case, order):
     assert e <= 1e-12
     # normalization: Z = -Tr G(beta) = 1 after the last update (block_sparse_api_two_fermions.py:110-112)
     assert S.diff_G < tol
+
+
+def three_orbital_request():
+    """A 3-orbital problem of the serving stream's kind (Kanamori U = 1.42, J = U/8, half filling) on
+    the grid of its beta = 90.6 request (Lambda = 1812): r = 57 and blocks up to 9, so the Dyson
+    systems reach N = r b = 513. At beta = 20 the loop is well conditioned: the oracle in double and
+    in long double agree to 2e-15 on every iterate. (At beta = 90.6 the early iterates grow like
+    exp(beta delta) before normalisation, the double and long-double oracles differ by 2e-4 after
+    five iterations and by 7.5e-12 at convergence, so the converged G is only determined to ~1e-11
+    there.)"""
+    U = 1.42
+    J = U / 8
+    Up = U - 2 * J
+    P = models.kanamori(3, U, J, (U + 2 * (2 * Up - J)) / 2, beta=20.0, lam=20 * 90.6)
+    assert P.grid.r * max(P.dims) == 513
+    return P
+
+
+def test_dyson_three_orbital_blocks_near_500():
+    """dyson_it_ppsc::solve (dlr_dyson_ppsc.hpp:135-168) on every block of the 3-orbital problem,
+    the largest systems 513 x 513, against the oracle."""
+    from concurrent.futures import ThreadPoolExecutor
+    from xca_amd.evaluator import DiagramEvaluator
+    P = three_orbital_request()
+    ev = DiagramEvaluator(P.poles, P.coeffs, P.grid, P.ad)
+    ev.dyson_init(P.G0)
+    S = ev.pseudo_particle_self_energy(P.G0, 2)
+    eta = 0.0
+    g_dev = ev.dyson_solve(S, eta)
+
+    def orc(a):
+        g = po.dyson(P.beta, P.grid.it, P.grid.rf, a[0], a[1], eta).real
+        return 0.5 * (g + g.transpose(0, 2, 1))
+
+    with ThreadPoolExecutor(max_workers=po.default_threads()) as ex:
+        g_orc = list(ex.map(orc, zip(P.G0, S)))
+    e = rel_err(g_dev, g_orc)
+    per_block = [np.abs(a - b).max() / np.abs(b).max() for a, b in zip(g_dev, g_orc)]
+    print(f"[gpu] dyson 3-orbital N<=513 rel_err={e:.2e} worst block {max(per_block):.2e}", flush=True)
+    assert e <= 1e-12
+
+
+def test_ppsc_three_orbital_matches_oracle():
+    """The whole NCA self-consistency of the 3-orbital problem (Dyson systems up to 513) against the
+    oracle's loop: the same iteration count, eta and G."""
+    from xca_amd.solver import BlockSparseSolver
+    P = three_orbital_request()
+    tol, maxiter = 1e-10, 60
+    S = BlockSparseSolver(P).solve(1, tol=tol, maxiter=maxiter, mix=1.0, hyb_comp=False)
+    G_o, eta_o, it_o, diff_o = po.ppsc_solve(P, 1, tol=tol, maxiter=maxiter, mix=1.0, threads=po.default_threads())
+    e = rel_err(S.G, G_o)
+    print(f"[gpu] ppsc 3-orbital: iters {S.n_ppsc_iter} vs {it_o}, eta {S.eta:.15e} vs {eta_o:.15e}, rel_err={e:.2e}", flush=True)
+    assert S.n_ppsc_iter == it_o
+    assert abs(S.eta - eta_o) <= 1e-12 * max(1.0, abs(eta_o))
+    assert e <= 1e-12

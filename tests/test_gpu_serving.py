@@ -54,16 +54,19 @@ def test_server_results_equal_alone(budget_factor):
     if budget_factor > 0:
         probe = Server(max_active=16, tile_variant=0)
         sizes = [probe.request_bytes(probe.submit(P, o, tol=tol, maxiter=maxiter, mix=mix)) for P, o in reqs]
+        shared = probe.shared_bytes()
         del probe
-        budget = int(budget_factor * max(sizes))
-        assert sum(sizes) > budget
+        budget = shared + int(budget_factor * max(sizes))
+        assert sum(sizes) > budget - shared
     sv = Server(mem_budget_bytes=budget, max_active=16, tile_variant=0)
-    out = run_stream(sv, reqs, rate=1e4, seed=5, tol=tol, maxiter=maxiter, mix=mix)
+    out = run_stream(sv, reqs, rate=1e4, seed=5, tol=tol, maxiter=maxiter, mix=mix, check_footprint=True)
     print(f"[gpu] stream of {len(reqs)}: {out['solves_per_s']:.1f} solves/s p50 {out['p50_s'] * 1e3:.1f} ms "
           f"p99 {out['p99_s'] * 1e3:.1f} ms max active {out['max_active']} max bytes {out['max_bytes_in_use']} stats {out['stats']}",
           flush=True)
+    # the admission estimate bounds what an admitted request really holds on the device
+    assert 0 < out["max_footprint_ratio"] <= 1.0
     if budget_factor > 0:
-        assert out["max_bytes_in_use"] <= budget
+        assert out["max_bytes_in_use"] + sv.shared_bytes() <= budget
         assert out["max_active"] < len(reqs)
     worst = 0.0
     for (P, order), res in zip(reqs, out["results"]):
@@ -87,3 +90,30 @@ def test_server_matches_oracle_on_small_problems():
         e = _rel(res["G"], G_o)
         print(f"[gpu] server vs oracle {P.name}: iters {res['iters']} vs {it_o} rel {e:.1e}", flush=True)
         assert res["iters"] == it_o and e <= 1e-12
+
+
+def test_full_serving_stream_equals_alone():
+    """The benchmark's whole stream (512 requests, seed 7: 1-3 orbitals, NCA or OCA, beta in
+    [10, 100], Lambda = 20 beta) through the server, all submitted at once so that the batch runs
+    full: every request converges and equals the same problem solved alone (bit for bit by
+    construction, asserted <= 1e-12)."""
+    import time
+    from xca_amd.serving import Server, run_stream
+    t0 = time.perf_counter()
+    reqs = models.serving_requests(512, seed=7)
+    print(f"[gpu] full stream: built {len(reqs)} requests in {time.perf_counter() - t0:.1f} s", flush=True)
+    tol, maxiter, mix = 1e-10, 200, 1.0
+    sv = Server(max_active=64, tile_variant=0)
+    out = run_stream(sv, reqs, rate=1e9, seed=7, tol=tol, maxiter=maxiter, mix=mix, check_footprint=True)
+    print(f"[gpu] full stream: {out['solves_per_s']:.1f} solves/s p50 {out['p50_s']:.3f} s p99 {out['p99_s']:.3f} s "
+          f"max active {out['max_active']} footprint ratio {out['max_footprint_ratio']:.3f}", flush=True)
+    assert 0 < out["max_footprint_ratio"] <= 1.0
+    worst = 0.0
+    for k, ((P, order), res) in enumerate(zip(reqs, out["results"])):
+        assert res["converged"], P.name
+        g, eta, iters, diff = _alone(P, order, tol, maxiter, mix)
+        assert res["iters"] == iters, P.name
+        worst = max(worst, _rel(res["G"], g), abs(res["eta"] - eta))
+        if k % 128 == 127:
+            print(f"[gpu] full stream: {k + 1} alone solves compared, worst {worst:.1e}", flush=True)
+    assert worst <= 1e-12

@@ -1,18 +1,17 @@
 """Large-block Σ engine (blocks above the tile engine's 12 rows) against the CPU oracle on identical
-inputs, max |S_gpu - S_cpu| / max |S_cpu| <= 1e-12 (or 4x the oracle's own rounding floor where that
-is higher): on the small test models with the engine forced (XCA_WIDE=1), against the tile engine,
+inputs, max |S_gpu - S_cpu| / max |S_cpu| <= 1e-12 (the extended-precision bar of
+parity_util.assert_parity where the double-precision oracle is itself further than that from the
+exact result): on the small test models with the engine forced (XCA_WIDE=1), against the tile engine,
 and on the 5-orbital Slater-Condon d shell whose (N, S_z) blocks reach 100."""
 import numpy as np
 import pytest
 
 import pyoracle as po
-from test_gpu_parity import oracle_noise_floor, rel_err
+from parity_util import RUNS, TOL, assert_parity, check_list, device_runs, rel_err
 from xca_amd import models
 from xca_amd.diag import all_connected_pairings
 
 pytestmark = pytest.mark.gpu
-
-TOL = 1e-12
 
 CASES = {
     "two_fermion": lambda: models.two_fermion(beta=2.0, lam=40.0, eps=1e-12, U=3.0, mu=0.5, pole=0.7),
@@ -44,16 +43,12 @@ def test_wide_engine_matches_oracle(case, order, monkeypatch):
     orc = po.OracleEvaluator(P, complex_arith=False)
     for _, topo in all_connected_pairings(order):
         t = np.array(topo, dtype=np.int32)
-        a = ev.compute_self_energy(G, t)
-        b = [x.real for x in orc.sigma(G, t)]
-        e = rel_err(a, b)
-        floor = oracle_noise_floor(orc, G, t, None)
-        print(f"  topo={topo} rel_err={e:.2e} oracle_floor={floor:.2e}", flush=True)
-        assert e <= max(TOL, 4 * floor)
+        assert_parity(f"wide {case} topo={topo}", ev.compute_self_energy(G, t), P, G, t, orc=orc)
 
 
 def test_wide_equals_tile_engine(monkeypatch):
-    """The two engines sum in different orders; they agree to the algorithm's rounding floor."""
+    """The two engines sum in different orders; each meets the parity bar against the oracle, and
+    they agree with each other to 1e-12 where the oracle does."""
     P = models.two_band_discrete_bath()
     G = P.perturbed_G()
     wide = _evaluator(P, True, monkeypatch)
@@ -62,10 +57,13 @@ def test_wide_equals_tile_engine(monkeypatch):
     for order in (1, 2, 3):
         for _, topo in all_connected_pairings(order):
             t = np.array(topo, dtype=np.int32)
-            e = rel_err(wide.compute_self_energy(G, t), tile.compute_self_energy(G, t))
-            floor = oracle_noise_floor(orc, G, t, None)
-            print(f"[gpu] wide vs tile topo={topo} rel={e:.2e} floor={floor:.2e}", flush=True)
-            assert e <= max(TOL, 4 * floor)
+            a, b = wide.compute_self_energy(G, t), tile.compute_self_energy(G, t)
+            ew, xw = assert_parity(f"wide topo={topo}", a, P, G, t, orc=orc)
+            et, xt = assert_parity(f"tile topo={topo}", b, P, G, t, orc=orc)
+            e = rel_err(a, b)
+            print(f"[gpu] wide vs tile topo={topo} rel={e:.2e}", flush=True)
+            if xw is None and xt is None:
+                assert e <= 2 * TOL
 
 
 def test_wide_flat_index_subsets(monkeypatch):
@@ -82,27 +80,37 @@ def test_wide_flat_index_subsets(monkeypatch):
     assert rel_err(halves, full) <= 1e-14
 
 
-@pytest.mark.parametrize("lam,fixes", [(80.0, [0, 7, 503, 998]), (800.0, [7, 1203])])
-def test_five_orbital_slater_blocks_to_100(lam, fixes):
-    """5-orbital d shell (1024 states, 36 (N, S_z) blocks up to 100): the NCA topology in full and
-    OCA on flat-index samples against the oracle. Lambda = 800 is the benchmark grid (r = 49);
-    Lambda = 80 (r = 25) keeps the oracle fast enough for more samples."""
+def test_five_orbital_resolved_blocks_to_100():
+    """5-orbital d shell (1024 states, 36 (N, S_z) blocks up to 100) on the benchmark grid
+    (Lambda = 800, eps = 1e-10, r = 49) at beta = 5, where the atomic propagator is resolved
+    (beta * E_max = 250 < Lambda): the NCA topology in full and OCA on 16 seeded flat indices, <= 1e-12
+    against the double-precision oracle."""
     from xca_amd.evaluator import DiagramEvaluator
-    P = models.oca_5orb_slater(lam=lam, eps=1e-8 if lam < 800 else 1e-10)
-    assert max(P.dims) == 100
+    P = models.oca_5orb_slater(beta=5.0)
+    assert max(P.dims) == 100 and P.grid.r == 49
     G = P.perturbed_G()
     ev = DiagramEvaluator(P.poles, P.coeffs, P.grid, P.ad)
-    orc = po.OracleEvaluator(P, complex_arith=False)
     t1 = np.array([[0, 1]], dtype=np.int32)
-    e1 = rel_err(ev.compute_self_energy(G, t1), [x.real for x in orc.sigma(G, t1)])
-    print(f"[gpu] 5-orbital lam={lam} r={P.grid.r} NCA rel_err={e1:.2e}", flush=True)
-    assert e1 <= TOL
+    f1 = np.arange(2, dtype=np.int32)
+    check_list("5-orbital resolved NCA", device_runs(ev, G, t1, f1), P, G, t1, f1)
     t2 = np.array([[0, 2], [1, 3]], dtype=np.int32)
-    fx = np.array(fixes, dtype=np.int32)
-    a = ev.compute_self_energy(G, t2, fx)
-    b = [x.real for x in orc.sigma(G, t2, fx)]
-    e2 = rel_err(a, b)
-    floor = oracle_noise_floor(orc, G, t2, fx)
-    print(f"[gpu] 5-orbital lam={lam} OCA sample {fixes} rel_err={e2:.2e} floor={floor:.2e}", flush=True)
-    assert max(np.abs(x).max() for x in b) > 0
-    assert e2 <= max(TOL, 4 * floor)
+    nf = ev.get_num_self_energy_backbones(t2)
+    fx = np.sort(np.random.default_rng(55).choice(nf, size=16, replace=False)).astype(np.int32)
+    check_list(f"5-orbital resolved OCA sample of {len(fx)}", device_runs(ev, G, t2, fx), P, G, t2, fx)
+
+
+def test_five_orbital_as_specified_extended_precision():
+    """The benchmark configuration as specified (beta = 40, Lambda = 800): beta * E_max = 2000 exceeds
+    the cutoff and the double-precision oracle is ~1e-7 from the exact result of these inputs. The
+    device must be at most twice as far from the long-double oracle as the double oracle is, on the
+    NCA topology and four OCA flat indices."""
+    from xca_amd.evaluator import DiagramEvaluator
+    P = models.oca_5orb_slater()
+    G = P.perturbed_G()
+    ev = DiagramEvaluator(P.poles, P.coeffs, P.grid, P.ad)
+    t1 = np.array([[0, 1]], dtype=np.int32)
+    f1 = np.arange(2, dtype=np.int32)
+    assert_parity("5-orbital as specified NCA", device_runs(ev, G, t1, f1), P, G, t1, fix=f1)
+    t2 = np.array([[0, 2], [1, 3]], dtype=np.int32)
+    fx = np.array([7, 777, 1203, 1960 - 5], dtype=np.int32)
+    assert_parity("5-orbital as specified OCA sample", device_runs(ev, G, t2, fx), P, G, t2, fix=fx)

@@ -31,6 +31,7 @@ EXPORTS = {
     "xca_ctx_destroy": (None, [C.c_void_p]),
     "xca_num_backbones": (C.c_int64, [C.c_void_p, C.c_int]),
     "xca_num_sym_sets": (C.c_int, [C.c_void_p]),
+    "xca_num_cached_plans": (C.c_int64, [C.c_void_p]),
     "xca_sigma": (C.c_int, [C.c_void_p, d_p, C.c_int, i_p, i32_p, C.c_int64, d_p]),
     "xca_set_G": (C.c_int, [C.c_void_p, d_p]),
     "xca_sigma_zero": (C.c_int, [C.c_void_p]),
@@ -46,6 +47,8 @@ EXPORTS = {
     "xca_server_poll": (C.c_int, [C.c_void_p, i64_p, C.c_int]),
     "xca_server_result": (C.c_int, [C.c_void_p, C.c_int64, d_p, d_p, C.POINTER(C.c_int), d_p, C.POINTER(C.c_int), d_p]),
     "xca_server_request_bytes": (C.c_int64, [C.c_void_p, C.c_int64]),
+    "xca_server_request_device_bytes": (C.c_int64, [C.c_void_p, C.c_int64]),
+    "xca_server_shared_bytes": (C.c_int64, [C.c_void_p]),
     "xca_server_stats": (C.c_int, [C.c_void_p, i64_p, i64_p, i64_p]),
     "xca_set_tile_variant": (C.c_int, [C.c_void_p, C.c_int]),
     "xca_set_launch_mode": (C.c_int, [C.c_void_p, C.c_int]),
@@ -53,6 +56,7 @@ EXPORTS = {
     "xca_dyson_init": (C.c_int, [C.c_void_p, d_p]),
     "xca_dyson_solve": (C.c_int, [C.c_void_p, d_p, C.c_double, d_p]),
     "xca_ppsc_solve": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_int, C.c_double, d_p, d_p, C.POINTER(C.c_int), d_p]),
+    "xca_ppsc_history": (C.c_int, [C.c_void_p, d_p, C.c_int]),
     "xca_num_correlator_backbones": (C.c_int64, [C.c_void_p, C.c_int]),
     "xca_spgf": (C.c_int, [C.c_void_p, d_p, C.c_int, i_p, i32_p, C.c_int64, d_p]),
     "xca_correlator": (C.c_int, [C.c_void_p, d_p, C.c_int, i_p, i32_p, C.c_int64, C.c_int, i_p, d_p, C.c_int, i_p, d_p, d_p]),

@@ -39,6 +39,7 @@ namespace xca {
 xca_ctx *ctx_init_host(const xca_problem *desc);
 void ctx_init_device(xca_ctx *c, int device);
 int64_t ctx_bytes(const xca_ctx *c);
+int64_t ctx_device_bytes(const xca_ctx *c);
 int64_t plan_bytes_estimate(xca_ctx *c, int m, const int *topo);
 
 constexpr int kStatusStride = 72;  // doubles per problem status slot (blocks <= 64)
@@ -55,6 +56,7 @@ struct BatchItem {
   std::vector<double> bmax;    // block maxima of the current G (zero-block flags of the next Sigma)
   std::vector<DevPlan *> plans;
   std::vector<double> scales;
+  std::vector<double> history;  // per iteration: max |G_new - G|, Z before the normalisation, eta
 };
 
 class Batch {
@@ -140,6 +142,7 @@ class Batch {
       it->iters += 1;
       it->eta = st[kStEta];
       it->diff = st[kStDiff];
+      it->history.insert(it->history.end(), {st[kStDiff], st[kStZ], st[kStEta]});
       for (int b = 0; b < it->c->M.P.nb; ++b) it->bmax[b] = st[kStBmax + b];
       if (st[kStConv] != 0.0) {
         it->done = true;
@@ -405,12 +408,22 @@ extern "C" int xca_ppsc_solve(xca_ctx *c, double eta0, int max_order, double tol
     if (eta_out) *eta_out = it.eta;
     if (iters_out) *iters_out = it.iters;
     if (diff_out) *diff_out = it.diff;
+    c->ppsc_history = it.history;
     batch.remove(&it);
     return 0;
   } catch (std::exception &e) {
     xca::set_error(e.what());
     return -1;
   }
+}
+
+// Per-iteration record of the last xca_ppsc_solve on this context: rows of (max |G_new - G|, Z before
+// the normalisation, eta); returns the number of iterations.
+extern "C" int xca_ppsc_history(xca_ctx *c, double *out, int max_iters) {
+  if (!c) return -1;
+  const int n = (int)(c->ppsc_history.size() / 3);
+  if (out && max_iters > 0) std::copy(c->ppsc_history.begin(), c->ppsc_history.begin() + 3 * std::min(n, max_iters), out);
+  return n;
 }
 
 // ======================================================================= full-order Sigma
@@ -607,7 +620,7 @@ struct XcaRequest {
 struct xca_server {
   int device = 0;
   hipStream_t stream = nullptr;
-  int64_t budget = 0, in_use = 0;
+  int64_t budget = 0, in_use = 0, reserved = 0;  // budget: for requests, after the shared launch buffers (reserved)
   int max_active = 0;
   int tile_variant = 0;
   bool graphs = true;
@@ -642,6 +655,15 @@ extern "C" xca_server *xca_server_create(int device, int64_t mem_budget_bytes, i
     size_t fr = 0, tot = 0;
     HIP_CHECK(hipMemGetInfo(&fr, &tot));
     sv->budget = mem_budget_bytes > 0 ? mem_budget_bytes : (int64_t)(0.8 * (double)fr);
+    // the batched chain launches share per-workgroup shared-prefix caches, one per (padded rank,
+    // tile variant) group: the server's variant and the widest one, which plans fall back to
+    for (int rb : {2, 4})
+      for (int v : {tile_variant, 0}) {
+        sv->reserved += (int64_t)chain_resident(rb, v) * 16 * rb * tile_columns(v) * (int64_t)sizeof(double);
+        if (v == tile_variant && tile_variant == 0) break;
+      }
+    sv->budget -= sv->reserved;
+    if (sv->budget <= 0) throw std::invalid_argument("xca_server_create: memory budget too small for the shared launch buffers");
     HIP_CHECK(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking));
     sv->batch = std::make_unique<Batch>(device, sv->stream, max_active);
     return sv;
@@ -794,6 +816,17 @@ extern "C" int64_t xca_server_request_bytes(xca_server *sv, int64_t id) {
   if (!sv) return -1;
   auto f = sv->all.find(id);
   return f == sv->all.end() ? -1 : f->second->bytes;
+}
+
+// Bytes of the budget reserved at creation for the launch buffers shared by all requests.
+extern "C" int64_t xca_server_shared_bytes(xca_server *sv) { return sv ? sv->reserved : -1; }
+
+// Device bytes an admitted request holds now (tables, Dyson state, plans); -1 if it is not active.
+extern "C" int64_t xca_server_request_device_bytes(xca_server *sv, int64_t id) {
+  if (!sv) return -1;
+  auto f = sv->all.find(id);
+  if (f == sv->all.end() || !f->second->admitted || f->second->finished) return -1;
+  return ctx_device_bytes(f->second->ctx.get());
 }
 
 // Counters: iterations run (ticks) and device bytes in use by admitted requests.

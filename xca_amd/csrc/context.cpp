@@ -19,6 +19,7 @@
 #include "host/linalg.hpp"
 #include "host/model.hpp"
 #include "plan.hpp"
+#include "ppsc.hpp"
 
 namespace xca {
 
@@ -198,6 +199,7 @@ struct ProgSpec {
   DProg d;
   std::vector<int> tuples;  // explicit subset: tuple indices in the class enumeration
   bool explicit_subset = false;
+  int tup_off = -1;         // its offset in the plan's tuple list
   int nl = 0;
   int fb[kMaxLines + 1];
   int cls = 0;
@@ -269,6 +271,29 @@ static int64_t variant_bytes(const DevVariant &V) {
                    V.red_range.n * sizeof(I2) + V.partial.n * sizeof(double));
 }
 
+// Mark the context's current zero-block pattern as most recently used. When more than
+// kMaxZeroPatterns patterns have plans, the plans of the least recently used one are released: the
+// combined-launch descriptors and every device launch that may use them are finished first.
+static void touch_zero_pattern(xca_ctx *c) {
+  auto &L = c->zero_lru;
+  auto f = std::find(L.begin(), L.end(), c->G_zero);
+  if (f != L.end()) L.erase(f);
+  L.push_back(c->G_zero);
+  while ((int)L.size() > xca_ctx::kMaxZeroPatterns) {
+    const std::vector<int> old = L.front();
+    L.erase(L.begin());
+    HIP_CHECK(hipDeviceSynchronize());
+    c->sigall.reset();
+    const size_t nb = old.size();
+    for (auto e = c->plans.begin(); e != c->plans.end();) {
+      const auto &k = e->first;
+      const size_t z0 = 1 + 2 * (size_t)k[0];
+      if (k.size() >= z0 + nb && std::equal(old.begin(), old.end(), k.begin() + z0)) e = c->plans.erase(e);
+      else ++e;
+    }
+  }
+}
+
 DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t nfix) {
   std::vector<int> key;
   key.push_back(m);
@@ -278,6 +303,7 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
     key.push_back(-1);
     key.insert(key.end(), fix, fix + nfix);
   }
+  touch_zero_pattern(c);
   auto it = c->plans.find(key);
   if (it != c->plans.end()) return it->second.get();
   if (fix)  // one explicit-subset plan at a time: a caller sweeping flat indices does not accumulate plans
@@ -348,7 +374,9 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
       auto w = wanted.find({pg.fb_ix, pg.o_ix});
       if (w == wanted.end()) continue;
       for (int pix : w->second) {
-        int u = 0, stride = 1, cls = 0, pp = pix;
+        // the pole index runs with line 0 fastest (backbone.cpp:141-150); the device's tuple index
+        // with line 0 slowest, in the order the chain kernel decodes it
+        int u = 0, cls = 0, pp = pix;
         for (int x = 0; x < nl; ++x) {
           int l = pp % p;
           pp /= p;
@@ -358,13 +386,14 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
           cls |= rb << x;
           const auto &lst = c->ps.get(fwd, rb);
           int pos = (int)(std::find(lst.begin(), lst.end(), l) - lst.begin());
-          u += pos * stride;
-          stride *= (int)lst.size();
+          u = u * (int)lst.size() + pos;
         }
         if (cls == pg.cls) sp.tuples.push_back(u);
       }
       if (sp.tuples.empty()) continue;
       ntup = (int64_t)sp.tuples.size();
+      sp.tup_off = (int)dp->h_tuples.size();
+      dp->h_tuples.insert(dp->h_tuples.end(), sp.tuples.begin(), sp.tuples.end());
     }
     d.ntup = (int)ntup;
     // ops before the first one that involves a line other than the first depend on the first
@@ -418,17 +447,20 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
     for (auto &sp : specs) {
       DProg d = sp.d;
       d.items_per_tile = std::max(1, NT / d.max_w);
-      if (!sp.explicit_subset) {
+      if (!sp.explicit_subset || c->launch_mode == 0) {
+        // an explicit subset keeps its requested tuples in a list that the items index
+        if (sp.explicit_subset) d.tup_off = sp.tup_off;
         int pidx = (int)V->h_progs.size();
         V->h_progs.push_back(d);
         int ntiles = (d.ntup + d.items_per_tile - 1) / d.items_per_tile;
         for (int t0 = 0; t0 < ntiles; t0 += ct) V->h_chunks.push_back(DChunk{pidx, t0, std::min(ntiles, t0 + ct), 0, 0});
       } else {
-        // each requested tuple becomes a one-tuple program whose pole lists have length 1
+        // launch mode 1: each requested tuple becomes a one-tuple program whose pole lists have
+        // length 1, so that one chunk is one (path, pole tuple) item
         for (int u : sp.tuples) {
           DProg d1 = d;
           int decode = u;
-          for (int x = 0; x < sp.nl; ++x) {
+          for (int x = sp.nl - 1; x >= 0; --x) {  // line nl - 1 is the fastest digit
             int len = d.set_len[x];
             int pos = decode % len;
             decode /= len;
@@ -475,11 +507,12 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
   }
   dp->h_tab = tab_jobs(plan.tables, c);
   dp->ops.upload(dp->h_ops, s);
+  dp->tuples.upload(dp->h_tuples.empty() ? std::vector<int>{0} : dp->h_tuples, s);
   dp->tabjobs.upload(dp->h_tab, s);
   dp->ftab.alloc(std::max<int64_t>(plan.ftab_size, 1));
   dp->queue.alloc(1);
   dp->ntabjobs = (int)dp->h_tab.size();
-  dp->device_bytes = (int64_t)(dp->ops.n * sizeof(DOp) + dp->tabjobs.n * sizeof(DTabJob) + dp->ftab.n * sizeof(double));
+  dp->device_bytes = (int64_t)(dp->ops.n * sizeof(DOp) + dp->tabjobs.n * sizeof(DTabJob) + dp->ftab.n * sizeof(double) + dp->tuples.n * sizeof(int));
   for (auto &V : dp->var)
     if (V) dp->device_bytes += variant_bytes(*V);
   HIP_CHECK(hipStreamSynchronize(s));
@@ -493,7 +526,7 @@ ChainArgs chain_args(xca_ctx *c, DevPlan *dp, DevVariant *V) {
   A.D = c->D;
   A.ops = dp->ops.ptr; A.progs = V->progs.ptr; A.chunks = V->chunks.ptr;
   A.plist = c->plist.ptr; A.G = c->G.ptr; A.opmats = c->opmats.ptr; A.ftab = dp->ftab.ptr; A.kp = c->kp.ptr; A.km = c->km.ptr;
-  A.kpow = c->kpow.ptr; A.partial = V->partial.ptr;
+  A.kpow = c->kpow.ptr; A.partial = V->partial.ptr; A.tuples = dp->tuples.ptr;
   A.nchunks = V->nchunks;
   A.r = c->M.P.r; A.p = c->M.P.p; A.maxexp = c->maxexp;
   return A;
@@ -781,31 +814,60 @@ int64_t ctx_bytes(const xca_ctx *c) {
          4 * (4 * P.p + 2 * P.nb);
 }
 
-// Device bytes a plan of this topology would take with the widest tile, from the host program list.
+// Upper bound of the device bytes a plan of this topology takes at the context's current zero-block
+// pattern: edge tables, op list and table jobs once, plus, for every tile variant get_plan allocates
+// (the forced one and the widest as its fallback, or all of them when the autotuner chooses), its
+// programs, chunk descriptors, partial slots and reduction lists at that variant's tile count.
 int64_t plan_bytes_estimate(xca_ctx *c, int m, const int *topo) {
   Plan plan;
   build_programs(c->M, c->ps, topo, m, 0, c->G_zero, plan);
   const Problem &P = c->M.P;
-  const int NT = tile_columns(0);
-  int64_t est_tiles = 0;
-  std::vector<std::pair<int64_t, int>> per_prog;  // (tiles, b0)
+  std::vector<int> cand;
+  if (c->tile_variant >= 0) {
+    cand.push_back(c->tile_variant);
+    if (c->tile_variant != 0) cand.push_back(0);
+  } else {
+    for (int v = 0; v < num_tile_variants(); ++v) cand.push_back(v);
+  }
+  struct PS { int64_t ntup; int max_w, b0; };
+  std::vector<PS> ps;
+  int64_t nops = 0;
   for (auto &pg : plan.progs) {
     int fb[kMaxLines + 1];
     for (int i = 0, f = pg.fb_ix; i < m; ++i, f /= 2) fb[i] = f % 2;
     int64_t ntup = 1;
     for (int x = 0; x < m - 1; ++x) ntup *= (int64_t)c->ps.get(fb[x + 1], (pg.cls >> x) & 1).size();
-    const int per = std::max(1, NT / std::max(1, pg.max_w));
-    const int64_t nt = (ntup + per - 1) / per;
-    est_tiles += nt;
-    per_prog.push_back({nt, pg.b0});
+    ps.push_back({ntup, std::max(1, pg.max_w), pg.b0});
+    nops += (int64_t)pg.ops.size();
   }
-  const int ct = chunk_tiles(est_tiles);
-  int64_t bytes = 8 * plan.ftab_size;
-  for (auto &pp : per_prog) {
-    const int64_t nch = (pp.first + ct - 1) / ct;
-    bytes += nch * (8 * (int64_t)P.r * pp.second * pp.second + (int64_t)sizeof(DChunk) + 12);
+  int64_t bytes = 8 * plan.ftab_size + nops * (int64_t)sizeof(DOp) + (int64_t)(plan.tables.size() + 1) * 64 * (int64_t)sizeof(DTabJob);
+  for (int v : cand) {
+    const int NT = tile_columns(v);
+    int64_t est_tiles = 0;
+    bool ok = true;
+    for (auto &q : ps) {
+      if (q.max_w > NT) { ok = false; break; }
+      const int per = std::max(1, NT / q.max_w);
+      est_tiles += (q.ntup + per - 1) / per;
+    }
+    if (!ok) continue;
+    const int ct = chunk_tiles(est_tiles);
+    for (auto &q : ps) {
+      const int per = std::max(1, NT / q.max_w);
+      const int64_t nt = (q.ntup + per - 1) / per, nch = (nt + ct - 1) / ct;
+      bytes += nch * (8 * (int64_t)P.r * q.b0 * q.b0 + (int64_t)sizeof(DChunk) + (int64_t)sizeof(int64_t) + (int64_t)sizeof(int));
+    }
+    bytes += (int64_t)ps.size() * (int64_t)sizeof(DProg) + (int64_t)P.nb * (int64_t)sizeof(I2);
   }
-  return bytes + (int64_t)plan.progs.size() * (sizeof(DProg) + 4 * sizeof(DOp));
+  return bytes;
+}
+
+// Device bytes a context holds now: resident tables, Dyson state and its plans (every variant
+// still allocated).
+int64_t ctx_device_bytes(const xca_ctx *c) {
+  int64_t b = ctx_bytes(c) + (c->dyson ? dyson_bytes(c) : 0);
+  for (auto &kv : c->plans) b += kv.second->device_bytes;
+  return b;
 }
 
 }  // namespace xca
@@ -840,6 +902,7 @@ extern "C" void xca_ctx_destroy(xca_ctx *c) {
 }
 
 extern "C" int64_t xca_num_backbones(xca_ctx *c, int m) { return c ? c->M.num_backbones(m) : -1; }
+extern "C" int64_t xca_num_cached_plans(xca_ctx *c) { return c ? (int64_t)c->plans.size() : -1; }
 extern "C" int xca_num_sym_sets(xca_ctx *c) { return c ? c->M.S.q : -1; }
 
 extern "C" int xca_set_G(xca_ctx *c, const double *G) {
@@ -897,7 +960,18 @@ extern "C" int xca_sigma(xca_ctx *c, const double *G, int m, const int *topo, co
     if (xca_set_G(c, G) < 0) return -1;
     HIP_CHECK(launch_fill(c->M.G_size, 0.0, c->sigma.ptr, c->stream));
     accumulate(c, m, topo, fix, fix ? nfix : 0, 1.0);
-    return xca_get_sigma(c, out);
+    if (xca_get_sigma(c, out) < 0) return -1;
+    // compute_self_energy returns the blocks whose max |.| is below 1e-16 as zero blocks
+    // (set_zero_block_indices, block_sparse.cpp:92-101, called at block_sparse_backbone.cpp:325,336)
+    const Problem &P = c->M.P;
+    for (int b = 0; b < P.nb; ++b) {
+      double *s = out + c->M.G_off[b];
+      const int64_t sz = (int64_t)P.r * P.dims[b] * P.dims[b];
+      double mx = 0.0;
+      for (int64_t x = 0; x < sz; ++x) mx = std::max(mx, std::fabs(s[x]));
+      if (mx < 1e-16) std::fill(s, s + sz, 0.0);
+    }
+    return 0;
   })
 }
 
@@ -923,6 +997,12 @@ extern "C" int xca_set_launch_mode(xca_ctx *c, int mode) {
   if (mode < 0 || mode > 1) {
     xca::set_error("xca_set_launch_mode: mode must be 0 or 1");
     return -1;
+  }
+  if (mode != c->launch_mode) {  // explicit-subset plans are laid out for the launch mode
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto e = c->plans.begin(); e != c->plans.end();)
+      e = std::find(e->first.begin(), e->first.end(), -1) != e->first.end() ? c->plans.erase(e) : std::next(e);
   }
   c->launch_mode = mode;
   return 0;

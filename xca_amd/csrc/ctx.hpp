@@ -76,6 +76,10 @@ struct xca_ctx {
   std::vector<int> G_zero;
   bool have_G = false;
   Plans plans;
+  // zero-block patterns that have plans, least recently used first; at most kMaxZeroPatterns are
+  // kept, so a G whose zero blocks keep changing does not accumulate plans
+  std::vector<std::vector<int>> zero_lru;
+  static constexpr int kMaxZeroPatterns = 4;
   std::map<std::vector<int>, std::unique_ptr<xca::CorrDevPlan>> corr_plans;
   bool wide = false;  // blocks above the tile engine's 12 rows: the large-block engine evaluates Sigma
   bool outer_events = false;  // ev0/ev1 bracket a sequence of evaluations: inner launches leave them
@@ -86,6 +90,7 @@ struct xca_ctx {
   XcaDyson *dyson = nullptr;
   int64_t last_items = 0;
   double last_macs = 0, last_ms = 0;
+  std::vector<double> ppsc_history;  // (diff, Z, eta) per iteration of the last xca_ppsc_solve
 };
 
 void xca_dyson_release(xca_ctx *c);

@@ -94,7 +94,7 @@ __device__ __forceinline__ v4d mfma16x16x4(double a, double b, v4d c) { return _
 constexpr int kDysonThreads = 1024;
 constexpr int kDysonMaxNp = 784;  // padded r n: DLR rank <= 64, blocks <= 12
 constexpr int kMaxRank = 64;      // DLR rank limit of the device path (checked at context creation)
-constexpr int kLdsSysMax = 112;   // launches whose largest padded system fits keep the systems in LDS
+constexpr int kLdsSysMax = kDysonLdsSysMax;
 
 // Orders the LDS accesses of one wave around this point (a wave executes its LDS operations in
 // issue order; this keeps the compiler from moving them across it).

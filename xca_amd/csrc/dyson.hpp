@@ -7,6 +7,10 @@
 
 namespace xca::dev {
 
+// Launches whose largest padded system (r n rounded up to 16) has at most this many rows keep every
+// block's system in LDS and never touch DysonArgs::sys.
+constexpr int kDysonLdsSysMax = 112;
+
 // Dyson solve of every block of one problem.
 struct DysonArgs {
   int r, nb;

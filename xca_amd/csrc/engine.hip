@@ -453,7 +453,7 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
     __syncthreads();
     // pole tuple of each item, prefactor
     for (int it = tid; it < P_items; it += NT) {
-      int u = tstart + it;
+      int u = P.tup_off >= 0 ? A.tuples[P.tup_off + tstart + it] : tstart + it;
       double pref = 1.0;
       int ls[kMaxLines];
 #pragma unroll

@@ -31,6 +31,8 @@ struct DProg {
   int split;      // ops [0, split) depend on the first line's pole only (0: nothing shared)
   int share_len;  // consecutive items with the same first-line pole
   int gtr;        // start from the transposed G block
+  int tup_off = -1;  // >= 0: item u of the program is pole tuple tuples[tup_off + u] (an explicit
+                     // flat-index subset); < 0: item u is tuple u
   int64_t out_off;  // < 0: accumulate into the chunk's Sigma slot; else store item u's T (r, rows,
                     // cols) at store + out_off + u r rows cols
 };
@@ -76,6 +78,7 @@ struct ChainArgs {
   const double *G, *opmats, *ftab, *kp, *km, *kpow;
   double *partial;
   double *store;               // per-item outputs of store-mode programs
+  const int *tuples;           // tuple lists of explicit-subset programs (DProg::tup_off)
   int nchunks;
   int r, p, maxexp;
 };

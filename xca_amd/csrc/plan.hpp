@@ -73,6 +73,8 @@ struct DevPlan {
   std::vector<std::unique_ptr<DevVariant>> var;  // index = tile variant (null if not applicable)
   std::vector<DOp> h_ops;
   std::vector<DTabJob> h_tab;
+  DevBuf<int> tuples;          // pole-tuple lists of explicit-subset programs
+  std::vector<int> h_tuples;
   // one-segment edge-table launch descriptor
   DevBuf<TabSeg> tseg;
   DevBuf<int> tseg_prefix;

@@ -246,3 +246,10 @@ class DiagramEvaluator:
         check(lib().xca_ppsc_solve(self._h, float(eta0), int(max_order), float(tol), int(maxiter), float(mix), dptr(out), C.byref(eta),
                                    C.byref(it), C.byref(diff)))
         return self.unpack(out), eta.value, it.value, diff.value
+
+    def ppsc_history(self):
+        """Per-iteration report of the last ppsc_solve: [(diff_G, Z before the normalisation, eta)]."""
+        n = check(lib().xca_ppsc_history(self._h, None, 0))
+        h = np.zeros(3 * max(n, 1))
+        check(lib().xca_ppsc_history(self._h, dptr(h), n))
+        return [tuple(h[3 * k:3 * k + 3]) for k in range(n)]

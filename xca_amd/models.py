@@ -184,6 +184,16 @@ def oca_5orb_slater(beta=40.0, lam=800.0, eps=1e-10, F=(4.0, 6.0, 3.75), qn="NSz
     return _problem(grid, ad, d[:, None, None] * np.eye(10)[None], "oca5_slater" if qn == "NSz" else f"oca5_slater_{qn}")
 
 
+def compressed(P, tol):
+    """The same problem with its hybridization compressed to fewer poles (xca_amd.hybfit, relative fit
+    error <= tol; the DLR expansion when no fit reaches tol)."""
+    from .hybfit import fit_hybridization
+    from .dlr import kernel
+    K = kernel(P.grid.it[:, None], P.beta * np.asarray(P.poles)[None, :])
+    poles, coeffs, err = fit_hybridization(P.grid, np.einsum("tl,lab->tab", K, P.coeffs), tol=tol, compression=True)
+    return ImpurityProblem(grid=P.grid, ad=P.ad, poles=poles, coeffs=coeffs, G0=P.G0, eta0=P.eta0, name=f"{P.name}_p{len(poles)}")
+
+
 def serving_request(rng, eps=1e-10):
     """One request of the serving stream: 1-3 orbitals, Kanamori U ~ U[1, 6], J = U/8, half filling
     mu = [U + (M - 1)(2U' - J)]/2 with U' = U - 2J, beta ~ U[10, 100], Lambda = 20 beta, semicircular

@@ -74,7 +74,16 @@ class Server:
                 "queued_s": times[0], "run_s": times[1], "total_s": times[2]}
 
     def request_bytes(self, rid):
+        """Estimated device footprint of a submitted request (an upper bound, used for admission)."""
         return int(lib().xca_server_request_bytes(self._h, int(rid)))
+
+    def shared_bytes(self):
+        """Bytes of the budget reserved for the launch buffers all requests share."""
+        return int(lib().xca_server_shared_bytes(self._h))
+
+    def request_device_bytes(self, rid):
+        """Device bytes an admitted request holds now; -1 while queued or after it finished."""
+        return int(lib().xca_server_request_device_bytes(self._h, int(rid)))
 
     def stats(self):
         t, u, b = C.c_int64(0), C.c_int64(0), C.c_int64(0)
@@ -82,10 +91,12 @@ class Server:
         return {"ticks": t.value, "bytes_in_use": u.value, "budget": b.value}
 
 
-def run_stream(server, requests, rate, seed=7, tol=1e-10, maxiter=200, mix=1.0, log=None):
+def run_stream(server, requests, rate, seed=7, tol=1e-10, maxiter=200, mix=1.0, log=None, check_footprint=False):
     """Submit `requests` [(ImpurityProblem, order), ...] with Poisson arrivals at `rate` per second and
     step the server until every result is back. Latency = time from a request's arrival to its
-    converged result (queueing included). Returns a dict of results and summary figures."""
+    converged result (queueing included). Returns a dict of results and summary figures;
+    check_footprint also records the largest ratio of a request's measured device bytes to its
+    admission estimate over every step."""
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(requests)))
     arrivals -= arrivals[0]
@@ -96,6 +107,7 @@ def run_stream(server, requests, rate, seed=7, tol=1e-10, maxiter=200, mix=1.0, 
     finish = np.zeros(len(requests))
     max_active_seen = 0
     max_bytes = 0
+    max_ratio = 0.0
     while done < len(requests):
         now = time.perf_counter() - t0
         while nxt < len(requests) and arrivals[nxt] <= now:
@@ -105,6 +117,11 @@ def run_stream(server, requests, rate, seed=7, tol=1e-10, maxiter=200, mix=1.0, 
         a, q, f = server.step()
         max_active_seen = max(max_active_seen, a + f)
         max_bytes = max(max_bytes, server.stats()["bytes_in_use"])
+        if check_footprint:
+            for rid in rid_of:
+                used = server.request_device_bytes(rid)
+                if used >= 0:
+                    max_ratio = max(max_ratio, used / server.request_bytes(rid))
         for rid in server.poll():
             i = rid_of.pop(rid)
             results[i] = server.result(rid)
@@ -118,4 +135,4 @@ def run_stream(server, requests, rate, seed=7, tol=1e-10, maxiter=200, mix=1.0, 
     span = finish.max() - arrivals[0]
     return {"results": results, "latency_s": lat, "arrivals_s": arrivals, "finish_s": finish, "solves_per_s": len(requests) / span,
             "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)), "span_s": float(span),
-            "max_active": max_active_seen, "max_bytes_in_use": max_bytes, "stats": server.stats()}
+            "max_active": max_active_seen, "max_bytes_in_use": max_bytes, "max_footprint_ratio": max_ratio, "stats": server.stats()}

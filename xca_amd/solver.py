@@ -1,43 +1,136 @@
 """Pseudo-particle self-consistency driver with the reference solver's interface
-(python/triqs_xca/block_sparse_solver.py: BlockSparseSolver.solve, eval_pseudo_particle_self_energy,
-solve_dyson, normalize_pseudo_particle_gf, partition_function_from_ppgf). Sigma evaluation, the Dyson
-solve, normalization and mixing all run on the GPU; this class only holds the state."""
+(python/triqs_xca/block_sparse_solver.py, BlockSparseSolver). The hybridization fit runs on the host
+(xca_amd.hybfit); Sigma, the Dyson solve, the normalisation, the convergence test and the mixing of
+every iteration run on the GPU (xca_ppsc_solve, one captured graph per iteration); the per-iteration
+report is read back afterwards.
+
+Parameters and defaults follow BlockSparseSolver.solve (block_sparse_solver.py:240-355): hyb_tol
+defaults to 0.1 tol, the hybridization is compressed unless hyb_comp is False or the order is 1 (NCA
+does not use the pole representation), and the single-particle Green's function is evaluated at the
+end to spgf_max_order. The normalisation is the classic one (block_sparse_solver.py:427-449); the
+root and ODE variants of the reference are not on the device path.
+"""
 import numpy as np
 
+from . import hybfit
+from .dlr import kernel
 from .evaluator import DiagramEvaluator
 
 
+class Hybridization:
+    """Pole representation in use: poles (energy units), coefficients (p, n, n) and how they were
+    obtained (block_sparse_solver.py:181-185)."""
+
+    def __init__(self, poles, coefficients, tol=None, compression=False, fit_error=None):
+        self.poles = np.asarray(poles, dtype=np.float64)
+        self.coefficients = np.asarray(coefficients)
+        self.tol, self.compression, self.fit_error = tol, compression, fit_error
+
+
 class BlockSparseSolver:
+    NORMALIZATIONS = ("classic",)
+
     def __init__(self, problem, device=0):
         self.problem = problem
+        self.device = device
         self.beta = problem.beta
         self.G0 = problem.G0
         self.eta0 = problem.eta0
         self.G = [g.copy() for g in problem.G0]
-        self.eta = 0.0  # shift on top of the atomic G0 (block_sparse_solver.py:124)
-        self.d = DiagramEvaluator(problem.poles, problem.coeffs, problem.grid, problem.ad, device=device)
+        self.eta = 0.0  # shift on top of the atomic G0 (block_sparse_solver.py:103)
+        self.history = []
+        self.set_hybridization_poles_and_coefficients(problem.poles, problem.coeffs)
+
+    # ------------------------------------------------------------------ hybridization
+    def delta_tau(self):
+        """Delta at the DLR nodes (r, n, n) of the problem's hybridization."""
+        P = self.problem
+        K = kernel(P.grid.it[:, None], P.beta * np.asarray(P.poles)[None, :])
+        return np.einsum("tl,lab->tab", K, P.coeffs)
+
+    def set_hybridization_poles_and_coefficients(self, poles, coefficients):
+        """block_sparse_solver.py:220-238: a new evaluator on these poles."""
+        self.hyb = Hybridization(poles, coefficients)
+        self.d = DiagramEvaluator(self.hyb.poles, self.hyb.coefficients, self.problem.grid, self.problem.ad, device=self.device)
         self.d.dyson_init(self.G0)
 
+    def fit_hybridization(self, tol=None, compression=False, verbose=False):
+        """block_sparse_solver.py:146-185: with compression a pole fit to relative error tol, else (or
+        when the fit fails) the DLR expansion."""
+        poles, coeffs, err = hybfit.fit_hybridization(self.problem.grid, self.delta_tau(), tol=tol, compression=compression, verbose=verbose)
+        if verbose:
+            print(f"Hybridization: {len(poles)} poles, fit error {err:.2E}")
+        self.set_hybridization_poles_and_coefficients(poles, coeffs)
+        self.hyb.tol, self.hyb.compression, self.hyb.fit_error = tol, compression, err
+
+    # ------------------------------------------------------------------ pieces of the loop
     def eval_pseudo_particle_self_energy(self, G, max_order):
         return self.d.pseudo_particle_self_energy(G, max_order)
 
     def solve_dyson(self, Sigma, eta):
         return self.d.dyson_solve(Sigma, eta)
 
-    def solve(self, max_order, tol=1e-4, maxiter=10, mix=1.0):
+    def partition_function_from_ppgf(self, G):
+        """Z = -Tr G(beta) (block_sparse_solver.py:482-487), G evaluated at tau = beta from its DLR
+        coefficients."""
+        grid = self.problem.grid
+        kb = kernel(np.ones(1), grid.rf[None, :])[0]
+        return float(-sum(np.trace(np.tensordot(kb, grid.vals2coefs(g), axes=(0, 0))) for g in G))
+
+    def normalize_pseudo_particle_gf(self, G):
+        """Classic normalisation (block_sparse_solver.py:427-449): eta += ln|Z| / beta and
+        G <- G exp(-tau ln|Z| / beta)."""
+        deta = np.log(abs(self.partition_function_from_ppgf(G))) / self.beta
+        self.eta += deta
+        tau = self.problem.grid.tau
+        return [g * np.exp(-tau * deta)[:, None, None] for g in G]
+
+    # ------------------------------------------------------------------ the self-consistency loop
+    def solve(self, max_order, tol=1e-4, maxiter=10, mix=1.0, hyb_tol=None, hyb_comp=True, normalization="classic",
+              spgf_max_order=None, verbose=False):
+        if normalization not in self.NORMALIZATIONS:
+            raise NotImplementedError(f"Normalization method {normalization} is not available on the device path "
+                                      f"(available: {', '.join(self.NORMALIZATIONS)})")
+        self.normalization = normalization
+        self.max_order = max_order
+        self.spgf_max_order = spgf_max_order if spgf_max_order is not None else max_order
+        self.hyb_tol = hyb_tol if hyb_tol is not None else 0.1 * tol
+        self.hyb_comp = hyb_comp if max_order > 1 else False
+        if self.hyb_comp:
+            self.fit_hybridization(tol=self.hyb_tol, compression=True, verbose=verbose)
         G, eta, it, diff = self.d.ppsc_solve(max_order, tol=tol, maxiter=maxiter, mix=mix, eta0=self.eta)
         self.G, self.eta, self.n_ppsc_iter, self.diff_G = G, eta, it, diff
+        self.history = self.d.ppsc_history()
+        if verbose:
+            for k, (d, Z, e) in enumerate(self.history, 1):
+                print(f"iter = {k}, diff_G = {d:2.2E}, Z-1 = {Z - 1:+2.2E}, eta = {e:2.2E}")
+            if diff < tol:
+                print(f"Converged after {it} iterations with diff_G = {diff:2.2E} < tol = {tol:2.2E}")
+        self.G_tau = self.single_particle_greens_function(self.spgf_max_order)
         return self
 
+    # ------------------------------------------------------------------ observables
     def single_particle_greens_function(self, max_order):
-        """g(tau) from the current G (block_sparse_solver.py:653-665): (r, n, n) at the DLR nodes."""
+        """g(tau) from the current G (block_sparse_solver.py:653-677): (r, n, n) at the DLR nodes."""
         self.spgf = self.d.single_particle_greens_function(self.G, max_order)
         return self.spgf
 
+    def pseudo_particle_chemical_potential(self):
+        return self.eta
+
     def partition_function(self):
-        """Z = -Tr G(beta) exp(-beta eta)."""
-        from .dlr import kernel
+        """Z = -Tr G(beta) exp(-beta eta) (block_sparse_solver.py:490-502)."""
+        return self.partition_function_from_ppgf(self.G) * np.exp(-self.beta * self.eta)
+
+    def many_body_density_matrix(self, G=None):
+        """rho_b = -G_b(beta) per block (block_sparse_solver.py:541-557)."""
         grid = self.problem.grid
         kb = kernel(np.ones(1), grid.rf[None, :])[0]
-        tr = sum(np.trace(np.tensordot(kb, grid.vals2coefs(g), axes=(0, 0))) for g in self.G)
-        return -tr * np.exp(-self.beta * self.eta)
+        return [-np.tensordot(kb, grid.vals2coefs(g), axes=(0, 0)) for g in (self.G if G is None else G)]
+
+    def expectation_value(self, operator):
+        """<O> = Tr[O rho] (block_sparse_solver.py:518-538) for O a Fock-basis matrix (dim, dim) that is
+        block diagonal in the problem's blocks."""
+        O = operator.toarray() if hasattr(operator, "toarray") else np.asarray(operator)
+        ad = self.problem.ad
+        return float(sum(np.trace(O[np.ix_(f, f)] @ rho) for f, rho in zip(ad.fock_states, self.many_body_density_matrix())))
