@@ -144,6 +144,17 @@ int xca_dyson_init(xca_ctx *ctx, const double *G0);
  * accumulator. G_out: concatenated blocks (host). */
 int xca_dyson_solve(xca_ctx *ctx, const double *sigma, double eta, double *G_out);
 
+/* dyson_it_ppsc::solve for every block, as xca_dyson_solve, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): it allocates nothing. No counterpart
+ * in the reference. G0 is passed here instead of through xca_dyson_init (the G0 coefficients are
+ * formed on the host); every device buffer the Dyson kernels use (descriptors, DLR factors, the
+ * block systems, right-hand sides, pivots, staging) lives in one static device array of 32 MiB per
+ * GPU, held until the process ends. A problem needing more is rejected ("static maximum"); 3-orbital
+ * problems with systems up to r b = 528 need about 11 MiB. The array keeps the last call's contents
+ * (nothing in it must start at zero); calls that overlap on one GPU, from several threads or
+ * streams, race. */
+int xca_dyson_solve_noalloc(xca_ctx *ctx, const double *G0, const double *sigma, double eta, double *G_out);
+
 /* BlockSparseSolver.solve(max_order, tol, maxiter, mix) with classic normalization
  * (python/triqs_xca/block_sparse_solver.py:240-355, 427-449): Sigma over orders 1..max_order -> Dyson
  * -> Z = -Tr G(beta), eta += ln|Z|/beta, G_new *= exp(-tau deta) -> stop when max|G_new - G| < tol,

@@ -55,6 +55,7 @@ EXPORTS = {
     "xca_last_tile": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "xca_dyson_init": (C.c_int, [C.c_void_p, d_p]),
     "xca_dyson_solve": (C.c_int, [C.c_void_p, d_p, C.c_double, d_p]),
+    "xca_dyson_solve_noalloc": (C.c_int, [C.c_void_p, d_p, d_p, C.c_double, d_p]),
     "xca_ppsc_solve": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_int, C.c_double, d_p, d_p, C.POINTER(C.c_int), d_p]),
     "xca_ppsc_history": (C.c_int, [C.c_void_p, d_p, C.c_int]),
     "xca_num_correlator_backbones": (C.c_int64, [C.c_void_p, C.c_int]),
