@@ -102,3 +102,26 @@ def test_ppsc_three_orbital_matches_oracle():
     assert S.n_ppsc_iter == it_o
     assert abs(S.eta - eta_o) <= 1e-12 * max(1.0, abs(eta_o))
     assert e <= 1e-12
+
+
+@pytest.mark.gpu
+def test_solver_archive_roundtrip_after_solve(tmp_path):
+    """test/python/h5_io_bs.py:30-52 on the device path: an NCA solve (one iteration), the solver
+    written to an HDF5 archive and read back compares equal, and the reloaded solver's evaluator
+    (created afresh on the device) gives the same Sigma and Green's function."""
+    from xca_amd import h5
+    from xca_amd.solver import BlockSparseSolver
+    P = models.one_spinful_fermion()
+    S = BlockSparseSolver(P).solve(1, tol=1e-9, maxiter=1, hyb_comp=False)
+    f = tmp_path / "data_h5_io_bs.h5"
+    with h5.HDFArchive(f, "w") as A:
+        A["S"] = S
+    with h5.HDFArchive(f, "r") as A:
+        R = A["S"]
+    assert R == S
+    # two device contexts on the same inputs; the bar allows for a different summation order
+    a = S.eval_pseudo_particle_self_energy(S.G, 1)
+    b = R.eval_pseudo_particle_self_energy(R.G, 1)
+    assert max(np.abs(x - y).max() for x, y in zip(a, b)) <= 1e-14 * max(np.abs(x).max() for x in a)
+    g, h = S.single_particle_greens_function(1), R.single_particle_greens_function(1)
+    assert np.abs(g - h).max() <= 1e-14 * np.abs(g).max()

@@ -204,9 +204,10 @@ class AtomDiagArrays:
     atom_diag (c++/triqs_xca/atom_diag_utils.cpp:39-185) and stands wherever an AtomDiag is accepted
     for evaluation."""
 
-    def __init__(self, n, dims, c_conn, cdag_conn, c_blocks, cdag_blocks):
+    def __init__(self, n, dims, c_conn, cdag_conn, c_blocks, cdag_blocks, fock_states=None):
         self.fs = _Flavours(int(n))
         self.dims = np.asarray(dims, dtype=np.int32)
+        self.fock_states = None if fock_states is None else [np.asarray(f, dtype=np.int64) for f in fock_states]
         self.nblocks = len(self.dims)
         self.c_conn = np.asarray(c_conn, dtype=np.int32).reshape(n, self.nblocks)
         self.cdag_conn = np.asarray(cdag_conn, dtype=np.int32).reshape(n, self.nblocks)
@@ -226,9 +227,34 @@ class AtomDiagArrays:
 
     packed_ops = AtomDiag.packed_ops
 
+    def __reduce_to_dict__(self):
+        d = {"n": self.fs.n, "dims": self.dims, "c_conn": self.c_conn, "cdag_conn": self.cdag_conn,
+             "c_blocks": {f"{o},{j}": m for (o, j), m in self.c_blocks.items()},
+             "cdag_blocks": {f"{o},{j}": m for (o, j), m in self.cdag_blocks.items()}}
+        if self.fock_states is not None:
+            d["fock_states"] = list(self.fock_states)
+        return d
+
+    @classmethod
+    def __factory_from_dict__(cls, name, d):
+        key = lambda k: tuple(int(x) for x in k.split(","))
+        return cls(int(d["n"]), d["dims"], d["c_conn"], d["cdag_conn"], {key(k): np.asarray(m) for k, m in d["c_blocks"].items()},
+                   {key(k): np.asarray(m) for k, m in d["cdag_blocks"].items()}, d.get("fock_states"))
+
+    def __eq__(self, other):
+        if not isinstance(other, AtomDiagArrays):
+            return False
+        same = lambda x, y: x.keys() == y.keys() and all(np.array_equal(x[k], y[k]) for k in x)
+        fa, fb = self.fock_states, other.fock_states
+        return (self.fs.n == other.fs.n and np.array_equal(self.dims, other.dims) and np.array_equal(self.c_conn, other.c_conn)
+                and np.array_equal(self.cdag_conn, other.cdag_conn) and same(self.c_blocks, other.c_blocks)
+                and same(self.cdag_blocks, other.cdag_blocks)
+                and ((fa is None and fb is None) or (fa is not None and fb is not None and len(fa) == len(fb)
+                                                    and all(np.array_equal(x, y) for x, y in zip(fa, fb)))))
+
     @classmethod
     def from_atom_diag(cls, ad):
-        return cls(ad.fs.n, ad.dims, ad.c_conn, ad.cdag_conn, ad.c_blocks, ad.cdag_blocks)
+        return cls(ad.fs.n, ad.dims, ad.c_conn, ad.cdag_conn, ad.c_blocks, ad.cdag_blocks, getattr(ad, "fock_states", None))
 
     @classmethod
     def from_triqs(cls, ad):
@@ -250,7 +276,7 @@ class AtomDiagArrays:
                     cb[(o, j)] = C[fock[cc[o, j]], :][:, fock[j]].toarray()
                 if dc[o, j] >= 0:
                     db[(o, j)] = Cd[fock[dc[o, j]], :][:, fock[j]].toarray()
-        return cls(n, [len(f) for f in fock], cc, dc, cb, db)
+        return cls(n, [len(f) for f in fock], cc, dc, cb, db, fock)
 
 
 # --------------------------------------------------------------------------- Hamiltonians

@@ -63,6 +63,18 @@ class DLRGrid:
         g._set_nodes(rf, it)
         return g
 
+    def __reduce_to_dict__(self):
+        return {"beta": self.beta, "lam": self.lam, "eps": self.eps, "it": self.it, "rf": self.rf}
+
+    @classmethod
+    def __factory_from_dict__(cls, name, d):
+        return cls.from_nodes(d["beta"], d["it"], d["rf"], lam=d["lam"], eps=d["eps"])
+
+    def __eq__(self, other):
+        return (isinstance(other, DLRGrid) and (self.beta, self.lam) == (other.beta, other.lam)
+                and (self.eps == other.eps or (self.eps != self.eps and other.eps != other.eps))
+                and np.array_equal(self.it, other.it) and np.array_equal(self.rf, other.rf))
+
     def vals2coefs(self, g):
         g = np.asarray(g)
         shp = g.shape

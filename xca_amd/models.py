@@ -6,7 +6,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import scipy.sparse as sp
 
-from .atom import AtomDiag, FockSpace, dlr_hybridization, kanamori_hamiltonian, kanamori_spins, semicircle_delta_tau
+from .atom import AtomDiag, AtomDiagArrays, FockSpace, dlr_hybridization, kanamori_hamiltonian, kanamori_spins, semicircle_delta_tau
 from .dlr import DLRGrid, kernel
 
 
@@ -43,6 +43,17 @@ class ImpurityProblem:
             out.append(flat[off:off + sz].reshape(r, d, d))
             off += sz
         return out
+
+    def __reduce_to_dict__(self):
+        """The problem as arrays: grid nodes, the atom's block-sparse operators (an AtomDiag is stored
+        in its AtomDiagArrays form, which is all an evaluator reads), hybridization, G0, eta0."""
+        ad = self.ad if isinstance(self.ad, AtomDiagArrays) else AtomDiagArrays.from_atom_diag(self.ad)
+        return {"grid": self.grid, "ad": ad, "poles": np.asarray(self.poles), "coeffs": np.asarray(self.coeffs),
+                "G0": list(self.G0), "eta0": self.eta0, "name": self.name}
+
+    @classmethod
+    def __factory_from_dict__(cls, name, d):
+        return cls(grid=d["grid"], ad=d["ad"], poles=d["poles"], coeffs=d["coeffs"], G0=list(d["G0"]), eta0=d["eta0"], name=d["name"])
 
     def perturbed_G(self, seed=20260101, amp=1e-3):
         """Atomic propagator with DLR coefficients scaled by (1 + u), u ~ U[-amp, amp] iid per

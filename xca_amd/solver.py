@@ -9,12 +9,36 @@ defaults to 0.1 tol, the hybridization is compressed unless hyb_comp is False or
 does not use the pole representation), and the single-particle Green's function is evaluated at the
 end to spgf_max_order. The normalisation is the classic one (block_sparse_solver.py:427-449); the
 root and ODE variants of the reference are not on the device path.
+
+A solver is stored in and read back from an HDF5 archive like the reference's
+(block_sparse_solver.py:1174-1204, 1441-1443): xca_amd.h5.HDFArchive, A["S"] = S, S2 = A["S"],
+S == S2. The device evaluator is not stored; it is created again on first use.
 """
 import numpy as np
 
-from . import hybfit
-from .dlr import kernel
+from . import h5, hybfit
+from .atom import AtomDiagArrays
+from .dlr import DLRGrid, kernel
 from .evaluator import DiagramEvaluator
+from .models import ImpurityProblem
+
+
+def _reduced(x):
+    return x.__reduce_to_dict__() if hasattr(x, "__reduce_to_dict__") else x
+
+
+def _same(a, b):
+    """Structural equality of what the archive stores: dicts, lists, arrays, scalars and objects that
+    reduce to dicts; arrays must be equal element for element (NaN equal to NaN)."""
+    a, b = _reduced(a), _reduced(b)
+    if isinstance(a, dict) or isinstance(b, dict):
+        return isinstance(a, dict) and isinstance(b, dict) and a.keys() == b.keys() and all(_same(a[k], b[k]) for k in a)
+    if isinstance(a, (list, tuple)) or isinstance(b, (list, tuple)):
+        return (isinstance(a, (list, tuple)) and isinstance(b, (list, tuple)) and len(a) == len(b)
+                and all(_same(x, y) for x, y in zip(a, b)))
+    if a is None or b is None or isinstance(a, str) or isinstance(b, str):
+        return a == b
+    return np.array_equal(np.asarray(a), np.asarray(b), equal_nan=True)
 
 
 class Hybridization:
@@ -25,6 +49,16 @@ class Hybridization:
         self.poles = np.asarray(poles, dtype=np.float64)
         self.coefficients = np.asarray(coefficients)
         self.tol, self.compression, self.fit_error = tol, compression, fit_error
+
+    def __reduce_to_dict__(self):
+        return dict(self.__dict__)
+
+    @classmethod
+    def __factory_from_dict__(cls, name, d):
+        return cls(d["poles"], d["coefficients"], d["tol"], bool(d["compression"]), d["fit_error"])
+
+    def __eq__(self, other):
+        return isinstance(other, Hybridization) and _same(self, other)
 
 
 class BlockSparseSolver:
@@ -41,6 +75,38 @@ class BlockSparseSolver:
         self.history = []
         self.set_hybridization_poles_and_coefficients(problem.poles, problem.coeffs)
 
+    @property
+    def d(self):
+        """The device evaluator on the current hybridization, created on first use."""
+        if self._d is None:
+            self._d = DiagramEvaluator(self.hyb.poles, self.hyb.coefficients, self.problem.grid, self.problem.ad, device=self.device)
+            self._d.dyson_init(self.G0)
+        return self._d
+
+    # ------------------------------------------------------------------ persistence
+    _SKIP = ("_d",)
+
+    def __reduce_to_dict__(self):
+        """Everything but the device evaluator (block_sparse_solver.py:1174-1182 likewise skips its
+        evaluator and Dyson solvers); the history as an (iterations, 3) array."""
+        d = {k: v for k, v in self.__dict__.items() if k not in self._SKIP}
+        d["history"] = np.asarray(self.history, dtype=np.float64).reshape(-1, 3)
+        return d
+
+    @classmethod
+    def __factory_from_dict__(cls, name, d):
+        """block_sparse_solver.py:1194-1204: construct from the stored problem, then restore the rest."""
+        ret = cls(d["problem"], device=int(d["device"]))
+        ret.__dict__.update({k: v for k, v in d.items() if k != "history"})
+        ret.history = [tuple(float(x) for x in row) for row in np.asarray(d["history"]).reshape(-1, 3)]
+        ret.G0 = list(ret.G0)
+        ret.G = list(ret.G)
+        ret._d = None
+        return ret
+
+    def __eq__(self, other):
+        return isinstance(other, BlockSparseSolver) and _same(self.__reduce_to_dict__(), other.__reduce_to_dict__())
+
     # ------------------------------------------------------------------ hybridization
     def delta_tau(self):
         """Delta at the DLR nodes (r, n, n) of the problem's hybridization."""
@@ -49,10 +115,9 @@ class BlockSparseSolver:
         return np.einsum("tl,lab->tab", K, P.coeffs)
 
     def set_hybridization_poles_and_coefficients(self, poles, coefficients):
-        """block_sparse_solver.py:220-238: a new evaluator on these poles."""
+        """block_sparse_solver.py:220-238: a new evaluator on these poles (created at its first use)."""
         self.hyb = Hybridization(poles, coefficients)
-        self.d = DiagramEvaluator(self.hyb.poles, self.hyb.coefficients, self.problem.grid, self.problem.ad, device=self.device)
-        self.d.dyson_init(self.G0)
+        self._d = None
 
     def fit_hybridization(self, tol=None, compression=False, verbose=False):
         """block_sparse_solver.py:146-185: with compression a pole fit to relative error tol, else (or
@@ -134,3 +199,7 @@ class BlockSparseSolver:
         O = operator.toarray() if hasattr(operator, "toarray") else np.asarray(operator)
         ad = self.problem.ad
         return float(sum(np.trace(O[np.ix_(f, f)] @ rho) for f, rho in zip(ad.fock_states, self.many_body_density_matrix())))
+
+
+for _cls in (BlockSparseSolver, Hybridization, ImpurityProblem, DLRGrid, AtomDiagArrays):
+    h5.register_class(_cls)
