@@ -262,7 +262,10 @@ def ppsc_solve(prob, max_order, tol=1e-10, maxiter=100, mix=1.0, complex_arith=F
     pool = ThreadPoolExecutor(max_workers=threads) if threads > 1 else None
     orc = OracleEvaluator(prob, complex_arith=complex_arith, extended=extended)
     grid = prob.grid
-    G = [np.array(g, dtype=float) for g in prob.G0]
+    # complex_arith keeps complex blocks (hermitisation with the conjugate transpose, Z = -Re Tr G(beta),
+    # block_sparse_solver.py:482-487, 586); otherwise the imaginary parts are dropped
+    part = (lambda x: x) if complex_arith else (lambda x: x.real)
+    G = [np.array(g, dtype=complex if complex_arith else float) for g in prob.G0]
     eta = 0.0
     topos = [(np.array(t, dtype=np.int32), (-1) ** o * s * (-1) ** (o + 1)) for o in range(1, max_order + 1)
              for s, t in all_connected_pairings(o)]
@@ -271,11 +274,11 @@ def ppsc_solve(prob, max_order, tol=1e-10, maxiter=100, mix=1.0, complex_arith=F
     for it in range(1, maxiter + 1):
         S = None
         for t, fac in topos:
-            s = [x.real for x in orc.sigma(G, t)]
+            s = [part(x) for x in orc.sigma(G, t)]
             S = [fac * x for x in s] if S is None else [a + fac * x for a, x in zip(S, s)]
         def solve(a):
-            g = dyson(prob.beta, grid.it, grid.rf, a[0], a[1], eta, extended=extended).real
-            return 0.5 * (g + g.transpose(0, 2, 1))
+            g = part(dyson(prob.beta, grid.it, grid.rf, a[0], a[1], eta, extended=extended))
+            return 0.5 * (g + g.transpose(0, 2, 1).conj())
         pairs = list(zip(prob.G0, S))
         Gn = list(pool.map(solve, pairs)) if pool else [solve(a) for a in pairs]
         tr = 0.0

@@ -99,7 +99,7 @@ def auto_partition(fs, H):
 class AtomDiag:
     """Block-diagonalised local Hamiltonian with block-sparse c and c^dagger.
 
-    H: (dim, dim) real matrix (dense or sparse) in the Fock basis. qn: "NSz" (spin[o] = +1/-1 gives
+    H: (dim, dim) Hermitian matrix, real or complex (dense or sparse), in the Fock basis. qn: "NSz" (spin[o] = +1/-1 gives
     S_z), "N", "none" (one block), or "auto" (the finest partition compatible with H and the
     operators, auto_partition).
     """
@@ -174,19 +174,19 @@ class AtomDiag:
         return np.concatenate(c) if c else np.zeros(0), np.concatenate(d) if d else np.zeros(0)
 
     def atomic_G0(self, beta, tau):
-        """G0_b(tau) = -U exp(-tau (E - eta0)) U^T and eta0 = -ln Z0 / beta with ground-state-shifted
+        """G0_b(tau) = -U exp(-tau (E - eta0)) U^dagger and eta0 = -ln Z0 / beta with ground-state-shifted
         energies (block_sparse_solver.py:1305-1328). tau: absolute times (r,). Returns (blocks, eta0)."""
         Z0 = sum(np.exp(-beta * E).sum() for E in self.energies)
         eta0 = -np.log(Z0) / beta
         G = []
         for E, U in zip(self.energies, self.eigvecs):
             d = -np.exp(-tau[:, None] * (E - eta0)[None, :])
-            G.append(np.einsum("ia,ta,ja->tij", U, d, U))
+            G.append(np.einsum("ia,ta,ja->tij", U, d, U.conj()))
         return G, eta0
 
     def dense_H(self):
         dim = self.fs.dim
-        Hd = np.zeros((dim, dim))
+        Hd = np.zeros((dim, dim), dtype=np.result_type(*self.H_blocks))
         for f, hb in zip(self.fock_states, self.H_blocks):
             Hd[np.ix_(f, f)] = hb
         return Hd

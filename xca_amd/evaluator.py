@@ -3,15 +3,20 @@
 Mirrors triqs_xca.block_sparse.DiagramEvaluator (python bindings block_sparse.wrap.cxx:173-208, C++
 c++/triqs_xca/block_sparse_backbone.hpp:23-165): compute_self_energy(G, topology[, f_ix]),
 get_num_self_energy_backbones(topology), reset(). G and the result are lists of (r, b, b) blocks (the
-data of the reference's BlockGf). Inputs must be real (imaginary parts exactly zero); the device path
-computes in real FP64 and raises otherwise.
+data of the reference's BlockGf). The device computes in real FP64. A problem with complex
+coefficients, operators or propagators (complex_arith, detected from the inputs unless given) runs
+on the device as its real embedding (xca_amd.cplx: blocks of twice the dimension, a second copy of
+every flavour carrying the factor i), and its results are complex128; in real mode an input with a
+nonzero imaginary part raises.
 """
 import ctypes as C
 
 import numpy as np
 
+from . import cplx
 from ._lib import XcaError, XcaProblem, as_f64, as_i32, check, dptr, i64_p, iptr, last_error, lib
 from .diag import all_connected_pairings
+from .dlr import kernel
 
 
 def _real(a, what):
@@ -44,13 +49,20 @@ class DiagramEvaluator:
     of an xca_amd.atom.AtomDiag.
     """
 
-    def __init__(self, hyb_poles, hyb_coeffs, grid, ad, device=0):
+    def __init__(self, hyb_poles, hyb_coeffs, grid, ad, device=0, complex_arith=None):
         self.grid = grid
         self.ad = ad
         self.beta = grid.beta
         self.r = grid.r
         self.n = ad.fs.n
-        self.dims = np.asarray(ad.dims, dtype=np.int32)
+        self.block_dims = np.asarray(ad.dims, dtype=np.int32)
+        if complex_arith is None:
+            complex_arith = cplx.is_complex(hyb_coeffs, *ad.packed_ops())
+        self.complex_arith = bool(complex_arith)
+        if self.complex_arith:
+            ad = cplx.embedded_atom(ad)
+            hyb_coeffs = cplx.embedded_coeffs(hyb_coeffs)
+        self.dims = np.asarray(ad.dims, dtype=np.int32)   # block dimensions on the device
         self._keep = []
 
         def keep(a):
@@ -74,32 +86,40 @@ class DiagramEvaluator:
 
     # ---------------------------------------------------------------- helpers
     def pack(self, G):
-        if len(G) != len(self.dims):
-            raise XcaError(f"expected {len(self.dims)} blocks, got {len(G)}")
+        if len(G) != len(self.block_dims):
+            raise XcaError(f"expected {len(self.block_dims)} blocks, got {len(G)}")
         out = []
-        for b, g in zip(self.dims, G):
-            g = _real(g, "G")
+        for b, g in zip(self.block_dims, G):
+            g = np.asarray(g) if self.complex_arith else _real(g, "G")
             if g.shape != (self.r, b, b):
                 raise XcaError(f"G block has shape {g.shape}, expected {(self.r, b, b)}")
-            out.append(g.ravel())
+            out.append((cplx.embed(g) if self.complex_arith else g).ravel())
         return np.ascontiguousarray(np.concatenate(out))
 
     def unpack(self, flat):
         out, off = [], 0
         for b in self.dims:
             sz = self.r * b * b
-            out.append(flat[off:off + sz].reshape(self.r, b, b).copy())
+            blk = flat[off:off + sz].reshape(self.r, b, b).copy()
+            out.append(cplx.extract(blk) if self.complex_arith else blk)
             off += sz
         return out
 
+    def _real_only(self, what):
+        if self.complex_arith:
+            raise XcaError(f"{what} is available for real problems; this evaluator runs a complex problem as its real embedding")
+
     # ---------------------------------------------------------------- reference API
     def get_num_self_energy_backbones(self, topology):
+        self._real_only("get_num_self_energy_backbones")
         return int(lib().xca_num_backbones(self._h, len(topology)))
 
     def compute_self_energy(self, G, topology, f_ix=None):
         g = self.pack(G)
         topo = as_i32(topology)
         out = np.zeros_like(g)
+        if f_ix is not None:
+            self._real_only("compute_self_energy with flat indices (f_ix)")
         if f_ix is None:
             st = lib().xca_sigma(self._h, dptr(g), topo.shape[0], iptr(topo), None, 0, dptr(out))
         else:
@@ -114,6 +134,7 @@ class DiagramEvaluator:
     # ---------------------------------------------------------------- correlators
     def get_num_single_ptcle_gf_backbones(self, topology):
         """2^(m-1) n^(m-1) p^(m-1) (block_sparse_backbone.cpp:732-735)."""
+        self._real_only("get_num_single_ptcle_gf_backbones")
         return int(lib().xca_num_correlator_backbones(self._h, len(topology)))
 
     def _fix(self, f_ix):
@@ -129,9 +150,14 @@ class DiagramEvaluator:
         G None uses the G already on the device (set_G or the previous call)."""
         g = None if G is None else dptr(self.pack(G))
         topo = as_i32(topology)
+        if f_ix is not None:
+            self._real_only("compute_single_ptcle_gf with flat indices (f_ix)")
         fx, nf = self._fix(f_ix)
-        out = np.zeros((self.r, self.n, self.n))
+        m = 2 * self.n if self.complex_arith else self.n
+        out = np.zeros((self.r, m, m))
         check(lib().xca_spgf(self._h, g, topo.shape[0], iptr(topo), fx, nf, dptr(out)))
+        if self.complex_arith:  # traces of the embedding: 2 Re X, and -2 Im X from the copies with i
+            return cplx.trace_from_copies(out, self.n)[:, :, :self.n]
         return out
 
     def compute_one_time_correlator(self, G, ops_tau, ops_0, topology, f_ix=None):
@@ -140,20 +166,33 @@ class DiagramEvaluator:
         packed matrices in (operator, source block) order): (r, len(ops_tau), len(ops_0)) real."""
         g = None if G is None else dptr(self.pack(G))
         topo = as_i32(topology)
+        if f_ix is not None:
+            self._real_only("compute_one_time_correlator with flat indices (f_ix)")
         fx, nf = self._fix(f_ix)
-        mc, mm = as_i32(ops_tau[0]), _real(ops_tau[1], "ops_tau")
-        kc, km = as_i32(ops_0[0]), _real(ops_0[1], "ops_0")
+        mc, kc = as_i32(ops_tau[0]), as_i32(ops_0[0])
         if mc.ndim != 2 or kc.ndim != 2 or mc.shape[1] != len(self.dims) or kc.shape[1] != len(self.dims):
             raise XcaError("operator connections must have shape (nops, nblocks)")
+        nmu = mc.shape[0]
+        if self.complex_arith:
+            try:
+                mc, mm = cplx.embedded_ops(mc, ops_tau[1], self.block_dims)
+                kc, km = cplx.embedded_ops(kc, ops_0[1], self.block_dims)
+            except ValueError as e:
+                raise XcaError(str(e)) from None
+            kc, km = kc[:kc.shape[0] // 2], km[:km.size // 2]   # the factor i on one side suffices
+            mc, mm, kc, km = as_i32(mc), as_f64(mm), as_i32(kc), as_f64(km)
+        else:
+            mm, km = _real(ops_tau[1], "ops_tau"), _real(ops_0[1], "ops_0")
+        self._keep_ops = (mc, mm, kc, km)
         out = np.zeros((self.r, mc.shape[0], kc.shape[0]))
         check(lib().xca_correlator(self._h, g, topo.shape[0], iptr(topo), fx, nf, mc.shape[0], iptr(mc), dptr(mm), kc.shape[0],
                                    iptr(kc), dptr(km), dptr(out)))
-        return out
+        return cplx.trace_from_copies(out, nmu) if self.complex_arith else out
 
     def single_particle_greens_function(self, G, max_order):
         """eval_single_particle_greens_function (block_sparse_solver.py:657-677): the sum over orders
         1..max_order and connected topologies of (-1)^order * s_topo * compute_single_ptcle_gf."""
-        tot = np.zeros((self.r, self.n, self.n))
+        tot = np.zeros((self.r, self.n, self.n), dtype=np.complex128 if self.complex_arith else np.float64)
         self.set_G(G)
         for order in range(1, max_order + 1):
             for sign, topo in all_connected_pairings(order):
@@ -181,7 +220,7 @@ class DiagramEvaluator:
         check(lib().xca_sigma_accumulate(self._h, topo.shape[0], iptr(topo), float(scale)))
 
     def get_sigma(self):
-        out = np.zeros(sum(self.r * b * b for b in self.dims))
+        out = np.zeros(sum(self.r * int(b) * int(b) for b in self.dims))
         check(lib().xca_get_sigma(self._h, dptr(out)))
         return self.unpack(out)
 
@@ -239,7 +278,9 @@ class DiagramEvaluator:
 
     def ppsc_solve(self, max_order, tol=1e-10, maxiter=100, mix=1.0, eta0=0.0):
         """Self-consistent pseudo-particle G; returns (G blocks, eta, iterations, last max|G_new - G|)."""
-        out = np.zeros(sum(self.r * b * b for b in self.dims))
+        if self.complex_arith:
+            return self._ppsc_solve_complex(max_order, tol, maxiter, mix, eta0)
+        out = np.zeros(sum(self.r * int(b) * int(b) for b in self.dims))
         eta = C.c_double(0)
         it = C.c_int(0)
         diff = C.c_double(0)
@@ -247,8 +288,38 @@ class DiagramEvaluator:
                                    C.byref(it), C.byref(diff)))
         return self.unpack(out), eta.value, it.value, diff.value
 
+    def _ppsc_solve_complex(self, max_order, tol, maxiter, mix, eta0):
+        """The loop of BlockSparseSolver.solve with classic normalisation (block_sparse_solver.py:
+        289-346, 427-449, 482-487) for a complex problem: Sigma and the Dyson solve (with its
+        hermitisation) on the device, in the embedding; Z = -Re Tr G_new(beta), the energy shift, the
+        convergence test on max |G_new - G| of the complex blocks and the mixing on the host (O(r sum b^2)
+        work per iteration)."""
+        if not hasattr(self, "_G0"):
+            raise XcaError("ppsc_solve: dyson_init(G0) first")
+        grid = self.grid
+        kb = kernel(np.ones(1), grid.rf[None, :])[0]   # K(tau = beta, w_k)
+        G = self.unpack(self._G0)
+        eta = float(eta0)
+        hist = []
+        it, diff = 0, np.inf
+        for it in range(1, maxiter + 1):
+            G_new = self.dyson_solve(self.pseudo_particle_self_energy(G, max_order), eta)
+            Z = float(-sum(np.trace(np.tensordot(kb, grid.vals2coefs(g), axes=(0, 0))) for g in G_new).real)
+            deta = np.log(abs(Z)) / self.beta
+            G_new = [g * np.exp(-grid.tau * deta)[:, None, None] for g in G_new]
+            eta += deta
+            diff = max(float(np.abs(a - b).max()) for a, b in zip(G_new, G))
+            hist.append((diff, Z, eta))
+            if diff < tol:
+                break
+            G = [mix * a + (1.0 - mix) * b for a, b in zip(G_new, G)]
+        self._history = hist
+        return G, eta, it, diff
+
     def ppsc_history(self):
         """Per-iteration report of the last ppsc_solve: [(diff_G, Z before the normalisation, eta)]."""
+        if self.complex_arith:
+            return list(getattr(self, "_history", []))
         n = check(lib().xca_ppsc_history(self._h, None, 0))
         h = np.zeros(3 * max(n, 1))
         check(lib().xca_ppsc_history(self._h, dptr(h), n))

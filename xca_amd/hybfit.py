@@ -9,8 +9,10 @@ aaa.py:22-106). The method is the published one those follow:
 
 1. AAA rational approximation (Nakatsukasa, Sete, Trefethen, SIAM J. Sci. Comput. 40, A1494 (2018))
    of the matrix-valued Delta(i nu) on fermionic Matsubara frequencies, with support points in
-   conjugate pairs and weights constrained to conjugate pairs, so that the rational function has
-   real coefficients; its real poles are the candidate pole energies.
+   conjugate pairs and weights constrained to conjugate pairs, so that the rational function
+   satisfies Delta(conj z) = Delta(z)^dagger (real coefficients for a real symmetric Delta); its real
+   poles are the candidate pole energies. A complex (Hermitian) Delta is fitted entry by entry with
+   entry (a, b) mirrored by (b, a), and gets complex residues.
 2. Residues by linear least squares against the DLR representation of Delta(tau) on a dense time
    grid (clustered geometrically at both ends, where the DLR basis varies on the scale 1/Lambda).
 3. Poles whose residues are negligible are dropped, and the remaining poles are refined by
@@ -54,13 +56,15 @@ def matsubara_values(w, coeffs, beta, k):
     return np.einsum("kl,lab->kab", 1.0 / den, coeffs)
 
 
-def aaa_real_poles(z, F, degree):
+def aaa_real_poles(z, F, degree, mirror=None):
     """Real poles of the AAA approximant of degree `degree` (even) of the matrix-valued samples
     F (K, e) at points z (K,) on the upper imaginary axis, mirrored to the lower half plane with
-    F(conj z) = conj F(z). Support points and barycentric weights come in conjugate pairs."""
+    F(conj z) = conj F(z) entry by entry, or, with mirror (a permutation of the e entries),
+    F_e(conj z) = conj F_mirror[e](z) (a Hermitian hybridization: entry (a, b) mirrors (b, a)).
+    Support points and barycentric weights come in conjugate pairs."""
     K, e = F.shape
     Z = np.concatenate([z, np.conj(z)])
-    FF = np.concatenate([F, np.conj(F)], axis=0)
+    FF = np.concatenate([F, np.conj(F if mirror is None else F[:, mirror])], axis=0)
     N = 2 * K
     support = []
     fit = np.full_like(FF, FF.mean())
@@ -111,17 +115,26 @@ def residues(t, x, target):
 
 
 def compress(w_dlr, coeffs, beta, tol, max_poles=None, lam=None, verbose=False):
-    """Poles (dimensionless, beta * energy) and real residues (p, n, n) of a hybridization given by
-    its DLR expansion (w_dlr dimensionless, coeffs (r, n, n) real) with relative fit error below
-    tol, or the best fit found. Returns (x, W, error)."""
-    coeffs = np.asarray(coeffs, dtype=np.float64)
+    """Poles (dimensionless, beta * energy) and residues (p, n, n) of a hybridization given by its
+    DLR expansion (w_dlr dimensionless, coeffs (r, n, n)) with relative fit error below tol, or the
+    best fit found. Real coefficients: Delta symmetric, real residues, the upper triangle fitted.
+    Complex coefficients: Delta Hermitian, complex residues, every nonzero entry fitted, the mirror
+    pairing (a, b) with (b, a). Returns (x, W, error)."""
+    coeffs = np.asarray(coeffs)
+    cx = np.iscomplexobj(coeffs) and bool(np.any(coeffs.imag != 0))
+    coeffs = coeffs.astype(np.complex128 if cx else np.float64)
     r, n, _ = coeffs.shape
     lam = lam or float(np.abs(w_dlr).max())
     t = dense_grid(lam)
-    # the independent nonzero entries (Delta is symmetric)
-    iu = [(a, b) for a in range(n) for b in range(a, n) if np.abs(coeffs[:, a, b]).max() > 0]
+    if cx:
+        iu = [(a, b) for a in range(n) for b in range(n) if np.abs(coeffs[:, a, b]).max() > 0 or np.abs(coeffs[:, b, a]).max() > 0]
+        pos = {ab: k for k, ab in enumerate(iu)}
+        mirror = np.array([pos[(b, a)] for a, b in iu], dtype=np.int64)
+    else:  # the independent nonzero entries (Delta is symmetric)
+        iu = [(a, b) for a in range(n) for b in range(a, n) if np.abs(coeffs[:, a, b]).max() > 0]
+        mirror = None
     if not iu:
-        return np.zeros(0), np.zeros((0, n, n)), 0.0
+        return np.zeros(0), np.zeros((0, n, n), dtype=coeffs.dtype), 0.0
     C = np.stack([coeffs[:, a, b] for a, b in iu], axis=1)   # (r, e)
     target = kernel_rel(t, w_dlr) @ C                          # (k, e)
     scale = float(np.abs(target).max())
@@ -131,7 +144,7 @@ def compress(w_dlr, coeffs, beta, tol, max_poles=None, lam=None, verbose=False):
     max_poles = max_poles or r
     best = (np.inf, None, None)
     for degree in range(2, 2 * max_poles + 1, 2):
-        pol = aaa_real_poles(z, F, degree)
+        pol = aaa_real_poles(z, F, degree, mirror)
         pol = pol[np.abs(pol.imag) <= 1e-6 * max(1.0, np.abs(pol).max())].real
         x = np.sort(beta * pol)
         if len(x) == 0 or len(x) > max_poles:
@@ -143,7 +156,8 @@ def compress(w_dlr, coeffs, beta, tol, max_poles=None, lam=None, verbose=False):
             continue
 
         def fun(xx):
-            return residues(t, xx, target)[1].ravel() / scale
+            res = residues(t, xx, target)[1].ravel() / scale
+            return np.concatenate([res.real, res.imag]) if cx else res
 
         while True:  # refine; drop the poles whose residues fall below the tolerance, then refine again
             sol = least_squares(fun, x, method="lm", xtol=1e-15, ftol=1e-15, gtol=1e-15, max_nfev=200 * (len(x) + 1))
@@ -161,10 +175,11 @@ def compress(w_dlr, coeffs, beta, tol, max_poles=None, lam=None, verbose=False):
         if err <= tol:
             break
     err, x, W = best
-    Wf = np.zeros((len(x), n, n))
+    Wf = np.zeros((len(x), n, n), dtype=coeffs.dtype)
     for k, (a, b) in enumerate(iu):
         Wf[:, a, b] = W[:, k]
-        Wf[:, b, a] = W[:, k]
+        if not cx:
+            Wf[:, b, a] = W[:, k]
     return x, Wf, err
 
 
@@ -174,7 +189,9 @@ def fit_hybridization(grid, delta_vals, tol=None, compression=False, max_poles=N
     grid's nodes. With compression, a pole fit to relative error tol (tol required and positive); if
     it does not reach tol, or without compression, the DLR expansion (p = r, error = the grid's
     eps)."""
-    delta_vals = np.asarray(delta_vals, dtype=np.float64)
+    delta_vals = np.asarray(delta_vals)
+    if not (np.iscomplexobj(delta_vals) and np.any(delta_vals.imag != 0)):
+        delta_vals = np.asarray(delta_vals.real, dtype=np.float64)
     coeffs = grid.vals2coefs(delta_vals)
     if compression:
         if tol is None or not tol > 0:

@@ -133,6 +133,30 @@ def two_band_discrete_bath(beta=2.0, lam=40.0, eps=1e-6, U=2.0, J=0.2, s=0.5, le
     return _problem(grid, ad, jt[:, None, None] * M[None], "two_band_discrete_bath")
 
 
+def two_flavour_hopping(rotation=None, qn="auto", beta=10.0, w_max=10.0, eps=1e-12, eps1=-0.1, t=0.5 + 0.5j, U=1.0, V=0.2,
+                        h=((0.1, 0.1), (0.1, -0.1))):
+    """The fixture of test/python/unitary_invariance_bs.py:26-101: two flavours,
+    H = eps1 N + U n0 n1 + t c+_0 c_1 + conj(t) c+_1 c_0, hybridization V^2 times the free Green's
+    function of h, Delta(tau) = V^2 sum_k K(tau, w_k) u_k u_k^dagger (two exact poles at the
+    eigenvalues of h). With rotation U (2 x 2 unitary) the problem in the rotated basis: H with
+    c_a -> sum_b U_ab c_b and Delta -> U^dagger Delta U, so that its single-particle Green's function
+    g2 satisfies U g2 U^dagger = g1. qn "auto" (block-sparse) or "none" (dense)."""
+    grid = DLRGrid(beta, beta * w_max, eps)
+    fs = FockSpace(2)
+    Urot = np.eye(2) if rotation is None else np.asarray(rotation, dtype=np.complex128)
+    c = [fs.c(0).astype(np.complex128), fs.c(1).astype(np.complex128)]
+    ct = [sum(Urot[a, b] * c[b] for b in range(2)) for a in range(2)]
+    cdt = [x.conj().T for x in ct]
+    n0, n1 = cdt[0] @ ct[0], cdt[1] @ ct[1]
+    H = eps1 * (n0 + n1) + U * (n0 @ n1) + t * (cdt[0] @ ct[1]) + np.conj(t) * (cdt[1] @ ct[0])
+    ad = AtomDiag(fs, H, qn=qn)
+    w, u = np.linalg.eigh(np.asarray(h, dtype=np.float64))
+    coeffs = np.stack([V ** 2 * np.outer(u[:, k], u[:, k].conj()) for k in range(2)]).astype(np.complex128)
+    coeffs = np.einsum("ab,kbc,cd->kad", Urot.conj().T, coeffs, Urot)
+    G0, eta0 = ad.atomic_G0(grid.beta, grid.tau)
+    return ImpurityProblem(grid=grid, ad=ad, poles=w, coeffs=coeffs, G0=G0, eta0=eta0, name=f"two_flavour_hopping_{qn}")
+
+
 def anderson_nca(beta=10.0, lam=100.0, eps=1e-10, U=2.0, mu=1.0, t2=0.25):
     """Single-orbital Anderson impurity, H = U n_up n_do - mu (n_up + n_do), blocks by N, semicircular
     hybridization (half-bandwidth 1, t^2 = 0.25) expanded in DLR."""

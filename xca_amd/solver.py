@@ -16,7 +16,7 @@ S == S2. The device evaluator is not stored; it is created again on first use.
 """
 import numpy as np
 
-from . import h5, hybfit
+from . import cplx, h5, hybfit
 from .atom import AtomDiagArrays
 from .dlr import DLRGrid, kernel
 from .evaluator import DiagramEvaluator
@@ -71,6 +71,8 @@ class BlockSparseSolver:
         self.G0 = problem.G0
         self.eta0 = problem.eta0
         self.G = [g.copy() for g in problem.G0]
+        # complex128 problems run on the device as their real embedding (xca_amd.cplx)
+        self.complex_arith = cplx.is_complex(problem.G0, problem.coeffs, *problem.ad.packed_ops())
         self.eta = 0.0  # shift on top of the atomic G0 (block_sparse_solver.py:103)
         self.history = []
         self.set_hybridization_poles_and_coefficients(problem.poles, problem.coeffs)
@@ -79,7 +81,8 @@ class BlockSparseSolver:
     def d(self):
         """The device evaluator on the current hybridization, created on first use."""
         if self._d is None:
-            self._d = DiagramEvaluator(self.hyb.poles, self.hyb.coefficients, self.problem.grid, self.problem.ad, device=self.device)
+            self._d = DiagramEvaluator(self.hyb.poles, self.hyb.coefficients, self.problem.grid, self.problem.ad, device=self.device,
+                                       complex_arith=self.complex_arith)
             self._d.dyson_init(self.G0)
         return self._d
 
@@ -136,11 +139,11 @@ class BlockSparseSolver:
         return self.d.dyson_solve(Sigma, eta)
 
     def partition_function_from_ppgf(self, G):
-        """Z = -Tr G(beta) (block_sparse_solver.py:482-487), G evaluated at tau = beta from its DLR
+        """Z = -Re Tr G(beta) (block_sparse_solver.py:482-487), G evaluated at tau = beta from its DLR
         coefficients."""
         grid = self.problem.grid
         kb = kernel(np.ones(1), grid.rf[None, :])[0]
-        return float(-sum(np.trace(np.tensordot(kb, grid.vals2coefs(g), axes=(0, 0))) for g in G))
+        return float(np.real(-sum(np.trace(np.tensordot(kb, grid.vals2coefs(g), axes=(0, 0))) for g in G)))
 
     def normalize_pseudo_particle_gf(self, G):
         """Classic normalisation (block_sparse_solver.py:427-449): eta += ln|Z| / beta and
@@ -198,7 +201,7 @@ class BlockSparseSolver:
         block diagonal in the problem's blocks."""
         O = operator.toarray() if hasattr(operator, "toarray") else np.asarray(operator)
         ad = self.problem.ad
-        return float(sum(np.trace(O[np.ix_(f, f)] @ rho) for f, rho in zip(ad.fock_states, self.many_body_density_matrix())))
+        return float(np.real(sum(np.trace(O[np.ix_(f, f)] @ rho) for f, rho in zip(ad.fock_states, self.many_body_density_matrix()))))
 
 
 for _cls in (BlockSparseSolver, Hybridization, ImpurityProblem, DLRGrid, AtomDiagArrays):
