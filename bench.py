@@ -80,7 +80,7 @@ def cpu_baseline(P, G, max_order, budget_s, cores):
     for t in topos:
         nmax = orcs[0].num_backbones(len(t))
         perm = rng.permutation(nmax).astype(np.int32)
-        k, done, el = 16 * cores, 0, 0.0
+        k, done, el = cores, 0, 0.0
         while done < nmax:
             k = min(k, nmax - done)
             el += timed(t, perm[done:done + k])

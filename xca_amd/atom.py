@@ -34,6 +34,16 @@ class FockSpace:
     def c(self, o):
         return self._c[o]
 
+    def action(self, o, dagger):
+        """c_o (or c^dagger_o) as a map of basis states: (target state or -1, sign) per source state;
+        every column of these operators holds at most one nonzero."""
+        s = np.arange(self.dim)
+        occ = (s >> o) & 1
+        ok = occ == (0 if dagger else 1)
+        below = s & ((1 << o) - 1)
+        par = np.array([bin(x).count("1") & 1 for x in below])
+        return np.where(ok, s ^ (1 << o), -1), np.where(par == 0, 1.0, -1.0)
+
     def cdag(self, o):
         return self._c[o].T.tocsr()
 
@@ -154,18 +164,21 @@ class AtomDiag:
         self.c_blocks = {}
         self.cdag_blocks = {}
         for o in range(n):
-            for which, conn, store in ((fs.c(o), self.c_conn, self.c_blocks), (fs.cdag(o), self.cdag_conn, self.cdag_blocks)):
-                M = which.tocsc()
+            for dagger, conn, store in ((False, self.c_conn, self.c_blocks), (True, self.cdag_conn, self.cdag_blocks)):
+                dst, sign = fs.action(o, dagger)
                 for j, fj in enumerate(self.fock_states):
-                    sub = M[:, fj].tocoo()
-                    if sub.nnz == 0:
+                    cols = np.nonzero(dst[fj] >= 0)[0]
+                    if len(cols) == 0:
                         continue
-                    tgt = np.unique(self.block_of[sub.row])
+                    rows = dst[fj[cols]]
+                    tgt = np.unique(self.block_of[rows])
                     if len(tgt) != 1:
                         raise ValueError("operator maps one block into several blocks")
                     t = int(tgt[0])
                     conn[o, j] = t
-                    store[(o, j)] = M[self.fock_states[t], :][:, fj].toarray()
+                    M = np.zeros((len(self.fock_states[t]), len(fj)))
+                    M[self.pos_in_block[rows], cols] = sign[fj[cols]]
+                    store[(o, j)] = M
 
     def packed_ops(self):
         """Concatenated c and c^dagger blocks in (o, j) order, as xca_problem expects."""
