@@ -25,7 +25,8 @@ def test_complex_sigma_and_spgf_match_complex_oracle(rotation, qn):
     from xca_amd.diag import all_connected_pairings
     from xca_amd.evaluator import DiagramEvaluator
     P = models.two_flavour_hopping(None if rotation == "none" else U_CPLX, qn=qn)
-    ev = DiagramEvaluator(P.poles, P.coeffs, P.grid, P.ad)
+    # unrotated, only G is complex (the constructor sees coefficients and operators): ask for it
+    ev = DiagramEvaluator(P.poles, P.coeffs, P.grid, P.ad, complex_arith=True if rotation == "none" else None)
     assert ev.complex_arith
     orc = po.OracleEvaluator(P, complex_arith=True)
     G = P.perturbed_G(seed=11, amp=1e-2)

@@ -84,3 +84,12 @@ def test_compressed_sigma_close_to_full():
     b = [s.real for s in po.OracleEvaluator(Q, complex_arith=False).sigma(P.G0, t)]
     d = max(np.abs(u - v).max() for u, v in zip(a, b)) / max(np.abs(u).max() for u in a)
     assert 0 < d < 1e-7
+
+
+def test_no_spurious_poles_at_a_tighter_tolerance():
+    """Two exact poles fitted at tol 1e-11 (the solver's hyb_tol for tol 1e-10): the fit that meets
+    tol keeps no pole of negligible weight."""
+    grid = DLRGrid(10.0, 100.0, 1e-12)
+    W = np.stack([0.04 * np.outer([1.0, 1.0], [1.0, 1.0]) / 2, 0.04 * np.outer([1.0, -1.0], [1.0, -1.0]) / 2])
+    x, Wf, err = hybfit.fit_hybridization(grid, few_pole_delta(grid, [-0.1414, 0.1414], W), tol=1e-11, compression=True)
+    assert err <= 1e-11 and len(x) == 2

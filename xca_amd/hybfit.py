@@ -17,6 +17,8 @@ aaa.py:22-106). The method is the published one those follow:
    grid (clustered geometrically at both ends, where the DLR basis varies on the scale 1/Lambda).
 3. Poles whose residues are negligible are dropped, and the remaining poles are refined by
    nonlinear least squares (variable projection: the residues are re-solved for every pole set).
+   Once a fit meets tol, its lightest pole is dropped and the rest refitted for as long as the fit
+   still meets tol.
 4. The AAA degree grows until the fit error is below tol.
 
 fit_hybridization keeps the reference's contract: without compression, or when the fit fails, the
@@ -168,6 +170,15 @@ def compress(w_dlr, coeffs, beta, tol, max_poles=None, lam=None, verbose=False):
                 break
             x = x[~small]
         err = float(np.abs(res).max()) / scale
+        while err <= tol and len(x) > 1:  # the lightest pole goes while the refitted rest still meets tol
+            drop = int(np.argmin(np.abs(W).max(axis=1)))
+            sol = least_squares(fun, np.delete(x, drop), method="lm", xtol=1e-15, ftol=1e-15, gtol=1e-15, max_nfev=200 * len(x))
+            x2 = np.sort(sol.x)
+            W2, res2 = residues(t, x2, target)
+            err2 = float(np.abs(res2).max()) / scale
+            if err2 > tol:
+                break
+            x, W, res, err = x2, W2, res2, err2
         if verbose:
             print(f"hybfit: degree {degree}: {len(x)} poles, error {err:.2e}")
         if err < best[0]:
