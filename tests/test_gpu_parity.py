@@ -198,3 +198,30 @@ def test_per_item_launch_form_equals_batched(case, order):
     ev.set_launch_mode(0)
     with pytest.raises(Exception, match="mode must be 0 or 1"):
         ev.set_launch_mode(2)
+
+
+def test_forced_width_timed_evaluation_after_graph_replays(monkeypatch):
+    """A tile width forced at context creation (no autotuner, whose launches would come first): graph
+    replays of the combined launch, then an eager, event-timed evaluation of the same sequence. The
+    timed result equals the replayed one and the oracle's."""
+    monkeypatch.setenv("XCA_TILE_VARIANT", "0")
+    P = models.oca_2orb()
+    G = P.perturbed_G()
+    ev = _evaluator(P)
+    ev.set_G(G)
+    for _ in range(3):
+        ev.sigma_order(2)
+    replayed = ev.get_sigma()
+    ev.sigma_order(2, timed=True)
+    ev.synchronize()  # reads the chain launch's event time
+    timed = ev.get_sigma()
+    items, macs, ms = ev.last_stats()
+    assert items == 5736 and ms > 0
+    assert all(np.array_equal(a, b) for a, b in zip(timed, replayed))
+    orc = po.OracleEvaluator(P, complex_arith=False)
+    ref = None
+    for order in (1, 2):
+        for s, t in all_connected_pairings(order):
+            x = [-s * y.real for y in orc.sigma(G, np.array(t, dtype=np.int32))]
+            ref = x if ref is None else [u + v for u, v in zip(ref, x)]
+    assert rel_err(timed, ref) <= TOL

@@ -772,6 +772,13 @@ void ctx_init_device(xca_ctx *c, int device) {
   HIP_CHECK(hipEventCreate(&c->ev0));
   HIP_CHECK(hipEventCreate(&c->ev1));
   hipStream_t s = c->stream;
+  HIP_CHECK(prepare_kernels(c->RB, s));
+  HIP_CHECK(launch_dyson(DysonBatch{}, s));  // the Dyson kernel's attribute, likewise
+  // Record the timing events once before any graph capture on this stream: an event whose first
+  // record comes after a capture on its stream is rejected as an invalid handle by the runtime
+  // (a forced tile width skips the autotuner, which otherwise records them first).
+  HIP_CHECK(hipEventRecord(c->ev0, s));
+  HIP_CHECK(hipEventRecord(c->ev1, s));
   c->opmats.upload(c->M.opmats, s);
   c->kp.upload(c->M.kp, s);
   c->km.upload(c->M.km, s);

@@ -1064,6 +1064,19 @@ hipError_t launch_wv2c(int RB, const WV2C &A, hipStream_t s) {
   return RB == 2 ? launch_wv2c_t<2, 8>(A, s) : launch_wv2c_t<4, 8>(A, s);
 }
 
+// Set the launch attributes (dynamic LDS) of every kernel instance for this RB, and the resident
+// workgroup counts of every chain variant, before any launch a stream capture may record. Each
+// launcher sets its attribute on its first call and returns at once on an empty batch.
+hipError_t prepare_kernels(int RB, hipStream_t s) {
+  hipError_t e;
+  if ((e = launch_tab(RB, TabBatch{}, s)) != hipSuccess) return e;
+  if ((e = launch_refl(RB, ReflArgs{}, s)) != hipSuccess) return e;
+  if ((e = launch_wv2c(RB, WV2C{}, s)) != hipSuccess) return e;
+  for (int v = 0; v < num_tile_variants(); ++v)
+    if (chain_resident(RB, v) <= 0) return hipErrorInvalidConfiguration;
+  return hipSuccess;
+}
+
 hipError_t launch_corr(const CorrArgs &A, hipStream_t s) {
   if (A.nchunks <= 0) return hipSuccess;
   hipLaunchKernelGGL(corr_kernel, dim3(A.nchunks), dim3(256), 0, s, A);
