@@ -428,6 +428,16 @@ def workload_leg(name, steps, cpu, threads, cpu_budget, cpu_cores):
                         "kernel": "chain_kernel, one launch for all topologies" if name != "oca5" else
                                   "large-block launch sequence (wgemm_kernel dominant)"}}
     log(f"{name}: {s_per:.3f} s per Sigma, {achieved:.2f} TFLOP/s ({achieved / PEAK_FP64:.3f} of peak)")
+    # one Dyson solve of every block with this Sigma (host arrays in and out); for the 5-orbital shell
+    # the blocks above 12 states take the device-wide path (LU of systems up to r n = 4900)
+    sig = ev.get_sigma()
+    ev.dyson_init(P.G0)
+    ev.dyson_solve(sig, 0.0)
+    t0 = time.perf_counter()
+    ev.dyson_solve(sig, 0.0)
+    out["dyson"] = {"s_per_solve": time.perf_counter() - t0, "largest_system": int(P.grid.r * max(P.dims)),
+                    "note": "all blocks, host arrays in and out, after one untimed solve"}
+    log(f"{name}: Dyson solve of every block {out['dyson']['s_per_solve']:.3f} s (largest system {out['dyson']['largest_system']})")
     topos = [np.array(t, dtype=np.int32) for o in range(1, max_order + 1) for _, t in all_connected_pairings(o)]
     t0 = time.perf_counter()
     for t in topos:

@@ -51,7 +51,9 @@ def build(verbose: bool = True) -> Path:
             jobs.append(ex.submit(_compile, s, objdir / (s.parent.name + "_" + s.stem + ".o"), False))
         objs = [j.result() for j in jobs]
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-Wl,-z,defs", "-o", str(tmp)] + [str(o) for o in objs]
+    # rocSOLVER / rocBLAS: LU factorisations and GEMMs of the Dyson systems too large for one workgroup
+    cmd = ([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-Wl,-z,defs", "-o", str(tmp)] + [str(o) for o in objs]
+           + ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas"])
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

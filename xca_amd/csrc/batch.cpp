@@ -84,6 +84,9 @@ class Batch {
     if (free_slots_.empty()) throw std::runtime_error("xca: batch is full");
     if (it->c->M.P.nb + kStBmax > kStatusStride) throw std::invalid_argument("xca: more than 64 blocks");
     require_dyson(it->c);
+    if (it->c->dyson->large)
+      throw std::invalid_argument("xca: the device self-consistency loop takes Dyson systems r * (block dimension) <= 784; "
+                                  "solve larger blocks with xca_dyson_solve in a host loop");
     it->slot = free_slots_.back();
     free_slots_.pop_back();
     it->serial = ++serial_;

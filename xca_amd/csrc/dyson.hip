@@ -132,6 +132,7 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
   const int pidx = find_seg(Bt.prefix, Bt.nprobs, blockIdx.x);
   const DysonArgs &A = Bt.probs[pidx];
   const int b = blockIdx.x - Bt.prefix[pidx], n = A.dims[b], r = A.r, nn = n * n, N = r * n;
+  if (A.large && A.large[b]) return;  // the whole workgroup: this block's system is solved device-wide
   const int Np = (N + 15) & ~15;
   const double eta = *A.eta;
   const int tid = threadIdx.x, NT = blockDim.x, lane = tid & 63, wv = tid >> 6, NWV = NT >> 6;

@@ -28,6 +28,7 @@ struct DysonArgs {
   const double *sigma;                        // packed blocks
   double *ggs, *wf, *sys, *rhs, *gout;       // scratch / output
   int *ipiv;                                  // pivots
+  const int *large;                           // (nb) 1: block solved by the large-block path (skipped here), or null
 };
 
 // Many problems: global block k belongs to problem s with prefix[s] <= k < prefix[s + 1].

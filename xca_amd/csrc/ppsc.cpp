@@ -33,6 +33,7 @@ DysonArgs dyson_args(xca_ctx *c, const double *eta_dev, const double *sigma_dev)
   A.cf2it = d->cf2it.ptr; A.tcf2it = d->tcf2it.ptr; A.W = d->W.ptr;
   A.G0 = d->G0.ptr; A.g0c = d->g0c.ptr; A.Wg0c = d->Wg0c.ptr; A.sigma = sigma_dev;
   A.ggs = d->ggs.ptr; A.wf = d->wf.ptr; A.sys = d->sys.ptr; A.rhs = d->rhs.ptr; A.gout = d->gnew.ptr; A.ipiv = d->ipiv.ptr;
+  A.large = d->large ? d->large_flag.ptr : nullptr;
   return A;
 }
 
@@ -52,17 +53,36 @@ int dyson_np_max(const xca_ctx *c) {
   return np;
 }
 
+// Blocks the one-workgroup kernel cannot take: a padded system above kDysonWgMax rows, or more than
+// 12 rows per block (its LDS panel of padded(r n) x 16 doubles holds the r n^2 vals2coefs columns
+// only while n <= 16; 12 is the tile engine's block limit).
+static bool is_large(const Problem &P, int b) {
+  return P.dims[b] > 12 || ((int64_t)P.r * P.dims[b] + 15) / 16 * 16 > kDysonWgMax;
+}
+
+int dyson_np_max_small(const xca_ctx *c) {
+  int np = 16;
+  for (int b = 0; b < c->M.P.nb; ++b)
+    if (!is_large(c->M.P, b)) np = std::max(np, (c->M.P.r * c->M.P.dims[b] + 15) / 16 * 16);
+  return np;
+}
+
 int64_t dyson_bytes(const xca_ctx *c) {
   const Problem &P = c->M.P;
   int64_t sys = 0, rhs = 0, piv = 0, ndiag = 0;
+  int n_large = 0;
   for (int b = 0; b < P.nb; ++b) {
+    ndiag += P.dims[b];
+    if (is_large(P, b)) {
+      n_large = std::max(n_large, P.dims[b]);
+      continue;
+    }
     const int64_t Np = ((int64_t)P.r * P.dims[b] + 15) / 16 * 16;
     sys += Np * Np;
     rhs += Np * P.dims[b];
     piv += Np;
-    ndiag += P.dims[b];
   }
-  return 8 * (6 * c->M.G_size + sys + rhs + P.r * ndiag + 6 * (int64_t)P.r * P.r) + 4 * piv;
+  return 8 * (6 * c->M.G_size + sys + rhs + P.r * ndiag + 6 * (int64_t)P.r * P.r) + 4 * piv + large_dyson_bytes(P.r, n_large);
 }
 
 void require_dyson(xca_ctx *c) {
@@ -74,7 +94,6 @@ void require_dyson(xca_ctx *c) {
 extern "C" int xca_dyson_init(xca_ctx *c, const double *G0) {
   try {
     HIP_CHECK(hipSetDevice(c->device));
-    if (dyson_np_max(c) > 784) throw std::invalid_argument("xca: the Dyson solve supports r * (block dimension) <= 784");
     xca_dyson_release(c);
     auto *d = new XcaDyson();
     c->dyson = d;
@@ -84,9 +103,16 @@ extern "C" int xca_dyson_init(xca_ctx *c, const double *G0) {
     hipStream_t s = c->stream;
     std::vector<int> dims(P.dims);
     std::vector<int64_t> goff(nb), gend(nb), soff(nb), poff(nb), roff(nb);
+    std::vector<int> large_blocks, large_flag(nb, 0);
     int64_t so = 0, po = 0, ro = 0;
     for (int b = 0; b < nb; ++b) {
-      const int64_t Np = ((int64_t)r * P.dims[b] + 15) / 16 * 16;
+      // blocks whose system exceeds one workgroup are solved device-wide and hold no slot here
+      const bool big = is_large(P, b);
+      const int64_t Np = big ? 0 : ((int64_t)r * P.dims[b] + 15) / 16 * 16;
+      if (big) {
+        large_blocks.push_back(b);
+        large_flag[b] = 1;
+      }
       goff[b] = c->M.G_off[b];
       gend[b] = goff[b] + (int64_t)r * P.dims[b] * P.dims[b];
       soff[b] = so;
@@ -132,6 +158,8 @@ extern "C" int xca_dyson_init(xca_ctx *c, const double *G0) {
     d->ipiv.alloc(po);
     d->trs.alloc((size_t)r * d->ndiag);
     d->status.alloc(status_stride(nb));
+    d->large_flag.upload(large_flag, s);
+    d->large = large_dyson_init(c, large_blocks);
     d->device_bytes = dyson_bytes(c);
     HIP_CHECK(hipStreamSynchronize(s));
     return 0;
@@ -162,7 +190,8 @@ extern "C" int xca_dyson_solve(xca_ctx *c, const double *sigma, double eta, doub
     DevBuf<int> dpre;
     dA.upload(A, s);
     dpre.upload(pre, s);
-    HIP_CHECK(launch_dyson(DysonBatch{dA.ptr, dpre.ptr, 1, c->M.P.nb, dyson_np_max(c)}, s));
+    HIP_CHECK(launch_dyson(DysonBatch{dA.ptr, dpre.ptr, 1, c->M.P.nb, dyson_np_max_small(c)}, s));
+    if (d->large) large_dyson_solve(c, d->large, sig, eta, d->gnew.ptr);
     HIP_CHECK(hipMemcpyAsync(G_out, d->gnew.ptr, c->M.G_size * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     return 0;

@@ -277,9 +277,12 @@ class DiagramEvaluator:
         return self.unpack(out)
 
     def ppsc_solve(self, max_order, tol=1e-10, maxiter=100, mix=1.0, eta0=0.0):
-        """Self-consistent pseudo-particle G; returns (G blocks, eta, iterations, last max|G_new - G|)."""
-        if self.complex_arith:
-            return self._ppsc_solve_complex(max_order, tol, maxiter, mix, eta0)
+        """Self-consistent pseudo-particle G; returns (G blocks, eta, iterations, last max|G_new - G|).
+        The whole loop runs as one device launch sequence per iteration (xca_ppsc_solve), except for
+        complex problems and problems with blocks above 12 rows (Sigma on the large-block engine),
+        whose loop runs on the host around the device's Sigma and Dyson solve."""
+        if self.complex_arith or int(self.block_dims.max()) > 12:
+            return self._ppsc_solve_host(max_order, tol, maxiter, mix, eta0)
         out = np.zeros(sum(self.r * int(b) * int(b) for b in self.dims))
         eta = C.c_double(0)
         it = C.c_int(0)
@@ -288,12 +291,12 @@ class DiagramEvaluator:
                                    C.byref(it), C.byref(diff)))
         return self.unpack(out), eta.value, it.value, diff.value
 
-    def _ppsc_solve_complex(self, max_order, tol, maxiter, mix, eta0):
+    def _ppsc_solve_host(self, max_order, tol, maxiter, mix, eta0):
         """The loop of BlockSparseSolver.solve with classic normalisation (block_sparse_solver.py:
-        289-346, 427-449, 482-487) for a complex problem: Sigma and the Dyson solve (with its
-        hermitisation) on the device, in the embedding; Z = -Re Tr G_new(beta), the energy shift, the
-        convergence test on max |G_new - G| of the complex blocks and the mixing on the host (O(r sum b^2)
-        work per iteration)."""
+        289-346, 427-449, 482-487) driven from the host: Sigma and the Dyson solve (with its
+        hermitisation) on the device (for a complex problem in the embedding); Z = -Re Tr G_new(beta),
+        the energy shift, the convergence test on max |G_new - G| and the mixing on the host
+        (O(r sum b^2) work per iteration)."""
         if not hasattr(self, "_G0"):
             raise XcaError("ppsc_solve: dyson_init(G0) first")
         grid = self.grid
