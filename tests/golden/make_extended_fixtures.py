@@ -45,7 +45,9 @@ def problem_from_fixture(fx):
     beta = float(fx["beta"])
     grid = DLRGrid.from_nodes(beta, fx["it"], fx["rf"], lam=float(fx["lam"]), eps=float(fx["eps"]))
     ad = tca3_atom()
-    P = models.ImpurityProblem(grid=grid, ad=ad, poles=grid.rf / beta, coeffs=np.array(fx["coeffs"]), G0=[], name="tca3_extended")
+    G0, eta0 = ad.atomic_G0(beta, grid.tau)
+    P = models.ImpurityProblem(grid=grid, ad=ad, poles=grid.rf / beta, coeffs=np.array(fx["coeffs"]), G0=G0, eta0=eta0,
+                               name="tca3_extended")
     G = P.unpack(np.array(fx["G"]))
     return P, G
 

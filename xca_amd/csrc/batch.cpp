@@ -297,6 +297,7 @@ class Batch {
       const int64_t stride = 16LL * g->RB * tile_columns(g->variant);
       g->scratch.alloc((size_t)chain_resident(g->RB, g->variant) * stride);
       g->B = ChainBatch{g->segs.ptr, g->prefix.ptr, (int)g->h_segs.size(), g->h_prefix.back(), queues_[qi]->ptr, g->scratch.ptr, stride};
+      g->B.maxb = max_block_of(g->h_segs);
       ++qi;
       chain_groups_.push_back(std::move(g));
     }
@@ -513,6 +514,7 @@ static SigmaAll *sigma_all(xca_ctx *c, int max_order) {
     const int64_t stride = 16LL * G.RB * tile_columns(G.variant);
     G.scratch.alloc((size_t)chain_resident(G.RB, G.variant) * stride);
     G.B = ChainBatch{G.segs.ptr, G.prefix.ptr, (int)G.h_segs.size(), G.h_prefix.back(), G.queue.ptr, G.scratch.ptr, stride};
+    G.B.maxb = max_block_of(G.h_segs);
   }
   S->red.upload(S->h_red, s);
   const Problem &P = c->M.P;

@@ -529,7 +529,14 @@ ChainArgs chain_args(xca_ctx *c, DevPlan *dp, DevVariant *V) {
   A.kpow = c->kpow.ptr; A.partial = V->partial.ptr; A.tuples = dp->tuples.ptr;
   A.nchunks = V->nchunks;
   A.r = c->M.P.r; A.p = c->M.P.p; A.maxexp = c->maxexp;
+  A.maxb = ctx_max_block(c);
   return A;
+}
+
+int ctx_max_block(const xca_ctx *c) {
+  int m = 1;
+  for (int b = 0; b < c->M.P.nb; ++b) m = std::max(m, c->M.P.dims[b]);
+  return m;
 }
 
 // Tile-width autotuner: time every applicable variant on this plan (all chunks when the plan is
@@ -570,7 +577,8 @@ void autotune(xca_ctx *c, DevPlan *dp) {
     DevBuf<double> scr;
     const int64_t sstride = (int64_t)c->RP * tile_columns(v);
     scr.alloc((size_t)chain_resident(c->RB, v) * sstride);
-    const ChainBatch B{dA.ptr, dpre.ptr, 1, A[0].nchunks, dp->queue.ptr, scr.ptr, sstride};
+    ChainBatch B{dA.ptr, dpre.ptr, 1, A[0].nchunks, dp->queue.ptr, scr.ptr, sstride};
+    B.maxb = A[0].maxb;
     float best_ms = 1e30f;
     for (int rep = 0; rep < 2; ++rep) {
       HIP_CHECK(hipEventRecord(c->ev0, s));
@@ -652,10 +660,15 @@ void accumulate(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_t 
   HIP_CHECK(hipEventRecord(c->ev0, s));
   if (c->launch_mode == 1) {
     // one launch per chunk; an explicit flat-index subset has one chunk per (path, pole tuple) item
-    for (int k = 0; k < V->nchunks; ++k)
-      HIP_CHECK(launch_chain(c->RB, dp->chosen, ChainBatch{V->item_seg.ptr + k, V->item_prefix.ptr, 1, 1, dp->queue.ptr, V->scratch.ptr, stride}, s));
+    for (int k = 0; k < V->nchunks; ++k) {
+      ChainBatch B{V->item_seg.ptr + k, V->item_prefix.ptr, 1, 1, dp->queue.ptr, V->scratch.ptr, stride};
+      B.maxb = V->h_seg[0].maxb;
+      HIP_CHECK(launch_chain(c->RB, dp->chosen, B, s));
+    }
   } else {
-    HIP_CHECK(launch_chain(c->RB, dp->chosen, ChainBatch{V->seg.ptr, V->seg_prefix.ptr, 1, V->nchunks, dp->queue.ptr, V->scratch.ptr, stride}, s));
+    ChainBatch B{V->seg.ptr, V->seg_prefix.ptr, 1, V->nchunks, dp->queue.ptr, V->scratch.ptr, stride};
+    B.maxb = V->h_seg[0].maxb;
+    HIP_CHECK(launch_chain(c->RB, dp->chosen, B, s));
   }
   HIP_CHECK(hipEventRecord(c->ev1, s));
   int maxsz = 0;

@@ -467,7 +467,9 @@ static void corr_eval(xca_ctx *c, const double *G, int m, const int *topo, const
     HIP_CHECK(hipEventCreate(&e1));
     ev.push_back(e1);
     HIP_CHECK(hipEventRecord(e0, s));
-    HIP_CHECK(launch_chain(c->RB, dp->variant, ChainBatch{B.seg.ptr, B.seg_prefix.ptr, 1, B.nchunks, dp->queue.ptr, nullptr, 0}, s));
+    ChainBatch CB{B.seg.ptr, B.seg_prefix.ptr, 1, B.nchunks, dp->queue.ptr, nullptr, 0};
+    CB.maxb = ctx_max_block(c);
+    HIP_CHECK(launch_chain(c->RB, dp->variant, CB, s));
     HIP_CHECK(hipEventRecord(e1, s));
     HIP_CHECK(launch_refl(c->RB, ReflArgs{c->D, B.rjobs.ptr, dp->store.ptr, B.nrjobs, 0}, s));
     CorrArgs A{B.terms.ptr, B.runs.ptr, B.cchunks.ptr, dp->store.ptr, dp->cval.ptr, dp->cidx.ptr, dp->partial.ptr, B.ncchunks, r, nout, 0};

@@ -400,12 +400,44 @@ __device__ __forceinline__ T uniform_load(const T *p) {
 // Chain kernel: persistent workgroups drain a queue of chunks (runs of tiles of one program); each
 // chunk writes its own partial slot, so the result does not depend on which workgroup took it.
 
-template <int RB, int NW, int NTILE>
+// S[t][e] = src(t, e) for t < r, e < w, with four global loads in flight per thread
+template <int NT, class Src>
+__device__ __forceinline__ void load_slab(double *S, int ldn, int r, int w, int tid, Src src) {
+  const int n = r * w;
+  for (int x0 = 0; x0 < n; x0 += 4 * NT) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int x = x0 + u * NT + tid;
+      v[u] = x < n ? src(x / w, x % w) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int x = x0 + u * NT + tid;
+      if (x < n) S[(size_t)(x / w) * ldn + x % w] = v[u];
+    }
+  }
+}
+
+// columns [0, w) of every node row copied to the columns of items 1 .. nit - 1
+template <int NT>
+__device__ __forceinline__ void replicate_items(double *S, int ldn, int r, int w, int nit, int tid) {
+  const int m = (nit - 1) * w;
+  for (int x = tid; x < r * m; x += NT) {
+    const int t = x / m, c = x % m;
+    S[(size_t)t * ldn + w + c] = S[(size_t)t * ldn + c % w];
+  }
+}
+
+// MB: largest block of the launch's programs (register arrays of operator and table rows)
+template <int RB, int NW, int NTILE, int MB>
 __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
   constexpr int RP = 16 * RB, NT = 64 * NW, LDN = NTILE + 4;
   constexpr int NCBT = NTILE / 16;           // 16-column blocks in a tile
   constexpr int NCB = NCBT * RB / NW;        // column blocks per wave in the row-partitioned GEMMs
   static_assert(NW % RB == 0 && (NCBT * RB) % NW == 0 && NCB >= 1, "waves must tile the (row, column) blocks");
+  constexpr bool PF = MB <= 6;               // table rows read a round ahead (register budget)
+  constexpr int VU = MB <= 6 ? 2 : 1;        // vertex rows per pass
   extern __shared__ double lds[];
   __shared__ int qnext;
   double *S0 = lds, *S1 = lds + RP * LDN;
@@ -479,12 +511,14 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
       cols = cache_cols;
       oi0 = P.split;
       const int w = rows * cols;
-      for (int x = tid; x < r * w; x += NT) S0[(size_t)(x / w) * LDN + x % w] = cache[x];
-    } else {  // T = G_b1, or its transpose
-      const int b1 = P.b1, w = b1 * b1;
-      for (int x = tid; x < r * nit * w; x += NT) {
-        const int t = x / (nit * w), c = x % (nit * w), e = c % w;
-        S0[(size_t)t * LDN + c] = A.G[P.g_init + (size_t)t * w + (P.gtr ? (e % b1) * b1 + e / b1 : e)];
+      load_slab<NT>(S0, LDN, r, w, tid, [&](int t, int e) { return cache[(size_t)t * w + e]; });
+    } else {  // T = G_b1, or its transpose: read once into item 0's columns, then copied to the others
+      const int b1 = P.b1, w = b1 * b1, gtr = P.gtr;
+      const double *Gb = A.G + P.g_init;
+      load_slab<NT>(S0, LDN, r, w, tid, [&](int t, int e) { return Gb[(size_t)t * w + (gtr ? (e % b1) * b1 + e / b1 : e)]; });
+      if (nit > 1) {
+        __syncthreads();
+        replicate_items<NT>(S0, LDN, r, w, nit, tid);
       }
     }
     __syncthreads();
@@ -500,34 +534,46 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
           cache_rows = rows;
           cache_cols = cols;
         }
-        for (int x = tid; x < r * (P_items - 1) * w; x += NT) {
-          const int t = x / ((P_items - 1) * w), c = x % ((P_items - 1) * w);
-          Scur[(size_t)t * LDN + w + c] = Scur[(size_t)t * LDN + c % w];
-        }
+        replicate_items<NT>(Scur, LDN, r, w, P_items, tid);
         nit = P_items;
         __syncthreads();
       }
       const DOp op = uniform_load(A.ops + P.op0 + oi);
       if (op.type == 0) {  // vertex: T' = K F T
         const int bin = op.bin, bout = op.bout, win = bin * cols, wout = bout * cols;
-        // one output row (t, item, i) per thread: the operator row is loaded once and applied to
-        // every column; lanes of one (t, item) read the same T entries (LDS broadcast)
-        for (int x = tid; x < r * nit * bout; x += NT) {
-          const int t = x / (nit * bout), rem = x % (nit * bout), it = rem / bout, i = rem % bout;
-          const double *F = A.opmats + op.fbase + (op.bar_line >= 0 ? (int64_t)ipole[it * kMaxLines + op.bar_line] * op.fstride_l : 0);
-          double fr[kMaxB];
+        // one output row (t, item, i) per thread, VU rows per pass with their global reads issued
+        // first: the operator row is loaded once and applied to every column; lanes of one
+        // (t, item) read the same T entries (LDS broadcast)
+        const int n = r * nit * bout, nib = nit * bout;
+        const double *kt = op.ks > 0 ? A.kp : A.km;
+        for (int x0 = tid; x0 < n; x0 += VU * NT) {
+          double fr[VU][MB], kf[VU];
+          int tt[VU], itt[VU], ii[VU];
 #pragma unroll
-          for (int k = 0; k < kMaxB; ++k) fr[k] = (k < bin) ? F[i * bin + k] : 0.0;
-          double kf = 1.0;
-          if (op.ks != 0) kf = (op.ks > 0 ? A.kp : A.km)[(size_t)ipole[it * kMaxLines + op.kx] * r + t];
-          const double *Tin = Scur + (size_t)t * LDN + it * win;
-          double *Tout = Soth + (size_t)t * LDN + it * wout + i * cols;
-          for (int j = 0; j < cols; ++j) {
-            double s = 0.0;
+          for (int u = 0; u < VU; ++u) {
+            const int x = x0 + u * NT;
+            const bool act = x < n;
+            const int t = act ? x / nib : 0, rem = act ? x % nib : 0, it = rem / bout, i = rem % bout;
+            tt[u] = t;
+            itt[u] = it;
+            ii[u] = i;
+            const double *F = A.opmats + op.fbase + (op.bar_line >= 0 ? (int64_t)ipole[it * kMaxLines + op.bar_line] * op.fstride_l : 0);
 #pragma unroll
-            for (int k = 0; k < kMaxB; ++k)
-              if (k < bin) s += fr[k] * Tin[k * cols + j];
-            Tout[j] = kf * s;
+            for (int k = 0; k < MB; ++k) fr[u][k] = (act && k < bin) ? F[i * bin + k] : 0.0;
+            kf[u] = (act && op.ks != 0) ? kt[(size_t)ipole[it * kMaxLines + op.kx] * r + t] : 1.0;
+          }
+#pragma unroll
+          for (int u = 0; u < VU; ++u) {
+            if (x0 + u * NT >= n) break;
+            const double *Tin = Scur + (size_t)tt[u] * LDN + itt[u] * win;
+            double *Tout = Soth + (size_t)tt[u] * LDN + itt[u] * wout + ii[u] * cols;
+            for (int j = 0; j < cols; ++j) {
+              double s = 0.0;
+#pragma unroll
+              for (int k = 0; k < MB; ++k)
+                if (k < bin) s += fr[u][k] * Tin[k * cols + j];
+              Tout[j] = kf[u] * s;
+            }
           }
         }
         rows = bout;
@@ -540,6 +586,37 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
         const int ncb = (nit * w + 15) >> 4;
         const int cpg = (ncb + NGRP - 1) / NGRP;
         const int cbw = grp * cpg, ncw = min(cpg, ncb - cbw);
+        // node-local phase layout (below): one output row (t, item, i) per thread, rounds of gpr
+        // whole (t, item) groups; the groups past the last full round are split into single
+        // elements when they fit one round
+        const int ngroups = r * nit, gpr = NT / b;
+        const int nfull = ngroups <= gpr ? ngroups : (ngroups / gpr) * gpr, ntail = ngroups - nfull;
+        const bool split_tail = ntail > 0 && ntail * b * cols <= NT;
+        const int gend = split_tail ? nfull : ngroups;
+        const int gl = tid / b, irow = tid % b;
+        // table rows f_i(t), (W f)_i(t) of group gi = (t, item) and row i (zeros when not act)
+        auto load_elem = [&](bool act, int gi, int i, double(&f_)[MB], double(&wf_)[MB]) {
+          const int t = act ? gi / nit : 0, it = act ? gi % nit : 0;
+          int64_t e = op.ebase;
+#pragma unroll
+          for (int q = 0; q < kMaxLines; ++q)
+            if (q < op.nlines) e += (int64_t)ipole[it * kMaxLines + op.lines[q]] * op.estride[q];
+          const double *f = A.ftab + e + (size_t)t * b * b + i * b;
+          const double *wf = f + (size_t)r * b * b;
+#pragma unroll
+          for (int k = 0; k < MB; ++k) {
+            f_[k] = (act && k < b) ? f[k] : 0.0;
+            wf_[k] = (act && k < b) ? wf[k] : 0.0;
+          }
+        };
+        // this thread's rows in the round starting at group g0
+        auto load_rows = [&](int g0, double(&f_)[MB], double(&wf_)[MB]) {
+          load_elem(gl < gpr && g0 + gl < gend, g0 + gl, irow, f_, wf_);
+        };
+        // with small blocks the first round's rows are read now, so that their latency overlaps the
+        // GEMMs, and each round reads the next round's rows before it computes (registers allow it)
+        double fr[MB], wfr[MB];
+        if (PF) load_rows(0, fr, wfr);
         {  // y = (L^{-1} P) T -> Soth
           v4d acc[NCB];
           zero_acc(acc);
@@ -563,32 +640,20 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
         __syncthreads();
         // c = -(Wf o g + f o Wg) -> Soth, d = f o g -> Scur, in place. One output row (t, item, i)
         // per thread with the table rows f_i, (W f)_i in registers; rounds hold whole (t, item)
-        // groups and 4 columns, whose reads all happen before any of them is overwritten.
+        // groups and 4 columns, whose reads all happen before any of them is overwritten. The next
+        // round's table rows (or the tail element's) are read before the current round computes.
         {
-          const int ngroups = r * nit, gpr = NT / b;
-          // groups past the last full round are split into single elements when they fit one round
-          const int full = ngroups <= gpr ? ngroups : (ngroups / gpr) * gpr, tail = ngroups - full;
-          const bool split_tail = tail > 0 && tail * b * cols <= NT;
-          const int gend = split_tail ? full : ngroups;
-          const int gl = tid / b, i = tid % b;
+          const int bc = b * cols;
+          const bool tail_act = split_tail && tid < ntail * bc;
+          const int tgi = nfull + (tail_act ? tid / bc : 0), tie = tail_act ? (tid % bc) / cols : 0, tje = tail_act ? tid % cols : 0;
           for (int g0 = 0; g0 < gend; g0 += gpr) {
             const int gi = g0 + gl;
             const bool act = gl < gpr && gi < gend;
             const int t = act ? gi / nit : 0, it = act ? gi % nit : 0;
-            double fr[kMaxB], wfr[kMaxB];
-            if (act) {
-              int64_t e = op.ebase;
-#pragma unroll
-              for (int q = 0; q < kMaxLines; ++q)
-                if (q < op.nlines) e += (int64_t)ipole[it * kMaxLines + op.lines[q]] * op.estride[q];
-              const double *f = A.ftab + e + (size_t)t * b * b + i * b;
-              const double *wf = f + (size_t)r * b * b;
-#pragma unroll
-              for (int k = 0; k < kMaxB; ++k) {
-                fr[k] = (k < b) ? f[k] : 0.0;
-                wfr[k] = (k < b) ? wf[k] : 0.0;
-              }
-            }
+            double fn[MB], wfn[MB];
+            if (!PF) load_rows(g0, fr, wfr);
+            else if (g0 + gpr < gend) load_rows(g0 + gpr, fn, wfn);
+            else if (split_tail) load_elem(tail_act, tgi, tie, fn, wfn);
             double *gcol = Soth + (size_t)t * LDN + it * w, *wgcol = Scur + (size_t)t * LDN + it * w;
             for (int j0 = 0; j0 < cols; j0 += 4) {
               double cj[4], dj[4];
@@ -598,7 +663,7 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
                 const int j = j0 + jj;
                 if (act && j < cols) {
 #pragma unroll
-                  for (int k = 0; k < kMaxB; ++k)
+                  for (int k = 0; k < MB; ++k)
                     if (k < b) {
                       const double g = gcol[k * cols + j], wg = wgcol[k * cols + j];
                       s1 += wfr[k] * g + fr[k] * wg;
@@ -613,40 +678,38 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
 #pragma unroll
                 for (int jj = 0; jj < 4; ++jj)
                   if (j0 + jj < cols) {
-                    gcol[i * cols + j0 + jj] = cj[jj];
-                    wgcol[i * cols + j0 + jj] = dj[jj];
+                    gcol[irow * cols + j0 + jj] = cj[jj];
+                    wgcol[irow * cols + j0 + jj] = dj[jj];
                   }
               }
               __syncthreads();
             }
-          }
-          if (split_tail) {  // one element (group, row, column) per thread
-            const int e = tid, bc = b * cols;
-            const bool act = e < tail * bc;
-            const int gi = full + (act ? e / bc : 0), ie = act ? (e % bc) / cols : 0, je = act ? e % cols : 0;
-            const int t = gi / nit, it = gi % nit;
-            double s1 = 0.0, s2 = 0.0;
-            if (act) {
-              int64_t en = op.ebase;
+            if (PF) {
 #pragma unroll
-              for (int q = 0; q < kMaxLines; ++q)
-                if (q < op.nlines) en += (int64_t)ipole[it * kMaxLines + op.lines[q]] * op.estride[q];
-              const double *f = A.ftab + en + (size_t)t * b * b + ie * b;
-              const double *wf = f + (size_t)r * b * b;
+              for (int k = 0; k < MB; ++k) {
+                fr[k] = fn[k];
+                wfr[k] = wfn[k];
+              }
+            }
+          }
+          if (split_tail) {  // one element (group, row, column) per thread; its rows are in fr, wfr
+            if (!PF || gend == 0) load_elem(tail_act, tgi, tie, fr, wfr);
+            const int t = tgi / nit, it = tgi % nit;
+            double s1 = 0.0, s2 = 0.0;
+            if (tail_act) {
               const double *gcol = Soth + (size_t)t * LDN + it * w, *wgcol = Scur + (size_t)t * LDN + it * w;
 #pragma unroll
-              for (int k = 0; k < kMaxB; ++k)
+              for (int k = 0; k < MB; ++k)
                 if (k < b) {
-                  const double fk = f[k], wfk = wf[k];
-                  const double g = gcol[k * cols + je], wg = wgcol[k * cols + je];
-                  s1 += wfk * g + fk * wg;
-                  s2 += fk * g;
+                  const double g = gcol[k * cols + tje], wg = wgcol[k * cols + tje];
+                  s1 += wfr[k] * g + fr[k] * wg;
+                  s2 += fr[k] * g;
                 }
             }
             __syncthreads();
-            if (act) {
-              Soth[(size_t)t * LDN + it * w + ie * cols + je] = -s1;
-              Scur[(size_t)t * LDN + it * w + ie * cols + je] = s2;
+            if (tail_act) {
+              Soth[(size_t)t * LDN + it * w + tie * cols + tje] = -s1;
+              Scur[(size_t)t * LDN + it * w + tie * cols + tje] = s2;
             }
             __syncthreads();
           }
@@ -674,26 +737,30 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
           const int t = x / (nit * b0), rem = x % (nit * b0), it = rem / b0, j = rem % b0;
           const double *Tt = Scur + (size_t)t * LDN + it * win;
           const double *Fht = A.opmats + op.fbase + (size_t)t * qs * b1 * b0 + j;
-          double out[kMaxB];
+          double out[MB], fh[MB];
 #pragma unroll
-          for (int i = 0; i < kMaxB; ++i) out[i] = 0.0;
+          for (int i = 0; i < MB; ++i) out[i] = 0.0;
+#pragma unroll
+          for (int c = 0; c < MB; ++c) fh[c] = (c < b1) ? Fht[(size_t)c * b0] : 0.0;
           for (int mu = 0; mu < qs; ++mu) {
-            double fh[kMaxB];
+            double fn[MB];  // the next mu's column, read before this one is used
 #pragma unroll
-            for (int c = 0; c < kMaxB; ++c) fh[c] = (c < b1) ? Fht[(size_t)(mu * b1 + c) * b0] : 0.0;
+            for (int c = 0; c < MB; ++c) fn[c] = (mu + 1 < qs && c < b1) ? Fht[(size_t)((mu + 1) * b1 + c) * b0] : 0.0;
             const double *Fm = A.opmats + op.mubase + (int64_t)mu * bvp * bv;
             for (int a = 0; a < bv; ++a) {
               double y = 0.0;
 #pragma unroll
-              for (int c = 0; c < kMaxB; ++c)
+              for (int c = 0; c < MB; ++c)
                 if (c < b1) y += Tt[a * b1 + c] * fh[c];
 #pragma unroll
-              for (int i = 0; i < kMaxB; ++i)
+              for (int i = 0; i < MB; ++i)
                 if (i < bvp) out[i] += Fm[i * bv + a] * y;
             }
+#pragma unroll
+            for (int c = 0; c < MB; ++c) fh[c] = fn[c];
           }
 #pragma unroll
-          for (int i = 0; i < kMaxB; ++i)
+          for (int i = 0; i < MB; ++i)
             if (i < bvp) Soth[(size_t)t * LDN + it * wout + i * b0 + j] = out[i];
         }
         // the columns past the last full round: one output element (t, item, i, j) per thread
@@ -702,19 +769,23 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
           const int t = x / (nit * b0), rem = x % (nit * b0), it = rem / b0, j = rem % b0;
           const double *Tt = Scur + (size_t)t * LDN + it * win;
           const double *Fht = A.opmats + op.fbase + (size_t)t * qs * b1 * b0 + j;
-          double o = 0.0;
-          for (int mu = 0; mu < qs; ++mu) {
-            double fh[kMaxB];
+          double o = 0.0, fh[MB];
 #pragma unroll
-            for (int c = 0; c < kMaxB; ++c) fh[c] = (c < b1) ? Fht[(size_t)(mu * b1 + c) * b0] : 0.0;
+          for (int c = 0; c < MB; ++c) fh[c] = (c < b1) ? Fht[(size_t)c * b0] : 0.0;
+          for (int mu = 0; mu < qs; ++mu) {
+            double fn[MB];
+#pragma unroll
+            for (int c = 0; c < MB; ++c) fn[c] = (mu + 1 < qs && c < b1) ? Fht[(size_t)((mu + 1) * b1 + c) * b0] : 0.0;
             const double *Fmi = A.opmats + op.mubase + (int64_t)mu * bvp * bv + i * bv;
             for (int a = 0; a < bv; ++a) {
               double y = 0.0;
 #pragma unroll
-              for (int c = 0; c < kMaxB; ++c)
+              for (int c = 0; c < MB; ++c)
                 if (c < b1) y += Tt[a * b1 + c] * fh[c];
               o += Fmi[a] * y;
             }
+#pragma unroll
+            for (int c = 0; c < MB; ++c) fh[c] = fn[c];
           }
           Soth[(size_t)t * LDN + it * wout + i * b0 + j] = o;
         }
@@ -914,17 +985,17 @@ static hipError_t launch_tab_t(const TabBatch &B, hipStream_t s) {
 
 // workgroups of this instance resident on the device at once (set on first use, with the LDS
 // attribute, outside any stream capture)
-template <int RB, int NW, int NTILE>
+template <int RB, int NW, int NTILE, int MB>
 static int &resident_of() {
   static int resident = 0;
   return resident;
 }
 
-template <int RB, int NW, int NTILE>
+template <int RB, int NW, int NTILE, int MB>
 static hipError_t init_chain_t() {
-  int &resident = resident_of<RB, NW, NTILE>();
+  int &resident = resident_of<RB, NW, NTILE, MB>();
   const size_t sh = lds_bytes<RB, NW, NTILE>();
-  const void *fn = (const void *)chain_kernel<RB, NW, NTILE>;
+  const void *fn = (const void *)chain_kernel<RB, NW, NTILE, MB>;
   if (resident == 0) {
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
     if (e != hipSuccess) return e;
@@ -937,17 +1008,30 @@ static hipError_t init_chain_t() {
   return hipSuccess;
 }
 
-// persistent grid: as many workgroups as fit on the device at once, at most one per chunk
+// Both block-size instances of a tile width; the workgroup count of the width is the larger of
+// the two, so that per-workgroup scratch sized by it serves either.
+template <int RB, int NW, int NTILE>
+static int init_width_t() {
+  if (init_chain_t<RB, NW, NTILE, 4>() != hipSuccess || init_chain_t<RB, NW, NTILE, kMaxB>() != hipSuccess) return 0;
+  return std::max(resident_of<RB, NW, NTILE, 4>(), resident_of<RB, NW, NTILE, kMaxB>());
+}
+
+// persistent grid: as many workgroups as fit on the device at once, at most one per chunk. The
+// instance with 4-element register rows runs launches whose blocks all have at most 4 states.
 template <int RB, int NW, int NTILE>
 static hipError_t launch_chain_t(const ChainBatch &B, hipStream_t s) {
-  hipError_t e = init_chain_t<RB, NW, NTILE>();
-  if (e != hipSuccess) return e;
-  const int resident = resident_of<RB, NW, NTILE>();
+  if (init_width_t<RB, NW, NTILE>() <= 0) return hipErrorInvalidConfiguration;
   const size_t sh = lds_bytes<RB, NW, NTILE>();
   if (B.total <= 0) return hipSuccess;
+  if (B.maxb < 1 || B.maxb > kMaxB) return hipErrorInvalidValue;
   hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, s, B.queue);
-  const int grid = std::min(B.total, resident);
-  hipLaunchKernelGGL((chain_kernel<RB, NW, NTILE>), dim3(grid), dim3(64 * NW), sh, s, B);
+  if (B.maxb <= 4) {
+    const int grid = std::min(B.total, resident_of<RB, NW, NTILE, 4>());
+    hipLaunchKernelGGL((chain_kernel<RB, NW, NTILE, 4>), dim3(grid), dim3(64 * NW), sh, s, B);
+  } else {
+    const int grid = std::min(B.total, resident_of<RB, NW, NTILE, kMaxB>());
+    hipLaunchKernelGGL((chain_kernel<RB, NW, NTILE, kMaxB>), dim3(grid), dim3(64 * NW), sh, s, B);
+  }
   return hipGetLastError();
 }
 
@@ -989,22 +1073,20 @@ hipError_t launch_chain(int RB, int variant, const ChainBatch &B, hipStream_t s)
 }
 
 int chain_resident(int RB, int variant) {
-  hipError_t e = hipErrorInvalidValue;
-  int res = 0;
   if (RB == 2) {
     switch (variant) {
-      case 0: e = init_chain_t<2, 8, 128>(); res = resident_of<2, 8, 128>(); break;
-      case 1: e = init_chain_t<2, 4, 64>(); res = resident_of<2, 4, 64>(); break;
-      case 2: e = init_chain_t<2, 4, 32>(); res = resident_of<2, 4, 32>(); break;
+      case 0: return init_width_t<2, 8, 128>();
+      case 1: return init_width_t<2, 4, 64>();
+      case 2: return init_width_t<2, 4, 32>();
     }
   } else if (RB == 4) {
     switch (variant) {
-      case 0: e = init_chain_t<4, 8, 128>(); res = resident_of<4, 8, 128>(); break;
-      case 1: e = init_chain_t<4, 4, 64>(); res = resident_of<4, 4, 64>(); break;
-      case 2: e = init_chain_t<4, 4, 32>(); res = resident_of<4, 4, 32>(); break;
+      case 0: return init_width_t<4, 8, 128>();
+      case 1: return init_width_t<4, 4, 64>();
+      case 2: return init_width_t<4, 4, 32>();
     }
   }
-  return e == hipSuccess ? res : 0;
+  return 0;
 }
 
 hipError_t launch_reduce(const int64_t *slot, const int *red_chunks, const I2 *red_range, const int64_t *sig_off, const int *sig_size,

@@ -81,6 +81,7 @@ struct ChainArgs {
   const int *tuples;           // tuple lists of explicit-subset programs (DProg::tup_off)
   int nchunks;
   int r, p, maxexp;
+  int maxb, pad1;              // largest block dimension of the problem
 };
 
 // One chain launch drains the chunks of several segments (plans, possibly of different problems
@@ -93,6 +94,8 @@ struct ChainBatch {
   int *queue;         // chunk counter (reset before each launch)
   double *scratch;    // per-workgroup cache of a shared prefix, scratch_stride doubles each
   int64_t scratch_stride;
+  int maxb = 12;      // largest block dimension of the segments' problems (selects the kernel instance)
+  int pad0 = 0;
 };
 
 // Edge-table work of one plan.

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <memory>
 #include <vector>
 
@@ -181,6 +182,13 @@ DOp device_op(const Op &op, const std::vector<EdgeTable> &tables, const xca_ctx 
 std::vector<DTabJob> tab_jobs(const std::vector<EdgeTable> &tables, const xca_ctx *c);
 void corr_release(xca_ctx *c);
 ChainArgs chain_args(xca_ctx *c, DevPlan *dp, DevVariant *V);
+int ctx_max_block(const xca_ctx *c);
+// largest block dimension over the segments of a chain launch
+inline int max_block_of(const std::vector<ChainArgs> &segs) {
+  int m = 1;
+  for (const auto &a : segs) m = std::max(m, a.maxb);
+  return m;
+}
 TabSeg tab_seg(xca_ctx *c, DevPlan *dp);
 void autotune(xca_ctx *c, DevPlan *dp);
 std::vector<std::pair<int, std::vector<int>>> connected_topologies(int order);
