@@ -292,7 +292,8 @@ class Batch {
       g->prefix.upload(g->h_prefix, s);
       if (qi >= queues_.size()) {
         queues_.push_back(std::make_unique<DevBuf<int>>());
-        queues_.back()->alloc(1);
+        queues_.back()->alloc(2);
+        HIP_CHECK(init_queue(queues_.back()->ptr, s));
       }
       const int64_t stride = 16LL * g->RB * tile_columns(g->variant);
       g->scratch.alloc((size_t)chain_resident(g->RB, g->variant) * stride);
@@ -510,7 +511,8 @@ static SigmaAll *sigma_all(xca_ctx *c, int max_order) {
     SigmaAll::Group &G = *gp;
     G.segs.upload(G.h_segs, s);
     G.prefix.upload(G.h_prefix, s);
-    G.queue.alloc(1);
+    G.queue.alloc(2);
+    HIP_CHECK(init_queue(G.queue.ptr, s));
     const int64_t stride = 16LL * G.RB * tile_columns(G.variant);
     G.scratch.alloc((size_t)chain_resident(G.RB, G.variant) * stride);
     G.B = ChainBatch{G.segs.ptr, G.prefix.ptr, (int)G.h_segs.size(), G.h_prefix.back(), G.queue.ptr, G.scratch.ptr, stride};

@@ -245,13 +245,15 @@ DOp device_op(const Op &op, const std::vector<EdgeTable> &tables, const xca_ctx 
   return o;
 }
 
-// edge-table jobs: entries of each table in runs that fill one tile
+// edge-table jobs: entries of each table in runs of at most 32 columns (64 for blocks above 4
+// states, one entry for blocks above 8), so that a table launch spreads over many workgroups
 std::vector<DTabJob> tab_jobs(const std::vector<EdgeTable> &tables, const xca_ctx *c) {
   const Problem &P = c->M.P;
   std::vector<DTabJob> out;
   for (const auto &tb : tables) {
     int b = P.dims[tb.block], bb = b * b;
-    int per = std::max(1, tile_columns(0) / bb);
+    if (bb > tile_columns(0)) throw std::invalid_argument("xca: block too large for the edge-table kernel");
+    int per = std::max(1, (bb <= 32 ? 32 : 64) / bb);
     for (int64_t e0 = 0; e0 < tb.nentries; e0 += per) {
       DTabJob j{};
       j.block = tb.block; j.b = b; j.nlines = tb.nlines; j.e0 = (int)e0;
@@ -474,21 +476,24 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
         }
       }
     }
-    // partial slots and the per-block reduction lists
+    // the per-block reduction lists, and the partial slots laid out block by block in reduction
+    // order: the slots of one Sigma block are contiguous (slot k of the block's list at its first
+    // slot + k r b0^2), which is how the reduction kernels address them
     int64_t off = 0;
     std::vector<std::vector<int>> per_block(P.nb);
     V->h_slot.resize(V->h_chunks.size());
-    for (size_t ci = 0; ci < V->h_chunks.size(); ++ci) {
-      const DProg &d = V->h_progs[V->h_chunks[ci].prog];
-      V->h_chunks[ci].slot = off;
-      V->h_slot[ci] = off;
-      off += (int64_t)r * d.b0 * d.b0;
-      per_block[d.sigma_block].push_back((int)ci);
-    }
+    for (size_t ci = 0; ci < V->h_chunks.size(); ++ci) per_block[V->h_progs[V->h_chunks[ci].prog].sigma_block].push_back((int)ci);
     for (int b = 0; b < P.nb; ++b) {
       I2 rg;
       rg.x = (int)V->h_red.size();
-      V->h_red.insert(V->h_red.end(), per_block[b].begin(), per_block[b].end());
+      for (int ci : per_block[b]) {
+        const DProg &d = V->h_progs[V->h_chunks[ci].prog];
+        if (d.b0 != P.dims[b]) throw std::logic_error("xca: program block dimension differs from its Sigma block");
+        V->h_chunks[ci].slot = off;
+        V->h_slot[ci] = off;
+        off += (int64_t)r * d.b0 * d.b0;
+        V->h_red.push_back(ci);
+      }
       rg.y = (int)V->h_red.size();
       V->h_red_range.push_back(rg);
     }
@@ -510,7 +515,8 @@ DevPlan *get_plan(xca_ctx *c, int m, const int *topo, const int32_t *fix, int64_
   dp->tuples.upload(dp->h_tuples.empty() ? std::vector<int>{0} : dp->h_tuples, s);
   dp->tabjobs.upload(dp->h_tab, s);
   dp->ftab.alloc(std::max<int64_t>(plan.ftab_size, 1));
-  dp->queue.alloc(1);
+  dp->queue.alloc(2);
+  HIP_CHECK(init_queue(dp->queue.ptr, s));
   dp->ntabjobs = (int)dp->h_tab.size();
   dp->device_bytes = (int64_t)(dp->ops.n * sizeof(DOp) + dp->tabjobs.n * sizeof(DTabJob) + dp->ftab.n * sizeof(double) + dp->tuples.n * sizeof(int));
   for (auto &V : dp->var)
@@ -831,7 +837,7 @@ int64_t ctx_bytes(const xca_ctx *c) {
   const int64_t RP = 16 * c->RB;
   return 8 * ((int64_t)c->M.opmats.size() + c->M.kp.size() + c->M.km.size() + c->M.delta.size() + c->M.delta_refl.size() +
               2 * c->maxexp * P.p + 2 * c->M.G_size + 13 * RP * RP) +
-         4 * (4 * P.p + 2 * P.nb);
+         4 * (4 * P.p) + 12 * P.nb;  // pole lists; Sigma block offsets (int64) and sizes (int)
 }
 
 // Upper bound of the device bytes a plan of this topology takes at the context's current zero-block
@@ -860,7 +866,10 @@ int64_t plan_bytes_estimate(xca_ctx *c, int m, const int *topo) {
     ps.push_back({ntup, std::max(1, pg.max_w), pg.b0});
     nops += (int64_t)pg.ops.size();
   }
-  int64_t bytes = 8 * plan.ftab_size + nops * (int64_t)sizeof(DOp) + (int64_t)(plan.tables.size() + 1) * 64 * (int64_t)sizeof(DTabJob);
+  // every device buffer holds at least one element (DevBuf), as get_plan allocates them
+  auto n1 = [](int64_t n) { return std::max<int64_t>(n, 1); };
+  int64_t bytes = 8 * n1(plan.ftab_size) + n1(nops) * (int64_t)sizeof(DOp) + n1((int64_t)tab_jobs(plan.tables, c).size()) * (int64_t)sizeof(DTabJob) +
+                  (int64_t)sizeof(int);  // the (empty) tuple list
   for (int v : cand) {
     const int NT = tile_columns(v);
     int64_t est_tiles = 0;
@@ -872,12 +881,15 @@ int64_t plan_bytes_estimate(xca_ctx *c, int m, const int *topo) {
     }
     if (!ok) continue;
     const int ct = chunk_tiles(est_tiles);
+    int64_t chunks = 0, partial = 0;
     for (auto &q : ps) {
       const int per = std::max(1, NT / q.max_w);
       const int64_t nt = (q.ntup + per - 1) / per, nch = (nt + ct - 1) / ct;
-      bytes += nch * (8 * (int64_t)P.r * q.b0 * q.b0 + (int64_t)sizeof(DChunk) + (int64_t)sizeof(int64_t) + (int64_t)sizeof(int));
+      chunks += nch;
+      partial += nch * (int64_t)P.r * q.b0 * q.b0;
     }
-    bytes += (int64_t)ps.size() * (int64_t)sizeof(DProg) + (int64_t)P.nb * (int64_t)sizeof(I2);
+    bytes += n1(chunks) * (int64_t)(sizeof(DChunk) + sizeof(int64_t) + sizeof(int)) + 8 * n1(partial) +
+             n1((int64_t)ps.size()) * (int64_t)sizeof(DProg) + n1(P.nb) * (int64_t)sizeof(I2);
   }
   return bytes;
 }

@@ -404,7 +404,8 @@ static CorrDevPlan *get_corr_plan(xca_ctx *c, int m, const int *topo, const int3
   dp->partial.alloc(std::max<int64_t>(max_partial, 1));
   dp->cidx.upload(h_cidx, s);
   dp->cval.alloc(std::max<size_t>(dp->cval_src.size(), 1));
-  dp->queue.alloc(1);
+  dp->queue.alloc(2);
+  HIP_CHECK(init_queue(dp->queue.ptr, s));
   dp->device_bytes += (int64_t)(dp->ops.n * sizeof(DOp) + dp->tabjobs.n * sizeof(DTabJob)) +
                       8 * (int64_t)(dp->ftab.n + dp->store.n + dp->partial.n + dp->cval.n) + 4 * (int64_t)dp->cidx.n;
   // launch descriptors

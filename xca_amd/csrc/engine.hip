@@ -90,25 +90,36 @@ __device__ __forceinline__ void store_acc(const v4d (&acc)[RB], double *S, int l
 
 // Row-partitioned GEMM: the wave computes row block rb of the product for NCB column blocks
 // cb0 .. cb0 + NCB - 1: acc[c] += sum_{ks < NK} A[rb][koff + ks] * S[4 ks + (lane >> 4)][16 (cb0 + c) + (lane & 15)].
-// One A fragment (global, prefetched two steps ahead) feeds NCB MFMAs; B fragments come from LDS.
+// The A fragments of the row block (global) are read in groups of four, the next group while the
+// current one is used (four to seven steps ahead), so that their latency is paid about once per
+// product however few column blocks are in use; B fragments come from LDS.
 template <int NCB>
 __device__ __forceinline__ void gemm_rows(const double *__restrict__ Afrag, int ldk, int rb, int koff, int NK, const double *S, int ldn,
                                           int cb0, int ncnt, v4d (&acc)[NCB], int lane) {
   if (ncnt <= 0) return;
   const int krow = lane >> 4, col = 16 * cb0 + (lane & 15);
   const double *Ap = Afrag + ((size_t)rb * ldk + koff) * 64 + lane;
-  double a0 = Ap[0], a1 = (NK > 1) ? Ap[64] : 0.0;
-#pragma unroll 2
-  for (int ks = 0; ks < NK; ++ks) {
-    const double an = (ks + 2 < NK) ? Ap[(size_t)(ks + 2) * 64] : 0.0;
-    double b[NCB];
+  double ac[4];
 #pragma unroll
-    for (int c = 0; c < NCB; ++c) b[c] = (c < ncnt) ? S[(size_t)(4 * ks + krow) * ldn + col + 16 * c] : 0.0;
+  for (int j = 0; j < 4; ++j) ac[j] = j < NK ? Ap[(size_t)j * 64] : 0.0;
+  for (int k0 = 0; k0 < NK; k0 += 4) {
+    double an[4];
 #pragma unroll
-    for (int c = 0; c < NCB; ++c)
-      if (c < ncnt) acc[c] = mfma_f64(a0, b[c], acc[c]);
-    a0 = a1;
-    a1 = an;
+    for (int j = 0; j < 4; ++j) an[j] = k0 + 4 + j < NK ? Ap[(size_t)(k0 + 4 + j) * 64] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int ks = k0 + j;
+      if (ks < NK) {
+        double bf[NCB];
+#pragma unroll
+        for (int c = 0; c < NCB; ++c) bf[c] = (c < ncnt) ? S[(size_t)(4 * ks + krow) * ldn + col + 16 * c] : 0.0;
+#pragma unroll
+        for (int c = 0; c < NCB; ++c)
+          if (c < ncnt) acc[c] = mfma_f64(ac[j], bf[c], acc[c]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) ac[j] = an[j];
   }
 }
 
@@ -223,6 +234,7 @@ __global__ void __launch_bounds__(64 * NW) tab_kernel(TabBatch B) {
   constexpr int RP = 16 * RB, NT = 64 * NW, LDN = 16 * NW + 8;
   extern __shared__ double lds[];
   double *SA = lds, *SB = lds + RP * LDN;
+  double *Ul = SB + RP * LDN, *Dl = Ul + RP * (RP - 1) / 2;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int sidx = find_segment(B.prefix, B.nsegs, blockIdx.x);
   const TabSeg &T = B.segs[sidx];
@@ -233,27 +245,61 @@ __global__ void __launch_bounds__(64 * NW) tab_kernel(TabBatch B) {
   const DTabJob J = T.jobs[blockIdx.x - B.prefix[sidx]];
   const int r = D.r, b = J.b, bb = b * b, ncol = J.ne * bb;
   for (int x = tid; x < RP * LDN; x += NT) { SA[x] = 0.0; SB[x] = 0.0; }
+  load_u<RB>(D, Ul, Dl, tid, NT);
   __syncthreads();
-  // GK(t) for each entry
-  for (int x = tid; x < r * ncol; x += NT) {
-    const int t = x / ncol, c = x % ncol, e = c / bb, ij = c % bb;
-    double v = G[J.g_off + (size_t)t * bb + (J.tr ? (ij % b) * b + ij / b : ij)];  // G_b or its transpose
-    int ee = J.e0 + e;
-    for (int q = 0; q < J.nlines; ++q) {
-      const int l = ee % p;
-      ee /= p;
-      v *= (J.lsign[q] > 0 ? kp : km)[(size_t)l * r + t];
+  // GK(t) for each entry: G_b (or its transpose) times the K factor of each line, four elements per
+  // thread with all their reads in flight together
+  const int n = r * ncol;
+  for (int x0 = 0; x0 < n; x0 += 4 * NT) {
+    double v[4], kf[4][kMaxLines];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int x = x0 + u * NT + tid;
+      const bool act = x < n;
+      const int t = act ? x / ncol : 0, c = act ? x % ncol : 0, e = c / bb, ij = c % bb;
+      v[u] = act ? G[J.g_off + (size_t)t * bb + (J.tr ? (ij % b) * b + ij / b : ij)] : 0.0;
+      int ee = J.e0 + e;
+#pragma unroll
+      for (int q = 0; q < kMaxLines; ++q) {
+        const int l = ee % p;
+        ee /= p;
+        kf[u][q] = (act && q < J.nlines) ? (J.lsign[q] > 0 ? kp : km)[(size_t)l * r + t] : 1.0;
+      }
     }
-    SA[(size_t)t * LDN + c] = v;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int x = x0 + u * NT + tid;
+      if (x < n) {
+        double y = v[u];
+#pragma unroll
+        for (int q = 0; q < kMaxLines; ++q)
+          if (q < J.nlines) y *= kf[u][q];
+        SA[(size_t)(x / ncol) * LDN + x % ncol] = y;
+      }
+    }
   }
   __syncthreads();
-  v4d acc[RB];
-  const int c0 = 16 * wv;
-  vals2coefs_block<RB>(D, SA, LDN, c0, acc, lane);
-  store_acc<RB>(acc, SB, LDN, c0, lane, r);
-  // f to global
-  {
-    const int col = c0 + (lane & 15);
+  // Row-partitioned GEMMs over the column blocks in use, as in the chain kernel: wave wv takes row
+  // block wv % RB of the column blocks of its group wv / RB. Then g = U^{-1} y one column block per
+  // wave (f written from the solved columns), and W g by the row-partitioned GEMM again.
+  constexpr int NGRP = NW / RB, NCB = RB;
+  static_assert(NW % RB == 0, "waves must tile the row blocks");
+  const int rbw = wv % RB, grp = wv / RB;
+  const int ncb = (ncol + 15) >> 4, cpg = (ncb + NGRP - 1) / NGRP, cbw = grp * cpg, ncw = min(cpg, ncb - cbw);
+  const int nk = (r + 3) >> 2;
+  {  // y = (L^{-1} P) GK -> SB
+    v4d acc[NCB];
+    zero_acc(acc);
+    gemm_rows<NCB>(D.LiP, 4 * RB, rbw, 0, nk, SA, LDN, cbw, ncw, acc, lane);
+    store_rows<NCB>(acc, SB, LDN, rbw, cbw, ncw, lane, r);
+  }
+  __syncthreads();
+  for (int cb = wv; cb < ncb; cb += NW) {  // f = U^{-1} y in place, and to global
+    v4d acc[RB];
+    load_acc<RB>(acc, SB, LDN, 16 * cb, lane);
+    usolve_lds<RB>(acc, Ul, Dl, r, lane);
+    store_acc<RB>(acc, SB, LDN, 16 * cb, lane, r);
+    const int col = 16 * cb + (lane & 15);
     if (col < ncol) {
       const int e = col / bb, ij = col % bb;
       double *out = ftab + J.out_off + (size_t)(J.e0 + e) * 2 * r * bb;
@@ -266,22 +312,23 @@ __global__ void __launch_bounds__(64 * NW) tab_kernel(TabBatch B) {
         }
     }
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  zero_acc(acc);
-  gemm_acc<RB>(D.W, 4 * RB, 0, (D.r + 3) >> 2, SB, LDN, c0, acc, lane);
-  {
-    const int col = c0 + (lane & 15);
-    if (col < ncol) {
-      const int e = col / bb, ij = col % bb;
-      double *out = ftab + J.out_off + (size_t)(J.e0 + e) * 2 * r * bb + (size_t)r * bb;
+  __syncthreads();
+  {  // W f -> global
+    v4d acc[NCB];
+    zero_acc(acc);
+    gemm_rows<NCB>(D.W, 4 * RB, rbw, 0, nk, SB, LDN, cbw, ncw, acc, lane);
 #pragma unroll
-      for (int rb = 0; rb < RB; ++rb)
+    for (int c = 0; c < NCB; ++c) {
+      const int col = 16 * (cbw + c) + (lane & 15);
+      if (c < ncw && col < ncol) {
+        const int e = col / bb, ij = col % bb;
+        double *out = ftab + J.out_off + (size_t)(J.e0 + e) * 2 * r * bb + (size_t)r * bb;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int t = acc_row(rb, v, lane);
-          if (t < r) out[(size_t)t * bb + ij] = acc[rb][v];
+          const int t = acc_row(rbw, v, lane);
+          if (t < r) out[(size_t)t * bb + ij] = acc[c][v];
         }
+      }
     }
   }
 }
@@ -825,6 +872,15 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
     (void)rows;
   }
   }
+  // The queue resets itself for the next launch: every workgroup has taken its last ticket before
+  // it counts itself out, so the last one out may clear both counters.
+  if (tid == 0) {
+    __threadfence();
+    if (atomicAdd(B.queue + 1, 1) == (int)gridDim.x - 1) {
+      B.queue[0] = 0;
+      B.queue[1] = 0;
+    }
+  }
 }
 
 // Sigma[block] += scale * sum over the block's chunks of the partial slots. Thread (e, g) of a
@@ -840,8 +896,12 @@ __global__ void __launch_bounds__(256) reduce_kernel(const int64_t *__restrict__
   const I2 rg = red_range[blk];
   const bool live = x < sig_size[blk];
   double s = 0.0;
-  if (live)
-    for (int c = rg.x + g; c < rg.y; c += 16) s += partial[slot[red_chunks[c]] + x];
+  if (live && rg.y > rg.x) {  // the block's slots are contiguous, sig_size[blk] doubles each
+    const double *base = partial + slot[red_chunks[rg.x]] + x;
+    const int64_t sz = sig_size[blk];
+#pragma unroll 4
+    for (int c = g; c < rg.y - rg.x; c += 16) s += base[(int64_t)c * sz];
+  }
   part[g][e] = s;
   __syncthreads();
   if (g == 0 && live && rg.y > rg.x) {
@@ -867,8 +927,12 @@ __global__ void __launch_bounds__(256) reduce_assign_kernel(RedBatch B) {
     const RedPlan &R = Q.plans[q];
     const I2 rg = R.red_range[blk];
     double s = 0.0;
-    if (live)
-      for (int c = rg.x + g; c < rg.y; c += 16) s += R.partial[R.slot[R.red_chunks[c]] + x];
+    if (live && rg.y > rg.x) {  // the block's slots are contiguous, sig_size[blk] doubles each
+      const double *base = R.partial + R.slot[R.red_chunks[rg.x]] + x;
+      const int64_t sz = Q.sig_size[blk];
+#pragma unroll 4
+      for (int c = g; c < rg.y - rg.x; c += 16) s += base[(int64_t)c * sz];
+    }
     __syncthreads();
     part[g][e] = s;
     __syncthreads();
@@ -882,8 +946,9 @@ __global__ void __launch_bounds__(256) reduce_assign_kernel(RedBatch B) {
   if (g == 0 && live) Q.sigma[Q.sig_off[blk] + x] = acc;
 }
 
+// chunk counter and exit counter of a persistent launch, cleared once when the queue is created
 __global__ void reset_queue_kernel(int *q) {
-  if (threadIdx.x == 0) q[0] = 0;
+  if (threadIdx.x < 2) q[threadIdx.x] = 0;
 }
 
 // One-time correlator contraction. Workgroup = chunk (pole tuples [k0, k1) of one group), wave = one
@@ -971,7 +1036,7 @@ static size_t lds_bytes() {
 template <int RB, int NW>
 static hipError_t launch_tab_t(const TabBatch &B, hipStream_t s) {
   constexpr int RP = 16 * RB, LDN = 16 * NW + 8;
-  const size_t sh = sizeof(double) * ((size_t)2 * RP * LDN + 64 * NW);
+  const size_t sh = sizeof(double) * ((size_t)2 * RP * LDN + RP * (RP - 1) / 2 + RP * 4);
   static bool ready = false;  // attribute set once, outside any stream capture after the first launch
   if (!ready) {
     hipError_t e = hipFuncSetAttribute((const void *)tab_kernel<RB, NW>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh);
@@ -1024,7 +1089,6 @@ static hipError_t launch_chain_t(const ChainBatch &B, hipStream_t s) {
   const size_t sh = lds_bytes<RB, NW, NTILE>();
   if (B.total <= 0) return hipSuccess;
   if (B.maxb < 1 || B.maxb > kMaxB) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, s, B.queue);
   if (B.maxb <= 4) {
     const int grid = std::min(B.total, resident_of<RB, NW, NTILE, 4>());
     hipLaunchKernelGGL((chain_kernel<RB, NW, NTILE, 4>), dim3(grid), dim3(64 * NW), sh, s, B);
@@ -1104,6 +1168,11 @@ hipError_t launch_reduce_assign(const RedBatch &B, int max_size, hipStream_t s) 
 }
 
 size_t chain_args_size() { return sizeof(ChainArgs); }
+
+hipError_t init_queue(int *q, hipStream_t s) {
+  hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, s, q);
+  return hipGetLastError();
+}
 
 template <int RB, int NW>
 static hipError_t launch_refl_t(const ReflArgs &A, hipStream_t s) {

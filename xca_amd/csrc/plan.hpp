@@ -16,6 +16,9 @@ int num_tile_variants();
 int tile_columns(int variant);
 hipError_t prepare_kernels(int RB, hipStream_t s);
 hipError_t launch_tab(int RB, const TabBatch &B, hipStream_t s);
+// a chain launch's queue: two ints (chunk counter, exit counter), zero before the first launch;
+// every launch leaves them zero again
+hipError_t init_queue(int *q, hipStream_t s);
 hipError_t launch_chain(int RB, int variant, const ChainBatch &B, hipStream_t s);
 // workgroups of one chain launch resident on the device at once (its grid never exceeds this)
 int chain_resident(int RB, int variant);
