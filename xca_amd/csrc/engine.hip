@@ -476,9 +476,15 @@ __device__ __forceinline__ void replicate_items(double *S, int ldn, int r, int w
   }
 }
 
-// MB: largest block of the launch's programs (register arrays of operator and table rows)
+// MB: largest block of the launch's programs (register arrays of operator and table rows).
+// The U factor sits in LDS except in the 64-column, four-wave tile, which leaves it in global
+// memory so that two workgroups fit one CU (LDS and registers): while one waits at a barrier, the
+// other's waves issue.
+template <int RB, int NW, int NTILE>
+constexpr bool chain_u_in_lds() { return !(NW == 4 && NTILE == 64); }
+
 template <int RB, int NW, int NTILE, int MB>
-__global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
+__global__ void __launch_bounds__(64 * NW, (chain_u_in_lds<RB, NW, NTILE>() ? 1 : 2)) chain_kernel(ChainBatch B) {
   constexpr int RP = 16 * RB, NT = 64 * NW, LDN = NTILE + 4;
   constexpr int NCBT = NTILE / 16;           // 16-column blocks in a tile
   constexpr int NCB = NCBT * RB / NW;        // column blocks per wave in the row-partitioned GEMMs
@@ -488,8 +494,9 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
   extern __shared__ double lds[];
   __shared__ int qnext;
   double *S0 = lds, *S1 = lds + RP * LDN;
+  constexpr bool ULDS = chain_u_in_lds<RB, NW, NTILE>();
   double *Ul = lds + 2 * RP * LDN, *Dl = Ul + RP * (RP - 1) / 2;
-  double *ipref = Dl + RP * 4;                                 // per item prefactor (NTILE)
+  double *ipref = ULDS ? Dl + RP * 4 : Ul;                     // per item prefactor (NTILE)
   unsigned long long *imax = (unsigned long long *)(ipref + NTILE);  // per item max |T|
   int *ipole = (int *)(imax + NTILE);                          // (NTILE, kMaxLines)
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -512,7 +519,7 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
   if (D.LiP != cur) {
     // another problem: clean slabs (the padding rows start at its own rank) and its U factor
     for (int x = tid; x < RP * LDN; x += NT) { S0[x] = 0.0; S1[x] = 0.0; }
-    load_u<RB>(D, Ul, Dl, tid, NT);
+    if (ULDS) load_u<RB>(D, Ul, Dl, tid, NT);
     cur = D.LiP;
     __syncthreads();
   }
@@ -674,7 +681,8 @@ __global__ void __launch_bounds__(64 * NW) chain_kernel(ChainBatch B) {
         for (int cb = wv; cb < ncb; cb += NW) {  // g = U^{-1} y in place, one column block per wave at a time
           v4d acc[RB];
           load_acc<RB>(acc, Soth, LDN, 16 * cb, lane);
-          usolve_lds<RB>(acc, Ul, Dl, r, lane);
+          if (ULDS) usolve_lds<RB>(acc, Ul, Dl, r, lane);
+          else usolve<RB>(acc, D, lane);
           store_acc<RB>(acc, Soth, LDN, 16 * cb, lane, r);
         }
         __syncthreads();
@@ -1030,7 +1038,8 @@ __global__ void __launch_bounds__(256) corr_reduce_kernel(const double *__restri
 template <int RB, int NW, int NTILE>
 static size_t lds_bytes() {
   constexpr int RP = 16 * RB, LDN = NTILE + 4;
-  return sizeof(double) * ((size_t)2 * RP * LDN + RP * (RP - 1) / 2 + RP * 4 + NTILE + NTILE) + sizeof(int) * NTILE * kMaxLines;
+  constexpr size_t u = chain_u_in_lds<RB, NW, NTILE>() ? (size_t)RP * (RP - 1) / 2 + RP * 4 : 0;
+  return sizeof(double) * ((size_t)2 * RP * LDN + u + NTILE + NTILE) + sizeof(int) * NTILE * kMaxLines;
 }
 
 template <int RB, int NW>
