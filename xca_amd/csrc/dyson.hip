@@ -91,7 +91,7 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ v4d mfma16x16x4(double a, double b, v4d c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
-constexpr int kDysonThreads = 1024;
+constexpr int kDysonThreads = 512;
 constexpr int kDysonMaxNp = 784;  // padded r n: DLR rank <= 64, blocks <= 12
 constexpr int kMaxRank = 64;      // DLR rank limit of the device path (checked at context creation)
 constexpr int kLdsSysMax = kDysonLdsSysMax;
@@ -103,7 +103,7 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// Dyson solve of one block per workgroup (1024 threads), in the coefficient form of
+// Dyson solve of one block per workgroup (512 threads), in the coefficient form of
 // dyson_it_ppsc::solve (dlr_dyson_ppsc.hpp:135-168):
 //   sigc = vals2coefs(Sigma_b), ggs = -(convolve_TO(beta, g0c, sigc) + eta G0_b), f = vals2coefs(ggs)
 //   (I + convmat(f)) G = G0  with  convmat(f) = Mc cf2it^{-1}  is solved as
@@ -123,6 +123,7 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
   __shared__ double s_lu[kMaxRank * kMaxRank];  // LU factors of cf2it
   __shared__ int s_piv[kMaxRank];
   __shared__ double s_dg[16 * 16];               // diagonal block of the substitution
+  __shared__ int s_ip[kDysonMaxNp];              // row interchanges of the factorisation
   // dynamic LDS: [the system (np_max^2 doubles), when np_max <= kLdsSysMax] then np_max x 16 doubles
   // for the LU panel ((Np - k0) rows x 16 columns), the r x n^2 columns of a vals2coefs, or the
   // Np x n right-hand sides
@@ -249,33 +250,41 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
     for (int x = tid; x < m * 16; x += NT) Pn[x] = S[(size_t)(k0 + (x >> 4)) * Np + k0 + (x & 15)];
     __syncthreads();
     if (wv == 0) {  // the panel on one wave: no workgroup barrier per column
+#pragma unroll
       for (int jj = 0; jj < 16; ++jj) {
         // pivot: first row of maximal |P[i][jj]|, i >= jj (lane scan, then a DPP reduction)
         double best = -1.0;
         int bi = Np;
+#pragma unroll 4
         for (int i = jj + lane; i < m; i += 64) {
           const double v = fabs(Pn[i * 16 + jj]);
           if (v > best) { best = v; bi = i; }
         }
         wave_argmax(best, bi, Np);
         const int p = bi;
-        if (lane == 0) ip[k0 + jj] = k0 + p;
+        if (lane == 0) s_ip[k0 + jj] = k0 + p;
         if (p != jj && lane < 16) {
           const double a = Pn[jj * 16 + lane];
           Pn[jj * 16 + lane] = Pn[p * 16 + lane];
           Pn[p * 16 + lane] = a;
         }
         wave_lds_sync();
-        const double pv = Pn[jj * 16 + jj];
         double prow[16];
 #pragma unroll
-        for (int c = 0; c < 16; ++c) prow[c] = Pn[jj * 16 + c];
+        for (int c = jj; c < 16; ++c) prow[c] = Pn[jj * 16 + c];
+        const double pv = prow[jj];
+        // each row's entries jj..15 read together, updated in registers, written back
+#pragma unroll 2
         for (int i = jj + 1 + lane; i < m; i += 64) {
-          const double l = Pn[i * 16 + jj] / pv;
-          Pn[i * 16 + jj] = l;
+          double v[16];
 #pragma unroll
-          for (int c = 0; c < 16; ++c)
-            if (c > jj) Pn[i * 16 + c] -= l * prow[c];
+          for (int c = jj; c < 16; ++c) v[c] = Pn[i * 16 + c];
+          const double l = v[jj] / pv;
+          v[jj] = l;
+#pragma unroll
+          for (int c = jj + 1; c < 16; ++c) v[c] -= l * prow[c];
+#pragma unroll
+          for (int c = jj; c < 16; ++c) Pn[i * 16 + c] = v[c];
         }
         wave_lds_sync();
       }
@@ -288,7 +297,7 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
       const int col = c < k0 ? c : c + 16;
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
-        const int pr = ip[k0 + q];
+        const int pr = s_ip[k0 + q];
         if (pr != k0 + q) {
           const double a = S[(size_t)(k0 + q) * Np + col];
           S[(size_t)(k0 + q) * Np + col] = S[(size_t)pr * Np + col];
@@ -338,10 +347,10 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
   __syncthreads();
   for (int col = tid; col < n; col += NT)
     for (int k = 0; k < Np; ++k)
-      if (ip[k] != k) {
+      if (s_ip[k] != k) {
         const double a = Y[(size_t)k * n + col];
-        Y[(size_t)k * n + col] = Y[(size_t)ip[k] * n + col];
-        Y[(size_t)ip[k] * n + col] = a;
+        Y[(size_t)k * n + col] = Y[(size_t)s_ip[k] * n + col];
+        Y[(size_t)s_ip[k] * n + col] = a;
       }
   __syncthreads();
   // forward substitution with the unit lower factor, 16 rows at a time (diagonal block in LDS)
@@ -391,6 +400,7 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
     }
     __syncthreads();
   }
+  for (int k = tid; k < Np; k += NT) ip[k] = s_ip[k];  // the pivots, kept for inspection
   // ---- G[t][i][j] = sum_k cf2it(t,k) g[(k,i)][j], hermitised
   double *Gout = A.gout + go;
   for (int x = tid; x < r * nn; x += NT) {

@@ -490,7 +490,7 @@ __global__ void __launch_bounds__(64 * NW, (chain_u_in_lds<RB, NW, NTILE>() ? 1 
   constexpr int NCB = NCBT * RB / NW;        // column blocks per wave in the row-partitioned GEMMs
   static_assert(NW % RB == 0 && (NCBT * RB) % NW == 0 && NCB >= 1, "waves must tile the (row, column) blocks");
   constexpr bool PF = MB <= 6;               // table rows read a round ahead (register budget)
-  constexpr int VU = MB <= 6 ? 2 : 1;        // vertex rows per pass
+  constexpr int VU = 2;                      // vertex rows per pass
   extern __shared__ double lds[];
   __shared__ int qnext;
   double *S0 = lds, *S1 = lds + RP * LDN;
@@ -621,12 +621,20 @@ __global__ void __launch_bounds__(64 * NW, (chain_u_in_lds<RB, NW, NTILE>() ? 1 
             if (x0 + u * NT >= n) break;
             const double *Tin = Scur + (size_t)tt[u] * LDN + itt[u] * win;
             double *Tout = Soth + (size_t)tt[u] * LDN + itt[u] * wout + ii[u] * cols;
-            for (int j = 0; j < cols; ++j) {
-              double s = 0.0;
+            for (int j0 = 0; j0 < cols; j0 += MB) {  // MB independent column sums at a time
+              double s[MB];
+#pragma unroll
+              for (int j = 0; j < MB; ++j) s[j] = 0.0;
 #pragma unroll
               for (int k = 0; k < MB; ++k)
-                if (k < bin) s += fr[u][k] * Tin[k * cols + j];
-              Tout[j] = kf[u] * s;
+                if (k < bin) {
+#pragma unroll
+                  for (int j = 0; j < MB; ++j)
+                    if (j0 + j < cols) s[j] += fr[u][k] * Tin[k * cols + j0 + j];
+                }
+#pragma unroll
+              for (int j = 0; j < MB; ++j)
+                if (j0 + j < cols) Tout[j0 + j] = kf[u] * s[j];
             }
           }
         }
