@@ -103,6 +103,122 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// One 16-column panel of the blocked LU (rows k0..Np-1 of columns k0..k0+15, m = Np - k0 rows of 16
+// in LDS), factorised by one wave with partial pivoting: for each column, the first row of maximal
+// modulus at or below the diagonal (lane scan, then a DPP reduction) is interchanged with the
+// diagonal row, then every row below is updated, its entries read together into registers and
+// written back. ipk[jj] receives the absolute pivot row k0 + p (LDS).
+__device__ __forceinline__ void panel_factor(double *Pn, int m, int k0, int Np, int *ipk, int lane) {
+#pragma unroll
+  for (int jj = 0; jj < 16; ++jj) {
+    double best = -1.0;
+    int bi = Np;
+#pragma unroll 4
+    for (int i = jj + lane; i < m; i += 64) {
+      const double v = fabs(Pn[i * 16 + jj]);
+      if (v > best) { best = v; bi = i; }
+    }
+    wave_argmax(best, bi, Np);
+    const int p = bi;
+    if (lane == 0) ipk[jj] = k0 + p;
+    if (p != jj && lane < 16) {
+      const double a = Pn[jj * 16 + lane];
+      Pn[jj * 16 + lane] = Pn[p * 16 + lane];
+      Pn[p * 16 + lane] = a;
+    }
+    wave_lds_sync();
+    double prow[16];
+#pragma unroll
+    for (int c = jj; c < 16; ++c) prow[c] = Pn[jj * 16 + c];
+    const double pv = prow[jj];
+#pragma unroll 2
+    for (int i = jj + 1 + lane; i < m; i += 64) {
+      double v[16];
+#pragma unroll
+      for (int c = jj; c < 16; ++c) v[c] = Pn[i * 16 + c];
+      const double l = v[jj] / pv;
+      v[jj] = l;
+#pragma unroll
+      for (int c = jj + 1; c < 16; ++c) v[c] -= l * prow[c];
+#pragma unroll
+      for (int c = jj; c < 16; ++c) Pn[i * 16 + c] = v[c];
+    }
+    wave_lds_sync();
+  }
+}
+
+// The row interchanges of the panel at k0 (absolute rows ipk[0..15], in order) applied to column col
+// of the system, and, right of the panel, U12 = L11^{-1} A12 for that column (L11 unit lower, its
+// rows 16 doubles apart at L11).
+__device__ __forceinline__ void panel_column(double *S, int Np, int k0, int col, const int *ipk, const double *L11) {
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int pr = ipk[q];
+    if (pr != k0 + q) {
+      const double a = S[(size_t)(k0 + q) * Np + col];
+      S[(size_t)(k0 + q) * Np + col] = S[(size_t)pr * Np + col];
+      S[(size_t)pr * Np + col] = a;
+    }
+  }
+  if (col >= k0 + 16) {
+    double x[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) x[q] = S[(size_t)(k0 + q) * Np + col];
+#pragma unroll
+    for (int q = 1; q < 16; ++q) {
+      double sacc = x[q];
+#pragma unroll
+      for (int q0 = 0; q0 < 16; ++q0)
+        if (q0 < q) sacc -= L11[q * 16 + q0] * x[q0];
+      x[q] = sacc;
+    }
+#pragma unroll
+    for (int q = 1; q < 16; ++q) S[(size_t)(k0 + q) * Np + col] = x[q];
+  }
+}
+
+// A22 -= L21 U12 on the 16 x 16 tile at (ti, tj) in four MFMA steps; row ti + i of L21, panel column
+// c, is Lrow[i * ldl + c].
+__device__ __forceinline__ void trailing_tile(double *S, int Np, int k0, int ti, int tj, const double *Lrow, int ldl, int lane) {
+  v4d acc;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) acc[v] = S[(size_t)(ti + 4 * v + (lane >> 4)) * Np + tj + (lane & 15)];
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks) {
+    const double a = -Lrow[(size_t)(lane & 15) * ldl + 4 * ks + (lane >> 4)];
+    const double bb = S[(size_t)(k0 + 4 * ks + (lane >> 4)) * Np + tj + (lane & 15)];
+    acc = mfma16x16x4(a, bb, acc);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) S[(size_t)(ti + 4 * v + (lane >> 4)) * Np + tj + (lane & 15)] = acc[v];
+}
+
+// P1 = A1 B1, and P2 = A2 B2 when TWO, for the r x nc products on 16 x 16 FP64 MFMA tiles (one wave
+// per tile at a time, K = r in steps of 4): the A factors r x r row-major in global memory, the B
+// factors r x nc with rows ldb apart. epi(t, c, p1, p2) receives each product entry with t < r, c < nc.
+template <bool TWO, class Epi>
+__device__ __forceinline__ void rr_gemm(const double *A1, const double *B1, const double *A2, const double *B2, int ldb, int r, int nc, int wv,
+                                        int nwv, int lane, Epi epi) {
+  const int tr = (r + 15) >> 4, tcn = (nc + 15) >> 4, nks = (r + 3) >> 2;
+  for (int q = wv; q < tr * tcn; q += nwv) {
+    const int t0 = 16 * (q / tcn), c0 = 16 * (q % tcn);
+    const int ta = t0 + (lane & 15), cb = c0 + (lane & 15);
+    v4d p1 = {0.0, 0.0, 0.0, 0.0}, p2 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int ks = 0; ks < nks; ++ks) {
+      const int k = 4 * ks + (lane >> 4);
+      const bool ak = ta < r && k < r, bk = cb < nc && k < r;
+      p1 = mfma16x16x4(ak ? A1[ta * r + k] : 0.0, bk ? B1[(size_t)k * ldb + cb] : 0.0, p1);
+      if (TWO) p2 = mfma16x16x4(ak ? A2[ta * r + k] : 0.0, bk ? B2[(size_t)k * ldb + cb] : 0.0, p2);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int t = t0 + 4 * v + (lane >> 4);
+      if (t < r && cb < nc) epi(t, cb, p1[v], p2[v]);
+    }
+  }
+}
+
 // Dyson solve of one block per workgroup (512 threads), in the coefficient form of
 // dyson_it_ppsc::solve (dlr_dyson_ppsc.hpp:135-168):
 //   sigc = vals2coefs(Sigma_b), ggs = -(convolve_TO(beta, g0c, sigc) + eta G0_b), f = vals2coefs(ggs)
@@ -143,6 +259,10 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
   double *S = sys_lds ? lds_dyn : A.sys + A.soff[b];  // Np x Np system, then its LU factors
   double *B = A.rhs + A.roff[b];
   int *ip = A.ipiv + A.poff[b];
+  if (Bt.mode == 2) {  // the system was factorised across workgroups: its pivots
+    for (int k = tid; k < Np; k += NT) s_ip[k] = ip[k];
+    __syncthreads();
+  } else {
   for (int x = tid; x < r * r; x += NT) s_lu[x] = A.lu[x];
   for (int x = tid; x < r; x += NT) s_piv[x] = A.piv[x];
   // ---- sigc = vals2coefs(Sigma_b), the columns solved in LDS (r n^2 <= r n 16 <= np_max 16)
@@ -154,14 +274,9 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
     if (lane < r) X[(size_t)lane * nn + c] = v;
   }
   __syncthreads();
-  // ---- W sigc -> wf (scratch)
-  for (int x = tid; x < r * nn; x += NT) {
-    const int t = x / nn, c = x % nn;
-    double s = 0.0;
-#pragma unroll 8
-    for (int k = 0; k < r; ++k) s += A.W[t * r + k] * X[(size_t)k * nn + c];  // X holds sigc
-    wf[x] = s;
-  }
+  // ---- W sigc -> wf (scratch), X holds sigc
+  rr_gemm<false>(A.W, X, nullptr, nullptr, nn, r, nn, wv, NWV, lane,
+                 [&](int t, int c, double p1, double) { wf[(size_t)t * nn + c] = p1; });
   __syncthreads();
   // ---- node products: c = -(Wg0c sigc + g0c Wsig) -> ggs (temporarily), d = g0c sigc -> B (scratch)
   for (int x = tid; x < r * nn; x += NT) {
@@ -177,16 +292,9 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
   }
   __syncthreads();
   // ---- ggs = -(beta (cf2it c + tcf2it d) + eta G0) -> wf, then copied to ggs
-  for (int x = tid; x < r * nn; x += NT) {
-    const int t = x / nn, c = x % nn;
-    double h1 = 0.0, h2 = 0.0;
-#pragma unroll 8
-    for (int k = 0; k < r; ++k) {
-      h1 += A.cf2it[t * r + k] * ggs[(size_t)k * nn + c];
-      h2 += A.tcf2it[t * r + k] * B[(size_t)k * nn + c];
-    }
-    wf[x] = -(A.beta * (h1 + h2) + eta * G0[x]);
-  }
+  rr_gemm<true>(A.cf2it, ggs, A.tcf2it, B, nn, r, nn, wv, NWV, lane, [&](int t, int c, double h1, double h2) {
+    wf[(size_t)t * nn + c] = -(A.beta * (h1 + h2) + eta * G0[(size_t)t * nn + c]);
+  });
   __syncthreads();
   for (int x = tid; x < r * nn; x += NT) ggs[x] = wf[x];
   __syncthreads();
@@ -198,14 +306,9 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
     if (lane < r) X[(size_t)lane * nn + c] = v;
   }
   __syncthreads();
-  for (int x = tid; x < r * nn; x += NT) {
-    const int t = x / nn, c = x % nn;
-    double s = 0.0;
-#pragma unroll 8
-    for (int k = 0; k < r; ++k) s += A.W[t * r + k] * X[(size_t)k * nn + c];
-    wf[x] = s;
-    ggs[x] = X[x];
-  }
+  rr_gemm<false>(A.W, X, nullptr, nullptr, nn, r, nn, wv, NWV, lane,
+                 [&](int t, int c, double p1, double) { wf[(size_t)t * nn + c] = p1; });
+  for (int x = tid; x < r * nn; x += NT) ggs[x] = X[x];
   // the LU factors are done with: cf2it takes their LDS copy for the system (X keeps f)
   double *s_cf = s_lu;
   __syncthreads();
@@ -213,35 +316,55 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
   __syncthreads();
   // ---- system cf2it x I + Mc, padded with identity to Np x Np. The sum over k of Mc is, for each
   // orbital pair (i, i'), the product (cf2it diag(f_{i i'})) W: 16 x 16 MFMA tiles over (t, k').
-  for (int x = tid; x < Np * Np; x += NT) {
-    const int row = x / Np, col = x % Np;
-    if (row >= N || col >= N) S[x] = (row == col) ? 1.0 : 0.0;
+  for (int x = tid; x < (Np - N) * Np; x += NT) {  // padding rows
+    const int row = N + x / Np, col = x % Np;
+    S[(size_t)row * Np + col] = (row == col) ? 1.0 : 0.0;
   }
+  for (int x = tid; x < N * (Np - N); x += NT) S[(size_t)(x / (Np - N)) * Np + N + x % (Np - N)] = 0.0;  // padding columns
   {
-    const int RBd = (r + 15) >> 4, nks = (r + 3) >> 2;
-    for (int q = wv; q < nn * RBd * RBd; q += NWV) {
-      const int c = q / (RBd * RBd), tile = q % (RBd * RBd), i = c / n, ipp = c % n;
+    // a wave takes one (t, k') tile and a run of orbital pairs: the tile's W fragments and tcf2it
+    // entries are read once for the run (runs sized so that every wave has a task)
+    const int RBd = (r + 15) >> 4, nks = (r + 3) >> 2, ntile = RBd * RBd;
+    const int run = max(1, ntile * nn / NWV), nrun = (nn + run - 1) / run;
+    for (int q = wv; q < ntile * nrun; q += NWV) {
+      const int tile = q % ntile, c0 = (q / ntile) * run, c1 = min(nn, c0 + run);
       const int t0 = 16 * (tile / RBd), kp0 = 16 * (tile % RBd);
       const int ta = t0 + (lane & 15), kpb = kp0 + (lane & 15);
-      v4d acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-      for (int ks = 0; ks < nks; ++ks) {
+      double bw[kMaxRank / 4], tc[4];
+#pragma unroll
+      for (int ks = 0; ks < kMaxRank / 4; ++ks) {
         const int k = 4 * ks + (lane >> 4);
-        const double a = (ta < r && k < r) ? s_cf[ta * r + k] * X[(size_t)k * nn + c] : 0.0;
-        const double bb = (k < r && kpb < r) ? A.W[k * r + kpb] : 0.0;
-        acc = mfma16x16x4(a, bb, acc);
+        bw[ks] = (ks < nks && k < r && kpb < r) ? A.W[k * r + kpb] : 0.0;
       }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int t = t0 + 4 * v + (lane >> 4), kp = kpb;
-        if (t < r && kp < r) {
-          const double sv = -s_cf[t * r + kp] * wf[(size_t)kp * nn + c] + A.tcf2it[t * r + kp] * X[(size_t)kp * nn + c] - acc[v];
-          S[(size_t)(t * n + i) * Np + kp * n + ipp] = A.beta * sv + (i == ipp ? s_cf[t * r + kp] : 0.0);
+        const int t = t0 + 4 * v + (lane >> 4);
+        tc[v] = (t < r && kpb < r) ? A.tcf2it[t * r + kpb] : 0.0;
+      }
+      for (int c = c0; c < c1; ++c) {
+        const int i = c / n, ipp = c % n;
+        v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < kMaxRank / 4; ++ks) {
+          if (ks < nks) {
+            const int k = 4 * ks + (lane >> 4);
+            const double a = (ta < r && k < r) ? s_cf[ta * r + k] * X[(size_t)k * nn + c] : 0.0;
+            acc = mfma16x16x4(a, bw[ks], acc);
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int t = t0 + 4 * v + (lane >> 4), kp = kpb;
+          if (t < r && kp < r) {
+            const double sv = -s_cf[t * r + kp] * wf[(size_t)kp * nn + c] + tc[v] * X[(size_t)kp * nn + c] - acc[v];
+            S[(size_t)(t * n + i) * Np + kp * n + ipp] = A.beta * sv + (i == ipp ? s_cf[t * r + kp] : 0.0);
+          }
         }
       }
     }
   }
   __syncthreads();
+  if (Bt.mode == 1) return;  // the workgroup-wide LU launches follow (lu_panel_kernel, lu_update_kernel)
   // ---- blocked LU with partial pivoting. Each 16-column panel is factorised in LDS (rows k0..Np-1),
   // written back, its row interchanges applied to the other columns together with U12 = L11^{-1} A12
   // (one column per thread), then A22 -= L21 U12 on MFMA tiles with L21 read from LDS.
@@ -249,96 +372,22 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
     const int m = Np - k0;
     for (int x = tid; x < m * 16; x += NT) Pn[x] = S[(size_t)(k0 + (x >> 4)) * Np + k0 + (x & 15)];
     __syncthreads();
-    if (wv == 0) {  // the panel on one wave: no workgroup barrier per column
-#pragma unroll
-      for (int jj = 0; jj < 16; ++jj) {
-        // pivot: first row of maximal |P[i][jj]|, i >= jj (lane scan, then a DPP reduction)
-        double best = -1.0;
-        int bi = Np;
-#pragma unroll 4
-        for (int i = jj + lane; i < m; i += 64) {
-          const double v = fabs(Pn[i * 16 + jj]);
-          if (v > best) { best = v; bi = i; }
-        }
-        wave_argmax(best, bi, Np);
-        const int p = bi;
-        if (lane == 0) s_ip[k0 + jj] = k0 + p;
-        if (p != jj && lane < 16) {
-          const double a = Pn[jj * 16 + lane];
-          Pn[jj * 16 + lane] = Pn[p * 16 + lane];
-          Pn[p * 16 + lane] = a;
-        }
-        wave_lds_sync();
-        double prow[16];
-#pragma unroll
-        for (int c = jj; c < 16; ++c) prow[c] = Pn[jj * 16 + c];
-        const double pv = prow[jj];
-        // each row's entries jj..15 read together, updated in registers, written back
-#pragma unroll 2
-        for (int i = jj + 1 + lane; i < m; i += 64) {
-          double v[16];
-#pragma unroll
-          for (int c = jj; c < 16; ++c) v[c] = Pn[i * 16 + c];
-          const double l = v[jj] / pv;
-          v[jj] = l;
-#pragma unroll
-          for (int c = jj + 1; c < 16; ++c) v[c] -= l * prow[c];
-#pragma unroll
-          for (int c = jj; c < 16; ++c) Pn[i * 16 + c] = v[c];
-        }
-        wave_lds_sync();
-      }
-    }
+    if (wv == 0) panel_factor(Pn, m, k0, Np, s_ip + k0, lane);  // one wave: no workgroup barrier per column
     __syncthreads();
     for (int x = tid; x < m * 16; x += NT) S[(size_t)(k0 + (x >> 4)) * Np + k0 + (x & 15)] = Pn[x];
     // the panel's row interchanges on every other column (in order, per column), and U12
-    const int c1 = k0 + 16;
-    for (int c = tid; c < Np - 16; c += NT) {
-      const int col = c < k0 ? c : c + 16;
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int pr = s_ip[k0 + q];
-        if (pr != k0 + q) {
-          const double a = S[(size_t)(k0 + q) * Np + col];
-          S[(size_t)(k0 + q) * Np + col] = S[(size_t)pr * Np + col];
-          S[(size_t)pr * Np + col] = a;
-        }
-      }
-      if (col >= c1) {
-        double x[16];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) x[q] = S[(size_t)(k0 + q) * Np + col];
-#pragma unroll
-        for (int q = 1; q < 16; ++q) {
-          double sacc = x[q];
-#pragma unroll
-          for (int q0 = 0; q0 < 16; ++q0)
-            if (q0 < q) sacc -= Pn[q * 16 + q0] * x[q0];
-          x[q] = sacc;
-        }
-#pragma unroll
-        for (int q = 1; q < 16; ++q) S[(size_t)(k0 + q) * Np + col] = x[q];
-      }
-    }
+    for (int c = tid; c < Np - 16; c += NT) panel_column(S, Np, k0, c < k0 ? c : c + 16, s_ip + k0, Pn);
     __syncthreads();
+    const int c1 = k0 + 16;
     if (c1 >= Np) break;
-    // A22 -= L21 U12 on 16 x 16 MFMA tiles, one tile per wave at a time
+    // A22 -= L21 U12 on 16 x 16 MFMA tiles, one tile per wave at a time, L21 read from LDS
     const int nt = (Np - c1) >> 4;
     for (int q = wv; q < nt * nt; q += NWV) {
       const int ti = c1 + 16 * (q / nt), tj = c1 + 16 * (q % nt);
-      v4d acc;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) acc[v] = S[(size_t)(ti + 4 * v + (lane >> 4)) * Np + tj + (lane & 15)];
-#pragma unroll
-      for (int ks = 0; ks < 4; ++ks) {
-        const double a = -Pn[(ti - k0 + (lane & 15)) * 16 + 4 * ks + (lane >> 4)];
-        const double bb = S[(size_t)(k0 + 4 * ks + (lane >> 4)) * Np + tj + (lane & 15)];
-        acc = mfma16x16x4(a, bb, acc);
-      }
-#pragma unroll
-      for (int v = 0; v < 4; ++v) S[(size_t)(ti + 4 * v + (lane >> 4)) * Np + tj + (lane & 15)] = acc[v];
+      trailing_tile(S, Np, k0, ti, tj, Pn + (size_t)(ti - k0) * 16, 16, lane);
     }
     __syncthreads();
+  }
   }
   // ---- right-hand sides in LDS (Np n <= np_max 16): Y[(t,i)][j] = G0[t][i][j], padding rows zero;
   // row interchanges in order
@@ -368,11 +417,16 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
       if (lane < 16) Y[(size_t)(k0 + lane) * n + col] = v;
     }
     __syncthreads();
-    for (int x = tid; x < (Np - k0 - 16) * n; x += NT) {
-      const int i = k0 + 16 + x / n, col = x % n;
-      double s = Y[(size_t)i * n + col];
-      for (int q = 0; q < 16; ++q) s -= S[(size_t)i * Np + k0 + q] * Y[(size_t)(k0 + q) * n + col];
-      Y[(size_t)i * n + col] = s;
+    for (int i = k0 + 16 + tid; i < Np; i += NT) {  // one row per thread, its 16 factor entries read together
+      double l[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) l[q] = S[(size_t)i * Np + k0 + q];
+      for (int col = 0; col < n; ++col) {
+        double s = Y[(size_t)i * n + col];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) s -= l[q] * Y[(size_t)(k0 + q) * n + col];
+        Y[(size_t)i * n + col] = s;
+      }
     }
     __syncthreads();
   }
@@ -392,24 +446,24 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
       if (lane < 16) Y[(size_t)(k0 + lane) * n + col] = v;
     }
     __syncthreads();
-    for (int x = tid; x < k0 * n; x += NT) {
-      const int i = x / n, col = x % n;
-      double s = Y[(size_t)i * n + col];
-      for (int q = 0; q < 16; ++q) s -= S[(size_t)i * Np + k0 + q] * Y[(size_t)(k0 + q) * n + col];
-      Y[(size_t)i * n + col] = s;
+    for (int i = tid; i < k0; i += NT) {
+      double u[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) u[q] = S[(size_t)i * Np + k0 + q];
+      for (int col = 0; col < n; ++col) {
+        double s = Y[(size_t)i * n + col];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) s -= u[q] * Y[(size_t)(k0 + q) * n + col];
+        Y[(size_t)i * n + col] = s;
+      }
     }
     __syncthreads();
   }
   for (int k = tid; k < Np; k += NT) ip[k] = s_ip[k];  // the pivots, kept for inspection
   // ---- G[t][i][j] = sum_k cf2it(t,k) g[(k,i)][j], hermitised
   double *Gout = A.gout + go;
-  for (int x = tid; x < r * nn; x += NT) {
-    const int t = x / nn, ij = x % nn;
-    double s = 0.0;
-#pragma unroll 8
-    for (int k = 0; k < r; ++k) s += A.cf2it[t * r + k] * Y[(size_t)k * nn + ij];
-    Gout[x] = s;
-  }
+  rr_gemm<false>(A.cf2it, Y, nullptr, nullptr, nn, r, nn, wv, NWV, lane,
+                 [&](int t, int c, double p1, double) { Gout[(size_t)t * nn + c] = p1; });
   __syncthreads();
   for (int x = tid; x < r * nn; x += NT) {
     const int t = x / nn, i = (x / n) % n, j = x % n;
@@ -420,6 +474,205 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
       Gout[(size_t)t * nn + j * n + i] = h;
     }
   }
+}
+
+constexpr int kPanelRows = (kDysonMaxNp + 255) / 256;  // panel rows per thread of lu_panel_kernel
+
+// Column JJ of a panel held in registers by lu_panel_kernel (row i in thread i mod 256, slot i / 256):
+// the pivot (first row of maximal modulus at or below the diagonal: DPP reduction per wave, then over
+// the four waves in a fixed order), the interchange through LDS and the update of every row below,
+// in the arithmetic of panel_factor. Column index and slots are compile-time constants, so the rows
+// stay in registers.
+template <int JJ>
+__device__ __forceinline__ void panel_step(double (&v)[kPanelRows][16], int m, int k0, int Np, int tid, double (*s_cv)[4], int (*s_ci)[4],
+                                           double (*s_prow)[16], double (*s_orow)[16], int *ipk) {
+  constexpr int bs = JJ & 1;
+  const int lane = tid & 63, wv = tid >> 6;
+  double best = -1.0;
+  int bi = Np;
+#pragma unroll
+  for (int k = 0; k < kPanelRows; ++k) {
+    const int i = tid + 256 * k;
+    if (i >= JJ && i < m) {
+      const double a = fabs(v[k][JJ]);
+      if (a > best) { best = a; bi = i; }
+    }
+  }
+  wave_argmax(best, bi, Np);
+  if (lane == 0) {
+    s_cv[bs][wv] = best;
+    s_ci[bs][wv] = bi;
+  }
+  __syncthreads();
+  double bv = -1.0;
+  int p = Np;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const double cv = s_cv[bs][w];
+    const int ci = s_ci[bs][w];
+    if (cv > bv || (cv == bv && ci < p)) { bv = cv; p = ci; }
+  }
+  if (tid == 0) ipk[JJ] = k0 + p;
+#pragma unroll
+  for (int k = 0; k < kPanelRows; ++k)
+    if (p == tid + 256 * k) {
+#pragma unroll
+      for (int c = 0; c < 16; ++c) s_prow[bs][c] = v[k][c];
+    }
+  if (p != JJ && tid == JJ) {
+#pragma unroll
+    for (int c = 0; c < 16; ++c) s_orow[bs][c] = v[0][c];
+  }
+  __syncthreads();
+  double prow[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) prow[c] = s_prow[bs][c];
+  if (p != JJ) {  // rows JJ and p interchanged
+    if (tid == JJ) {
+#pragma unroll
+      for (int c = 0; c < 16; ++c) v[0][c] = prow[c];
+    }
+#pragma unroll
+    for (int k = 0; k < kPanelRows; ++k)
+      if (p == tid + 256 * k) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) v[k][c] = s_orow[bs][c];
+      }
+  }
+  const double pv = prow[JJ];
+#pragma unroll
+  for (int k = 0; k < kPanelRows; ++k) {
+    const int i = tid + 256 * k;
+    if (i > JJ && i < m) {
+      const double l = v[k][JJ] / pv;
+      v[k][JJ] = l;
+#pragma unroll
+      for (int c = JJ + 1; c < 16; ++c) v[k][c] -= l * prow[c];
+    }
+  }
+}
+
+template <int JJ>
+__device__ __forceinline__ void panel_steps(double (&v)[kPanelRows][16], int m, int k0, int Np, int tid, double (*s_cv)[4], int (*s_ci)[4],
+                                            double (*s_prow)[16], double (*s_orow)[16], int *ipk) {
+  if constexpr (JJ < 16) {
+    panel_step<JJ>(v, m, k0, Np, tid, s_cv, s_ci, s_prow, s_orow, ipk);
+    panel_steps<JJ + 1>(v, m, k0, Np, tid, s_cv, s_ci, s_prow, s_orow, ipk);
+  }
+}
+
+// Panel k0 of every block's system (assembled in global memory by dyson_kernel in mode 1), one
+// workgroup of four waves per block with the panel's rows in registers, two workgroup barriers per
+// column (panel_step). Blocks whose system is smaller than k0 + 16 rows, or solved by the
+// large-block path, exit.
+__global__ void __launch_bounds__(256) lu_panel_kernel(DysonBatch Bt, int k0) {
+  __shared__ double s_cv[2][4], s_prow[2][16], s_orow[2][16];
+  __shared__ int s_ci[2][4], ipk[16];
+  const int pidx = find_seg(Bt.prefix, Bt.nprobs, blockIdx.x);
+  const DysonArgs &A = Bt.probs[pidx];
+  const int b = blockIdx.x - Bt.prefix[pidx], Np = (A.r * A.dims[b] + 15) & ~15;
+  if ((A.large && A.large[b]) || k0 >= Np) return;
+  double *S = A.sys + A.soff[b];
+  const int tid = threadIdx.x, m = Np - k0;
+  double v[kPanelRows][16];
+#pragma unroll
+  for (int k = 0; k < kPanelRows; ++k) {
+    const int i = tid + 256 * k;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) v[k][c] = i < m ? S[(size_t)(k0 + i) * Np + k0 + c] : 0.0;
+  }
+  panel_steps<0>(v, m, k0, Np, tid, s_cv, s_ci, s_prow, s_orow, ipk);
+#pragma unroll
+  for (int k = 0; k < kPanelRows; ++k) {
+    const int i = tid + 256 * k;
+    if (i < m) {
+#pragma unroll
+      for (int c = 0; c < 16; ++c) S[(size_t)(k0 + i) * Np + k0 + c] = v[k][c];
+    }
+  }
+  __syncthreads();
+  if (tid < 16) A.ipiv[A.poff[b] + k0 + tid] = ipk[tid];
+}
+
+// After panel k0: one workgroup per 16-column strip (blockIdx.x) of block blockIdx.y, every strip
+// but the panel's. The strip's columns take the panel's row interchanges, applied as one gather of
+// the (at most 32) rows they touch, and, right of the panel, U12 = L11^{-1} A12 (one column per
+// thread); then its column of trailing tiles A22 -= L21 U12 (one tile per wave at a time, L21 read
+// from the factorised panel in global memory). The same arithmetic in the same order as the
+// single-workgroup LU of dyson_kernel (panel_column, trailing_tile).
+__global__ void __launch_bounds__(256) lu_update_kernel(DysonBatch Bt, int k0) {
+  __shared__ double L11[256], U[256];
+  __shared__ int ipk[16], rows[32], src[32], nrows;
+  const int pidx = find_seg(Bt.prefix, Bt.nprobs, blockIdx.y);
+  const DysonArgs &A = Bt.probs[pidx];
+  const int b = blockIdx.y - Bt.prefix[pidx], Np = (A.r * A.dims[b] + 15) & ~15;
+  if ((A.large && A.large[b]) || k0 >= Np) return;
+  const int j = blockIdx.x, nstrips = Np / 16 - 1;
+  if (j >= nstrips) return;
+  const int col0 = 16 * j < k0 ? 16 * j : 16 * (j + 1);
+  double *S = A.sys + A.soff[b];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  L11[tid] = S[(size_t)(k0 + (tid >> 4)) * Np + k0 + (tid & 15)];
+  if (tid < 16) ipk[tid] = A.ipiv[A.poff[b] + k0 + tid];
+  __syncthreads();
+  if (wv == 0) {
+    // the panel's interchanges, in order, as a permutation of the rows they touch, one row per lane
+    // (lanes 0..15 rows k0..k0+15, then the other pivot rows): after them row rows[x] holds what row
+    // src[x] held before
+    int row = lane < 16 ? k0 + lane : -1, cnt = 16;
+    for (int q = 0; q < 16; ++q) {
+      const int pr = ipk[q];
+      if (__ballot(row == pr) == 0) {
+        if (lane == cnt) row = pr;
+        ++cnt;
+      }
+    }
+    int sr = row;
+    for (int q = 0; q < 16; ++q) {
+      const int pr = ipk[q];
+      if (pr == k0 + q) continue;
+      const int xb = __ffsll((unsigned long long)__ballot(row == pr)) - 1;
+      const int sa = __shfl(sr, q, 64), sb = __shfl(sr, xb, 64);
+      if (lane == q) sr = sb;
+      if (lane == xb) sr = sa;
+    }
+    if (lane < 32) {
+      rows[lane] = row;
+      src[lane] = sr;
+    }
+    if (lane == 0) nrows = cnt;
+  }
+  __syncthreads();
+  {  // the strip's columns gathered through the permutation (all reads before any write)
+    const int c = tid & 15, x0 = tid >> 4, cnt = nrows;
+    double v0 = 0.0, v1 = 0.0;
+    if (x0 < cnt) v0 = S[(size_t)src[x0] * Np + col0 + c];
+    if (x0 + 16 < cnt) v1 = S[(size_t)src[x0 + 16] * Np + col0 + c];
+    U[x0 * 16 + c] = v0;  // rows k0..k0+15 of the strip after the interchanges
+    __syncthreads();
+    if (x0 < cnt && src[x0] != rows[x0]) S[(size_t)rows[x0] * Np + col0 + c] = v0;
+    if (x0 + 16 < cnt && src[x0 + 16] != rows[x0 + 16]) S[(size_t)rows[x0 + 16] * Np + col0 + c] = v1;
+  }
+  __syncthreads();
+  if (col0 >= k0 + 16 && tid < 16) {  // U12 = L11^{-1} A12, one column per thread, as in panel_column
+    double x[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) x[q] = U[q * 16 + tid];
+#pragma unroll
+    for (int q = 1; q < 16; ++q) {
+      double sacc = x[q];
+#pragma unroll
+      for (int q0 = 0; q0 < 16; ++q0)
+        if (q0 < q) sacc -= L11[q * 16 + q0] * x[q0];
+      x[q] = sacc;
+    }
+#pragma unroll
+    for (int q = 1; q < 16; ++q) S[(size_t)(k0 + q) * Np + col0 + tid] = x[q];
+  }
+  __syncthreads();
+  const int c1 = k0 + 16;
+  if (col0 < c1) return;
+  for (int ti = c1 + 16 * wv; ti < Np; ti += 64) trailing_tile(S, Np, k0, ti, col0, S + (size_t)ti * Np + k0, Np, lane);
 }
 
 // Per problem (one workgroup each): Z = -Tr G_new(beta) from the DLR coefficients of the diagonal
@@ -508,7 +761,7 @@ __global__ void fill_kernel(int64_t n, double v, double *p) {
 
 // host-callable launchers
 hipError_t launch_dyson(const DysonBatch &B, hipStream_t s) {
-  static bool ready = false;  // attribute set once, outside any stream capture after the first launch
+  static bool ready = false;  // attributes set once, outside any stream capture after the first launch
   if (!ready) {
     const size_t most = std::max<size_t>((size_t)kDysonMaxNp * 16, (size_t)kLdsSysMax * (kLdsSysMax + 16));
     hipError_t e = hipFuncSetAttribute((const void *)dyson_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -520,7 +773,22 @@ hipError_t launch_dyson(const DysonBatch &B, hipStream_t s) {
   if (B.np_max > kDysonMaxNp) return hipErrorInvalidValue;
   const size_t np = (size_t)B.np_max;
   const size_t sh = sizeof(double) * ((np <= (size_t)kLdsSysMax ? np * np : 0) + np * 16);
-  hipLaunchKernelGGL(dyson_kernel, dim3(B.total), dim3(kDysonThreads), sh, s, B);
+  if (B.np_max <= kDysonMultiNp) {
+    DysonBatch one = B;
+    one.mode = 0;
+    hipLaunchKernelGGL(dyson_kernel, dim3(B.total), dim3(kDysonThreads), sh, s, one);
+    return hipGetLastError();
+  }
+  // systems in global memory, factorised across workgroups panel by panel
+  DysonBatch pre = B, post = B;
+  pre.mode = 1;
+  post.mode = 2;
+  hipLaunchKernelGGL(dyson_kernel, dim3(B.total), dim3(kDysonThreads), sh, s, pre);
+  for (int k0 = 0; k0 < B.np_max; k0 += 16) {
+    hipLaunchKernelGGL(lu_panel_kernel, dim3(B.total), dim3(256), 0, s, B, k0);
+    hipLaunchKernelGGL(lu_update_kernel, dim3(B.np_max / 16 - 1, B.total), dim3(256), 0, s, B, k0);
+  }
+  hipLaunchKernelGGL(dyson_kernel, dim3(B.total), dim3(kDysonThreads), sh, s, post);
   return hipGetLastError();
 }
 

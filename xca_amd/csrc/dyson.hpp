@@ -37,7 +37,15 @@ struct DysonBatch {
   const int *prefix;
   int nprobs, total;
   int np_max;               // largest padded system dimension among the blocks (LDS panel rows)
+  int mode = 0;             // dyson_kernel: 0 the whole solve; 1 up to the assembled system; 2 from the
+                            // factorised system (set by launch_dyson)
 };
+
+// Launches whose largest padded system exceeds this factorise every block's system across
+// workgroups: one launch per 16-column panel (one workgroup per block) and one per trailing update
+// (one workgroup per 16-column strip of every block), between a launch that assembles the systems
+// and one that solves with the factors.
+constexpr int kDysonMultiNp = 128;
 
 // Layout of the per-problem iteration status (doubles) read back after every iteration.
 constexpr int kStDiff = 0;   // max |G_new - G| (bits of a non-negative double, atomic max)
