@@ -65,7 +65,10 @@ def test_large_block_self_consistency_fixed_point(shell):
     S = BlockSparseSolver(P).solve(1, tol=tol, maxiter=40, hyb_comp=False)
     print(f"[gpu] 5-orbital NCA solve: {S.n_ppsc_iter} iterations, diff {S.diff_G:.2e}, eta {S.eta:.6f}", flush=True)
     assert S.diff_G < tol
-    assert S.G_tau is None  # correlators take blocks of at most 12 states
+    # g(tau) of the converged G on the large-block engine (its parity: test_gpu_wide_corr.py)
+    assert S.G_tau.shape == (P.grid.r, P.n, P.n) and np.isfinite(S.G_tau).all()
+    g01 = po.interp(P.grid.it, P.grid.rf, S.G_tau, np.array([0.0, 1.0])).real
+    print(f"[gpu] 5-orbital NCA g(0) + g(beta) diagonal: {np.diag(g01[0] + g01[1])}", flush=True)
     Z = S.partition_function_from_ppgf(S.G)
     assert abs(Z - 1.0) < 1e-7
     G2 = S.d.dyson_solve(S.d.pseudo_particle_self_energy(S.G, 1), S.eta)

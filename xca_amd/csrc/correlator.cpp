@@ -146,13 +146,17 @@ static CorrDevPlan *get_corr_plan(xca_ctx *c, int m, const int *topo, const int3
   build_corr_programs(c->M, c->ps, topo, m, c->G_zero, co, hp);
   int variant = c->tile_variant >= 0 ? c->tile_variant : 0;
   int max_w = 0;
+  // a context with blocks above the tile engine's (or XCA_WIDE=1) runs the chains on the large-block
+  // engine, as its Σ evaluations do
+  const bool wide = c->wide;
   for (auto &pg : hp.progs) {
-    for (auto &op : pg.ops)
-      if (op.bin > 12 || op.bout > 12) throw std::invalid_argument("xca: device path supports blocks of dimension <= 12 and symmetry sets of size <= 8");
+    if (!wide)
+      for (auto &op : pg.ops)
+        if (op.bin > 12 || op.bout > 12) throw std::logic_error("xca: block above 12 rows in a tile-engine context");
     max_w = std::max(max_w, pg.max_w);
   }
-  if (max_w > tile_columns(variant)) variant = 0;
-  if (max_w > tile_columns(variant)) throw std::invalid_argument("xca: block path too wide for one tile");
+  if (!wide && max_w > tile_columns(variant)) variant = 0;
+  if (!wide && max_w > tile_columns(variant)) throw std::invalid_argument("xca: block path too wide for one tile");
   const int NT = tile_columns(variant);
   auto dp = std::make_unique<CorrDevPlan>();
   dp->nmu = co.nmu;
@@ -170,7 +174,8 @@ static CorrDevPlan *get_corr_plan(xca_ctx *c, int m, const int *topo, const int3
     DProg d{};
     d.op0 = (int)h_ops.size();
     d.nops = (int)pg.ops.size();
-    for (auto &op : pg.ops) h_ops.push_back(device_op(op, hp.tables, c));
+    if (!wide)
+      for (auto &op : pg.ops) h_ops.push_back(device_op(op, hp.tables, c));
     d.sigma_block = -1;
     d.b0 = pg.out_cols;
     d.b1 = pg.b1;
@@ -287,6 +292,7 @@ static CorrDevPlan *get_corr_plan(xca_ctx *c, int m, const int *topo, const int3
 
   // batches
   int64_t max_store = 0, max_partial = 0;
+  std::vector<WideCorrProg> wide_progs;
   size_t gi0 = 0;
   while (gi0 < dgroups.size()) {
     // groups [gi0, gi1) fill one batch
@@ -331,6 +337,20 @@ static CorrDevPlan *get_corr_plan(xca_ctx *c, int m, const int *topo, const int3
         d.ntup = (int)dg.ntup;
         d.out_off = off;
         out_of[pi] = off;
+        if (wide) {  // the items in the chain kernel's store order: last line fastest
+          WideCorrProg w{&hp.progs[pi], {}, off, (int)dp->batches.size()};
+          w.poles.reserve((size_t)dg.ntup);
+          for (int64_t u = 0; u < dg.ntup; ++u) {
+            std::vector<int> ls(nl);
+            int64_t v = u;
+            for (int x = nl - 1; x >= 0; --x) {
+              ls[x] = c->plist_h[d.set_off[x] + (int)(v % d.set_len[x])];
+              v /= d.set_len[x];
+            }
+            w.poles.push_back(std::move(ls));
+          }
+          wide_progs.push_back(std::move(w));
+        }
         if (hp.progs[pi].refl) {  // reflection jobs over the program's stored items
           const int w = hp.progs[pi].out_rows * hp.progs[pi].out_cols;
           const int64_t ncols = dg.ntup * w;
@@ -338,7 +358,7 @@ static CorrDevPlan *get_corr_plan(xca_ctx *c, int m, const int *topo, const int3
             rjobs.push_back(RJob{off, w, (int)c0, (int)std::min<int64_t>(refl_columns(), ncols - c0), 0});
         }
         off += per_item[pi] * dg.ntup;
-        B->h_progs.push_back(d);
+        if (!wide) B->h_progs.push_back(d);
         est_tiles += (d.ntup + d.items_per_tile - 1) / d.items_per_tile;
         dp->items += dg.ntup;
         dp->macs += canonical_macs(hp.progs[pi], r) * (double)dg.ntup;
@@ -396,10 +416,12 @@ static CorrDevPlan *get_corr_plan(xca_ctx *c, int m, const int *topo, const int3
   }
   hipStream_t s = c->stream;
   dp->ops.upload(h_ops, s);
-  dp->h_tab = tab_jobs(hp.tables, c);
-  dp->tabjobs.upload(dp->h_tab, s);
-  dp->ntabjobs = (int)dp->h_tab.size();
-  dp->ftab.alloc(std::max<int64_t>(hp.ftab_size, 1));
+  if (!wide) {
+    dp->h_tab = tab_jobs(hp.tables, c);
+    dp->tabjobs.upload(dp->h_tab, s);
+    dp->ntabjobs = (int)dp->h_tab.size();
+    dp->ftab.alloc(std::max<int64_t>(hp.ftab_size, 1));
+  }
   dp->store.alloc(std::max<int64_t>(max_store, 1));
   dp->partial.alloc(std::max<int64_t>(max_partial, 1));
   dp->cidx.upload(h_cidx, s);
@@ -408,6 +430,11 @@ static CorrDevPlan *get_corr_plan(xca_ctx *c, int m, const int *topo, const int3
   HIP_CHECK(init_queue(dp->queue.ptr, s));
   dp->device_bytes += (int64_t)(dp->ops.n * sizeof(DOp) + dp->tabjobs.n * sizeof(DTabJob)) +
                       8 * (int64_t)(dp->ftab.n + dp->store.n + dp->partial.n + dp->cval.n) + 4 * (int64_t)dp->cidx.n;
+  if (wide) {
+    dp->wide = build_wide_corr(c, hp.tables, hp.ftab_size, wide_progs, (int)dp->batches.size(), dp->store.ptr, (int64_t)dp->store.n);
+    dp->device_bytes += dp->wide->device_bytes;
+    dp->variant = -1;
+  }
   // launch descriptors
   for (auto &B : dp->batches) {
     ChainArgs A{};
@@ -459,6 +486,7 @@ static void corr_eval(xca_ctx *c, const double *G, int m, const int *topo, const
   for (size_t x = 0; x < cval.size(); ++x) cval[x] = co.mats[dp->cval_src[x]];
   if (!cval.empty()) HIP_CHECK(hipMemcpyAsync(dp->cval.ptr, cval.data(), cval.size() * sizeof(double), hipMemcpyHostToDevice, s));
   if (dp->ntabjobs > 0) HIP_CHECK(launch_tab(c->RB, TabBatch{dp->tseg.ptr, dp->tseg_prefix.ptr, 1, dp->ntabjobs}, s));
+  if (dp->wide) wide_launch_range(c, *dp->wide, 0, dp->wide->seg[0]);
   std::vector<hipEvent_t> ev;
   for (size_t bi = 0; bi < dp->batches.size(); ++bi) {
     CorrBatchDev &B = *dp->batches[bi];
@@ -468,12 +496,17 @@ static void corr_eval(xca_ctx *c, const double *G, int m, const int *topo, const
     HIP_CHECK(hipEventCreate(&e1));
     ev.push_back(e1);
     HIP_CHECK(hipEventRecord(e0, s));
-    ChainBatch CB{B.seg.ptr, B.seg_prefix.ptr, 1, B.nchunks, dp->queue.ptr, nullptr, 0};
-    CB.maxb = ctx_max_block(c);
-    HIP_CHECK(launch_chain(c->RB, dp->variant, CB, s));
+    if (dp->wide) {
+      wide_launch_range(c, *dp->wide, dp->wide->seg[bi], dp->wide->seg[bi + 1]);
+    } else {
+      ChainBatch CB{B.seg.ptr, B.seg_prefix.ptr, 1, B.nchunks, dp->queue.ptr, nullptr, 0};
+      CB.maxb = ctx_max_block(c);
+      HIP_CHECK(launch_chain(c->RB, dp->variant, CB, s));
+    }
     HIP_CHECK(hipEventRecord(e1, s));
     HIP_CHECK(launch_refl(c->RB, ReflArgs{c->D, B.rjobs.ptr, dp->store.ptr, B.nrjobs, 0}, s));
-    CorrArgs A{B.terms.ptr, B.runs.ptr, B.cchunks.ptr, dp->store.ptr, dp->cval.ptr, dp->cidx.ptr, dp->partial.ptr, B.ncchunks, r, nout, 0};
+    CorrArgs A{B.terms.ptr, B.runs.ptr, B.cchunks.ptr, dp->store.ptr, dp->cval.ptr, dp->cidx.ptr, dp->partial.ptr, B.ncchunks, r, nout,
+               dp->wide ? 1 : 0};
     HIP_CHECK(launch_corr(A, s));
     HIP_CHECK(launch_corr_reduce(dp->partial.ptr, B.ncchunks, r * nout, c->corr_out.ptr, bi > 0 ? 1 : 0, s));
   }

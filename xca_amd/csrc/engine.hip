@@ -973,6 +973,7 @@ __global__ void reset_queue_kernel(int *q) {
 // with the operators as sparse rows (wave-uniform, scalar loads) and R, L^T read with the time index
 // fastest (coalesced). Runs are summed over the chunk's tuples in a fixed order, so the result does
 // not depend on the schedule; every chunk writes its whole (r, nout) slot.
+template <bool LARGE>
 __global__ void __launch_bounds__(256) corr_kernel(CorrArgs A) {
   const CChunk ch = uniform_load(A.chunks + blockIdx.x);
   const int r = A.r, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, t = lane;
@@ -992,6 +993,21 @@ __global__ void __launch_bounds__(256) corr_kernel(CorrArgs A) {
         if (t >= r) break;
         const double *R = A.store + T.r_off + (size_t)k * d0 * db * r + t;  // element (x, a) at R[(x db + a) r]
         const double *L = A.store + T.l_off + (size_t)k * d1 * dc * r + t;  // element (i, j) at L[(i dc + j) r]
+        if (LARGE) {  // the same sums in the same order, each element of the mu product in turn
+          for (int i = 0; i < d1; ++i) {
+            const int e0 = __builtin_amdgcn_readfirstlane(mp[i]), e1 = __builtin_amdgcn_readfirstlane(mp[i + 1]);
+            const double *Li = L + (size_t)i * dc * r;
+            for (int a = 0; a < db; ++a) {
+              double nra = 0.0;
+              for (int e = e0; e < e1; ++e) nra += mv[e] * R[((size_t)__builtin_amdgcn_readfirstlane(mc[e]) * db + a) * r];
+              double qv = 0.0;
+              const int f0 = __builtin_amdgcn_readfirstlane(kp[a]), f1 = __builtin_amdgcn_readfirstlane(kp[a + 1]);
+              for (int e = f0; e < f1; ++e) qv += kv[e] * Li[(size_t)__builtin_amdgcn_readfirstlane(kc[e]) * r];
+              sum += nra * qv;
+            }
+          }
+          continue;
+        }
         for (int i = 0; i < d1; ++i) {
           double nr[kMaxB];
 #pragma unroll
@@ -1247,7 +1263,8 @@ hipError_t prepare_kernels(int RB, hipStream_t s) {
 
 hipError_t launch_corr(const CorrArgs &A, hipStream_t s) {
   if (A.nchunks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(corr_kernel, dim3(A.nchunks), dim3(256), 0, s, A);
+  if (A.large) hipLaunchKernelGGL(corr_kernel<true>, dim3(A.nchunks), dim3(256), 0, s, A);
+  else hipLaunchKernelGGL(corr_kernel<false>, dim3(A.nchunks), dim3(256), 0, s, A);
   return hipGetLastError();
 }
 

@@ -190,7 +190,8 @@ struct CorrArgs {
   const double *store, *cval;
   const int *cidx;
   double *partial;   // (nchunks, r, nout)
-  int nchunks, r, nout, pad0;
+  int nchunks, r, nout;
+  int large;         // 1: blocks of any dimension (one product element at a time), 0: at most kMaxB
 };
 
 }  // namespace xca::dev

@@ -104,6 +104,35 @@ struct CorrBatchDev {
   std::vector<DChunk> h_chunks;
 };
 
+// Large-block plan: its launch list (edge tables first, then every program's ops and its final
+// accumulation into Σ or store into the correlator buffer), the per-item operand offsets and
+// prefactors, and the workspace. seg (correlators): launches [0, seg[0]) are the tables and
+// [seg[i], seg[i + 1]) the chains of batch i.
+struct WidePlan {
+  std::vector<WLaunch> launches;
+  DevBuf<double> work, ftab, pref, vmax, zperm;
+  DevBuf<int64_t> offs;
+  std::vector<size_t> seg;
+  int64_t items = 0;
+  double macs = 0;
+  int64_t device_bytes = 0;
+};
+
+// One correlator program over its items (pole tuples in store order), stored at out_off of the
+// correlator buffer in batch `batch`.
+struct WideCorrProg {
+  const Program *pg;
+  std::vector<std::vector<int>> poles;
+  int64_t out_off;
+  int batch;
+};
+
+// Large-block correlator chains of every batch (wide.cpp).
+std::unique_ptr<WidePlan> build_wide_corr(xca_ctx *c, const std::vector<EdgeTable> &tables, int64_t ftab_size,
+                                          const std::vector<WideCorrProg> &progs, int nbatch, double *store, int64_t store_n);
+// Launches [l0, l1) of a large-block plan on the context's stream.
+void wide_launch_range(xca_ctx *c, const WidePlan &wp, size_t l0, size_t l1);
+
 // Everything one correlator evaluation (topology, zero-block pattern, flat-index subset, operator
 // connections) needs on the device.
 struct CorrDevPlan {
@@ -120,17 +149,7 @@ struct CorrDevPlan {
   int64_t device_bytes = 0;
   std::vector<DTabJob> h_tab;
   std::vector<int64_t> cval_src;  // operator value x sits at position cval_src[x] of the call's matrices
-};
-
-// Large-block Σ plan: its launch list (edge tables first, then every program's ops and final
-// accumulation), the per-item operand offsets and prefactors, and the workspace.
-struct WidePlan {
-  std::vector<WLaunch> launches;
-  DevBuf<double> work, ftab, pref, vmax, zperm;
-  DevBuf<int64_t> offs;
-  int64_t items = 0;
-  double macs = 0;
-  int64_t device_bytes = 0;
+  std::unique_ptr<WidePlan> wide;  // blocks above the tile engine's: the chains on the large-block engine
 };
 
 // Problem-level DLR matrix of the large-block engine (plain row-major): beta [cf2it | tcf2it].

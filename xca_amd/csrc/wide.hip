@@ -96,23 +96,45 @@ __global__ void __launch_bounds__(256) wgemm_kernel(WGemm G) {
 }
 
 // dst[u][x] = src[x] for x < n
-__global__ void winit_kernel(const double *__restrict__ src, double *__restrict__ dst, int64_t n, int nitem) {
+__global__ void winit_kernel(const double *__restrict__ src, double *__restrict__ dst, int64_t n, int nitem, int ld) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n * nitem) return;
-  dst[g] = src[g % n];
+  int64_t x = g % n;
+  if (ld > 0) {  // node block t, element (i, j) from (j, i)
+    const int64_t bb = (int64_t)ld * ld, e = x % bb;
+    x = x - e + (e % ld) * ld + e / ld;
+  }
+  dst[g] = src[x];
+}
+
+// Correlator store: item u's T (r x n, node-major) to dst with the node index fastest, times pref[u]
+__global__ void wstore_kernel(const double *__restrict__ src, double *__restrict__ dst, int64_t n, int r, int nitem,
+                              const double *__restrict__ pref) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = n * r;
+  if (g >= per * nitem) return;
+  const int64_t u = g / per, x = g % per;
+  const int64_t e = x / r;
+  const int t = (int)(x % r);
+  dst[g] = pref[u] * src[u * per + (int64_t)t * n + e];
 }
 
 // Edge-table inputs: dst[e] = G (r x n) times the K factors of the table's lines for entry e
 // (line q's pole = (e / p^q) mod p, sign ls[q])
 __global__ void wgk_kernel(const double *__restrict__ G, double *__restrict__ dst, int r, int64_t n, int nent, int nlines, int ls0, int ls1,
-                           int ls2, int p, const double *__restrict__ kp, const double *__restrict__ km) {
+                           int ls2, int p, const double *__restrict__ kp, const double *__restrict__ km, int ld) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t per = (int64_t)r * n;
   if (g >= per * nent) return;
   const int e = (int)(g / per);
   const int64_t x = g % per;
   const int t = (int)(x / n);
-  double v = G[x];
+  int64_t xs = x;
+  if (ld > 0) {  // element (i, j) of the transposed block
+    const int64_t e = x % n;
+    xs = x - e + (e % ld) * ld + e / ld;
+  }
+  double v = G[xs];
   const int ls[3] = {ls0, ls1, ls2};
   int ee = e;
   for (int q = 0; q < nlines; ++q) {
@@ -273,7 +295,7 @@ hipError_t launch_wide(const WLaunch &L, const WArgs &A, double scale, hipStream
     }
     case W_INIT: {
       const int64_t tot = L.n * L.nitem;
-      hipLaunchKernelGGL(winit_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, L.src, L.dst, L.n, L.nitem);
+      hipLaunchKernelGGL(winit_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, L.src, L.dst, L.n, L.nitem, L.ld);
       break;
     }
     case W_V2C:
@@ -290,7 +312,7 @@ hipError_t launch_wide(const WLaunch &L, const WArgs &A, double scale, hipStream
     case W_GK: {
       const int64_t tot = (int64_t)A.r * L.n * L.nitem;
       hipLaunchKernelGGL(wgk_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, L.src, L.dst, A.r, L.n, L.nitem, L.nlines, L.ls[0],
-                         L.ls[1], L.ls[2], L.p, A.kp, A.km);
+                         L.ls[1], L.ls[2], L.p, A.kp, A.km, L.ld);
       break;
     }
     case W_FH: {
@@ -308,6 +330,11 @@ hipError_t launch_wide(const WLaunch &L, const WArgs &A, double scale, hipStream
       hipLaunchKernelGGL(wfinal_kernel, dim3((unsigned)((L.n + 255) / 256)), dim3(256), 0, s, L.src, L.n, L.nitem, L.vmax, L.pref, L.dst,
                          scale);
       break;
+    case W_STORE: {
+      const int64_t tot = (int64_t)A.r * L.n * L.nitem;
+      hipLaunchKernelGGL(wstore_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, L.src, L.dst, L.n, A.r, L.nitem, L.pref);
+      break;
+    }
     default:
       return hipErrorInvalidValue;
   }

@@ -37,17 +37,20 @@ struct WLocal {
   int b, cols, r, nitem;
 };
 
-enum WKind : int { W_GEMM = 0, W_INIT, W_V2C, W_GK, W_FH, W_MAX, W_FINAL, W_LOCAL, W_ZERO };
+enum WKind : int { W_GEMM = 0, W_INIT, W_V2C, W_GK, W_FH, W_MAX, W_FINAL, W_LOCAL, W_ZERO, W_STORE };
 
 // One launch of a wide plan. Fields by kind:
-//   W_INIT:   dst[u] = src (n doubles)
+//   W_INIT:   dst[u] = src (n doubles); ld > 0: src holds r blocks of ld x ld, each copied transposed
 //   W_V2C:    v.G = vals2coefs(v.X), v.WG = W v.G (launch_wv2c)
 //   W_LOCAL:  the node-local products of an edge
 //   W_ZERO:   dst[0, n) = 0
-//   W_GK:     dst[e] (r x n) = G block src (r x n) times prod_q K^{ls_q}(t, pole of line q of entry e)
+//   W_GK:     dst[e] (r x n) = G block src (r x n) times prod_q K^{ls_q}(t, pole of line q of entry e);
+//             ld > 0: the block transposed (n = ld^2)
 //   W_FH:     dst[t] (b1 x qs b0), block mu = sum_kap h(t, mu, kap) F_kap (F_kap at src + kap b1 b0)
 //   W_MAX:    vmax[u] = max(vmax[u], max |src[u]|) over n doubles (vmax zeroed by W_ZERO first)
 //   W_FINAL:  dst (r b0^2, the Sigma block) += scale sum_u [vmax[u] |pref[u]| > 1e-16] pref[u] src[u]
+//   W_STORE:  dst[(u n + e) r + t] = pref[u] src[(u r + t) n + e]  (the item's T of n elements per node,
+//             node index fastest: the correlator store)
 struct WLaunch {
   int kind;
   int nitem, ncol, ld;

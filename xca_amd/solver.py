@@ -174,14 +174,7 @@ class BlockSparseSolver:
                 print(f"iter = {k}, diff_G = {d:2.2E}, Z-1 = {Z - 1:+2.2E}, eta = {e:2.2E}")
             if diff < tol:
                 print(f"Converged after {it} iterations with diff_G = {diff:2.2E} < tol = {tol:2.2E}")
-        if int(np.max(self.problem.dims)) > 12:
-            # the correlator engine takes blocks of at most 12 states; an explicit
-            # single_particle_greens_function call raises with the library's message
-            self.G_tau = None
-            if verbose:
-                print("single-particle Green's function: not evaluated on the device for blocks above 12 states")
-        else:
-            self.G_tau = self.single_particle_greens_function(self.spgf_max_order)
+        self.G_tau = self.single_particle_greens_function(self.spgf_max_order)
         return self
 
     # ------------------------------------------------------------------ observables
