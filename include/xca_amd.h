@@ -160,7 +160,9 @@ int xca_dyson_solve_noalloc(xca_ctx *ctx, const double *G0, const double *sigma,
  * -> Z = -Tr G(beta), eta += ln|Z|/beta, G_new *= exp(-tau deta) -> stop when max|G_new - G| < tol,
  * else G = mix G_new + (1 - mix) G. Starts from the G0 given to xca_dyson_init and pseudo-particle
  * chemical potential eta0. Outputs the solver's G, eta, the iteration count and the last difference.
- * Each iteration runs as one captured HIP graph (set XCA_NO_GRAPH to launch eagerly). */
+ * Each iteration runs as one captured HIP graph (set XCA_NO_GRAPH to launch eagerly); a problem
+ * with blocks above 12 rows runs each iteration as launches on the large-block engine and the
+ * device-wide Dyson path, with one status read per iteration. */
 int xca_ppsc_solve(xca_ctx *ctx, double eta0, int max_order, double tol, int maxiter, double mix, double *G_out, double *eta_out,
                    int *iters_out, double *diff_out);
 /* The per-iteration report of the last xca_ppsc_solve (what BlockSparseSolver.solve prints each

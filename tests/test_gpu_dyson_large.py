@@ -69,7 +69,30 @@ def test_large_block_self_consistency_fixed_point(shell):
     assert S.G_tau.shape == (P.grid.r, P.n, P.n) and np.isfinite(S.G_tau).all()
     g01 = po.interp(P.grid.it, P.grid.rf, S.G_tau, np.array([0.0, 1.0])).real
     print(f"[gpu] 5-orbital NCA g(0) + g(beta) diagonal: {np.diag(g01[0] + g01[1])}", flush=True)
+    # <c c^dag> + <c^dag c> = 1 with Z = 1: g(0+) + g(beta-) = -1 on the diagonal
+    assert np.abs(np.diag(g01[0] + g01[1]) + 1.0).max() < 1e-6
     Z = S.partition_function_from_ppgf(S.G)
     assert abs(Z - 1.0) < 1e-7
     G2 = S.d.dyson_solve(S.d.pseudo_particle_self_energy(S.G, 1), S.eta)
     assert max(float(np.abs(a - b).max()) for a, b in zip(G2, S.G)) < 10 * tol
+
+
+def test_device_loop_matches_host_loop(shell):
+    """xca_ppsc_solve on the large-block problem (Sigma on the large-block engine, device-wide Dyson,
+    device normalisation / convergence / mixing) against the same loop driven from the host around
+    the device's Sigma and Dyson (Z, eta, diff and mixing in numpy): same iterates and history."""
+    P, ev = shell
+    tol = 1e-9
+    Gd, etad, itd, diffd = ev.ppsc_solve(1, tol=tol, maxiter=40, mix=0.7)
+    hd = ev.ppsc_history()
+    Gh, etah, ith, diffh = ev._ppsc_solve_host(1, tol, 40, 0.7, 0.0)
+    hh = ev._history
+    print(f"[gpu] large-block loop: device {itd} iterations diff {diffd:.2e} eta {etad:.9f}; host {ith} iterations diff {diffh:.2e} "
+          f"eta {etah:.9f}", flush=True)
+    assert diffd < tol and diffh < tol
+    assert itd == ith and len(hd) == itd
+    for (a, b, c), (x, y, z) in zip(hd, hh):
+        assert abs(b - y) <= 1e-12 * abs(y) and abs(c - z) <= 1e-12 * max(1.0, abs(z))
+        assert abs(a - x) <= 1e-10 * max(x, 1e-12) + 1e-14
+    assert abs(etad - etah) <= 1e-12
+    assert max(rel(a, b) for a, b in zip(Gd, Gh)) <= 1e-12

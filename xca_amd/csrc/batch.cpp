@@ -387,6 +387,89 @@ class Batch {
 
 using namespace xca;
 
+namespace {
+
+// Self-consistency of a problem with blocks above 12 rows: each iteration is Sigma topology by
+// topology on the large-block engine (net factor -s per topology), the Dyson solve (blocks up to 12
+// rows in the one-workgroup kernel, larger ones device-wide, both hermitised), then the same
+// normalisation / convergence / mixing and block-maxima launches as the batched engine, and one
+// copy of the status words to pinned host memory. G stays on the device throughout; eta goes to the
+// device-wide Dyson path from the last status read.
+struct LargeLoop {
+  double *host = nullptr;
+  ~LargeLoop() {
+    if (host) (void)hipHostFree(host);
+  }
+};
+
+void ppsc_solve_large(xca_ctx *c, BatchItem &it) {
+  XcaDyson *d = c->dyson;
+  const Problem &P = c->M.P;
+  const int nb = P.nb, stride = status_stride(nb);
+  hipStream_t s = c->stream;
+  LargeLoop hl;
+  HIP_CHECK(hipHostMalloc((void **)&hl.host, sizeof(double) * stride, hipHostMallocDefault));
+  double *st = d->status.ptr;
+  HIP_CHECK(hipMemcpyAsync(c->G.ptr, d->G0.ptr, c->M.G_size * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HIP_CHECK(launch_fill(stride, 0.0, st, s));
+  HIP_CHECK(launch_fill(1, it.eta0, st + kStEta, s));
+  std::vector<double> bmax(nb, 0.0);
+  for (int b = 0; b < nb; ++b)
+    for (int64_t x = c->M.G_off[b]; x < c->M.G_off[b] + (int64_t)P.r * P.dims[b] * P.dims[b]; ++x)
+      bmax[b] = std::max(bmax[b], std::fabs(d->G0_h[x]));
+  // one-problem descriptors of the Dyson and iteration launches
+  std::vector<DysonArgs> hd(1, dyson_args(c, st + kStEta, c->sigma.ptr));
+  std::vector<IterArgs> hi(1, iter_args(c, st, it.tol, it.mix));
+  std::vector<int> pre{0, nb};
+  DevBuf<DysonArgs> dd;
+  DevBuf<IterArgs> di;
+  DevBuf<int> dp;
+  dd.upload(hd, s);
+  di.upload(hi, s);
+  dp.upload(pre, s);
+  const DysonBatch DB{dd.ptr, dp.ptr, 1, nb, dyson_np_max_small(c)};
+  const IterBatch IB{di.ptr, dp.ptr, 1, nb, c->M.G_size};
+  it.iters = 0;
+  it.done = it.converged = false;
+  it.eta = it.eta0;
+  it.history.clear();
+  while (!it.done) {
+    c->G_zero.assign(nb, 0);
+    for (int b = 0; b < nb; ++b) c->G_zero[b] = !(bmax[b] >= 1e-16);
+    c->have_G = true;
+    HIP_CHECK(launch_fill(c->M.G_size, 0.0, c->sigma.ptr, s));
+    c->outer_events = true;
+    try {
+      for (int order = 1; order <= it.max_order; ++order)
+        for (auto &tp : connected_topologies(order)) accumulate(c, order, tp.second.data(), nullptr, 0, -(double)tp.first);
+    } catch (...) {
+      c->outer_events = false;
+      throw;
+    }
+    c->outer_events = false;
+    HIP_CHECK(launch_dyson(DB, s));
+    if (d->large) large_dyson_solve(c, d->large, c->sigma.ptr, it.eta, d->gnew.ptr);
+    HIP_CHECK(launch_iterate(IB, s));
+    HIP_CHECK(launch_block_max(IB, s));
+    HIP_CHECK(hipMemcpyAsync(hl.host, st, sizeof(double) * stride, hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    const double *h = hl.host;
+    it.iters += 1;
+    it.eta = h[kStEta];
+    it.diff = h[kStDiff];
+    it.history.insert(it.history.end(), {h[kStDiff], h[kStZ], h[kStEta]});
+    for (int b = 0; b < nb; ++b) bmax[b] = h[kStBmax + b];
+    if (h[kStConv] != 0.0) {
+      it.done = true;
+      it.converged = true;
+    } else if (it.iters >= it.maxiter) {
+      it.done = true;
+    }
+  }
+}
+
+}  // namespace
+
 // Pseudo-particle self-consistency of one problem through the batched engine (a batch of one).
 // On return G_out holds the solver's G (the last mixed iterate, block_sparse_solver.py:338-346),
 // eta_out the pseudo-particle chemical potential.
@@ -397,7 +480,6 @@ extern "C" int xca_ppsc_solve(xca_ctx *c, double eta0, int max_order, double tol
     if (max_order < 1 || max_order > 4) throw std::invalid_argument("xca_ppsc_solve: max_order must be between 1 and 4");
     if (maxiter < 1) throw std::invalid_argument("xca_ppsc_solve: maxiter must be positive");
     HIP_CHECK(hipSetDevice(c->device));
-    Batch batch(c->device, c->stream, 1);
     BatchItem it;
     it.c = c;
     it.max_order = max_order;
@@ -405,16 +487,21 @@ extern "C" int xca_ppsc_solve(xca_ctx *c, double eta0, int max_order, double tol
     it.tol = tol;
     it.mix = mix;
     it.eta0 = eta0;
-    batch.add(&it);
-    const bool graphs = !std::getenv("XCA_NO_GRAPH");
-    while (!it.done) batch.tick(graphs);
+    if (c->dyson->large) {  // blocks above 12 rows: the large-block engine and the device-wide Dyson path
+      ppsc_solve_large(c, it);
+    } else {
+      Batch batch(c->device, c->stream, 1);
+      batch.add(&it);
+      const bool graphs = !std::getenv("XCA_NO_GRAPH");
+      while (!it.done) batch.tick(graphs);
+      batch.remove(&it);
+    }
     HIP_CHECK(hipMemcpyAsync(G_out, c->G.ptr, c->M.G_size * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
     if (eta_out) *eta_out = it.eta;
     if (iters_out) *iters_out = it.iters;
     if (diff_out) *diff_out = it.diff;
     c->ppsc_history = it.history;
-    batch.remove(&it);
     return 0;
   } catch (std::exception &e) {
     xca::set_error(e.what());

@@ -278,10 +278,11 @@ class DiagramEvaluator:
 
     def ppsc_solve(self, max_order, tol=1e-10, maxiter=100, mix=1.0, eta0=0.0):
         """Self-consistent pseudo-particle G; returns (G blocks, eta, iterations, last max|G_new - G|).
-        The whole loop runs as one device launch sequence per iteration (xca_ppsc_solve), except for
-        complex problems and problems with blocks above 12 rows (Sigma on the large-block engine),
-        whose loop runs on the host around the device's Sigma and Dyson solve."""
-        if self.complex_arith or int(self.block_dims.max()) > 12:
+        The loop runs on the device (xca_ppsc_solve): one launch sequence per iteration, or, for
+        blocks above 12 rows, Sigma on the large-block engine and the device-wide Dyson path around
+        the same update launches. Complex problems run the loop on the host around the device's
+        Sigma and Dyson solve."""
+        if self.complex_arith:
             return self._ppsc_solve_host(max_order, tol, maxiter, mix, eta0)
         out = np.zeros(sum(self.r * int(b) * int(b) for b in self.dims))
         eta = C.c_double(0)
