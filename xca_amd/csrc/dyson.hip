@@ -147,6 +147,85 @@ __device__ __forceinline__ void panel_factor(double *Pn, int m, int k0, int Np, 
   }
 }
 
+// The same panel factorisation with the rows in registers, for panels of at most 64 kWavePanelSlots
+// rows. Rows stay where they were loaded (slot k of a lane holds the row now at position pos[k]); an
+// interchange only swaps two positions, and the rows are written to their final positions at the
+// end. The pivot is the row of maximal modulus at the smallest position (the first such row of
+// panel_factor's scan), found by a DPP reduction over (modulus, position) and broadcast with lane
+// reads; the arithmetic is panel_factor's.
+constexpr int kWavePanelSlots = 2;  // wave_panel_step's pivot broadcast is written for two slots
+static_assert(64 * kWavePanelSlots >= kDysonMultiNp, "one-launch systems must fit the register panel");
+
+template <int JJ>
+__device__ __forceinline__ void wave_panel_step(double (&v)[kWavePanelSlots][16], int (&pos)[kWavePanelSlots], int m, int k0, int Np, int *ipk,
+                                                int lane) {
+  double best = -1.0;
+  int bp = Np;
+#pragma unroll
+  for (int k = 0; k < kWavePanelSlots; ++k)
+    if (pos[k] >= JJ && pos[k] < m) {
+      const double a = fabs(v[k][JJ]);
+      if (a > best || (a == best && pos[k] < bp)) { best = a; bp = pos[k]; }
+    }
+  wave_argmax(best, bp, Np);
+  const int p = bp;
+  if (lane == 0) ipk[JJ] = k0 + p;
+  const unsigned long long h0 = __ballot(pos[0] == p);
+  double prow[16];
+  if (h0) {
+    const int pl = __builtin_ctzll(h0);
+#pragma unroll
+    for (int c = JJ; c < 16; ++c) prow[c] = lane_value(v[0][c], pl);
+  } else {
+    const int pl = __builtin_ctzll(__ballot(pos[1] == p));
+#pragma unroll
+    for (int c = JJ; c < 16; ++c) prow[c] = lane_value(v[1][c], pl);
+  }
+  // rows JJ and p interchanged: the row at position JJ moves to p, the pivot row to JJ
+#pragma unroll
+  for (int k = 0; k < kWavePanelSlots; ++k) pos[k] = pos[k] == p ? JJ : (pos[k] == JJ ? p : pos[k]);
+  const double pv = prow[JJ];
+#pragma unroll
+  for (int k = 0; k < kWavePanelSlots; ++k)
+    if (pos[k] > JJ && pos[k] < m) {
+      const double l = v[k][JJ] / pv;
+      v[k][JJ] = l;
+#pragma unroll
+      for (int c = JJ + 1; c < 16; ++c) v[k][c] -= l * prow[c];
+    }
+}
+
+template <int JJ>
+__device__ __forceinline__ void wave_panel_steps(double (&v)[kWavePanelSlots][16], int (&pos)[kWavePanelSlots], int m, int k0, int Np, int *ipk,
+                                                 int lane) {
+  if constexpr (JJ < 16) {
+    wave_panel_step<JJ>(v, pos, m, k0, Np, ipk, lane);
+    wave_panel_steps<JJ + 1>(v, pos, m, k0, Np, ipk, lane);
+  }
+}
+
+// Panel of m <= 64 kWavePanelSlots rows at Pn (16 doubles per row) by one wave, through registers.
+__device__ __forceinline__ void panel_factor_regs(double *Pn, int m, int k0, int Np, int *ipk, int lane) {
+  double v[kWavePanelSlots][16];
+  int pos[kWavePanelSlots];
+#pragma unroll
+  for (int k = 0; k < kWavePanelSlots; ++k) {
+    const int i = lane + 64 * k;
+    pos[k] = i;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) v[k][c] = i < m ? Pn[i * 16 + c] : 0.0;
+  }
+  wave_lds_sync();  // every row is read before any is written back
+  wave_panel_steps<0>(v, pos, m, k0, Np, ipk, lane);
+#pragma unroll
+  for (int k = 0; k < kWavePanelSlots; ++k)
+    if (pos[k] < m) {
+#pragma unroll
+      for (int c = 0; c < 16; ++c) Pn[pos[k] * 16 + c] = v[k][c];
+    }
+  wave_lds_sync();
+}
+
 // The row interchanges of the panel at k0 (absolute rows ipk[0..15], in order) applied to column col
 // of the system, and, right of the panel, U12 = L11^{-1} A12 for that column (L11 unit lower, its
 // rows 16 doubles apart at L11).
@@ -372,7 +451,10 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
     const int m = Np - k0;
     for (int x = tid; x < m * 16; x += NT) Pn[x] = S[(size_t)(k0 + (x >> 4)) * Np + k0 + (x & 15)];
     __syncthreads();
-    if (wv == 0) panel_factor(Pn, m, k0, Np, s_ip + k0, lane);  // one wave: no workgroup barrier per column
+    if (wv == 0) {  // one wave: no workgroup barrier per column
+      if (m <= 64 * kWavePanelSlots) panel_factor_regs(Pn, m, k0, Np, s_ip + k0, lane);
+      else panel_factor(Pn, m, k0, Np, s_ip + k0, lane);
+    }
     __syncthreads();
     for (int x = tid; x < m * 16; x += NT) S[(size_t)(k0 + (x >> 4)) * Np + k0 + (x & 15)] = Pn[x];
     // the panel's row interchanges on every other column (in order, per column), and U12
@@ -478,95 +560,79 @@ __global__ void __launch_bounds__(kDysonThreads) dyson_kernel(DysonBatch Bt) {
 
 constexpr int kPanelRows = (kDysonMaxNp + 255) / 256;  // panel rows per thread of lu_panel_kernel
 
-// Column JJ of a panel held in registers by lu_panel_kernel (row i in thread i mod 256, slot i / 256):
-// the pivot (first row of maximal modulus at or below the diagonal: DPP reduction per wave, then over
-// the four waves in a fixed order), the interchange through LDS and the update of every row below,
-// in the arithmetic of panel_factor. Column index and slots are compile-time constants, so the rows
-// stay in registers.
+// Column JJ of a panel held in registers by lu_panel_kernel (the row loaded as row i sits in thread
+// i mod 256, slot i / 256, and pos[k] is its current position): the pivot (row of maximal modulus at
+// the smallest position: DPP reduction per wave, then over the four waves in a fixed order), the
+// interchange (two positions swapped) and the update of every row below, in the arithmetic of
+// panel_factor. Each wave publishes its candidate row with its candidate before the one workgroup
+// barrier of the column; the LDS slots alternate between columns, so a column's writes never meet
+// the previous column's reads. Column index and slots are compile-time constants, so the rows stay
+// in registers.
 template <int JJ>
-__device__ __forceinline__ void panel_step(double (&v)[kPanelRows][16], int m, int k0, int Np, int tid, double (*s_cv)[4], int (*s_ci)[4],
-                                           double (*s_prow)[16], double (*s_orow)[16], int *ipk) {
+__device__ __forceinline__ void panel_step(double (&v)[kPanelRows][16], int (&pos)[kPanelRows], int m, int k0, int Np, int tid,
+                                           double (*s_cv)[4], int (*s_ci)[4], double (*s_crow)[4][16], int *ipk) {
   constexpr int bs = JJ & 1;
   const int lane = tid & 63, wv = tid >> 6;
   double best = -1.0;
-  int bi = Np;
+  int bp = Np;
 #pragma unroll
-  for (int k = 0; k < kPanelRows; ++k) {
-    const int i = tid + 256 * k;
-    if (i >= JJ && i < m) {
+  for (int k = 0; k < kPanelRows; ++k)
+    if (pos[k] >= JJ && pos[k] < m) {
       const double a = fabs(v[k][JJ]);
-      if (a > best) { best = a; bi = i; }
+      if (a > best || (a == best && pos[k] < bp)) { best = a; bp = pos[k]; }
     }
-  }
-  wave_argmax(best, bi, Np);
+  wave_argmax(best, bp, Np);
   if (lane == 0) {
     s_cv[bs][wv] = best;
-    s_ci[bs][wv] = bi;
+    s_ci[bs][wv] = bp;
   }
+#pragma unroll
+  for (int k = 0; k < kPanelRows; ++k)
+    if (pos[k] == bp) {
+#pragma unroll
+      for (int c = 0; c < 16; ++c) s_crow[bs][wv][c] = v[k][c];
+    }
   __syncthreads();
   double bv = -1.0;
-  int p = Np;
+  int p = Np, pw = 0;
 #pragma unroll
   for (int w = 0; w < 4; ++w) {
     const double cv = s_cv[bs][w];
     const int ci = s_ci[bs][w];
-    if (cv > bv || (cv == bv && ci < p)) { bv = cv; p = ci; }
+    if (cv > bv || (cv == bv && ci < p)) { bv = cv; p = ci; pw = w; }
   }
   if (tid == 0) ipk[JJ] = k0 + p;
-#pragma unroll
-  for (int k = 0; k < kPanelRows; ++k)
-    if (p == tid + 256 * k) {
-#pragma unroll
-      for (int c = 0; c < 16; ++c) s_prow[bs][c] = v[k][c];
-    }
-  if (p != JJ && tid == JJ) {
-#pragma unroll
-    for (int c = 0; c < 16; ++c) s_orow[bs][c] = v[0][c];
-  }
-  __syncthreads();
   double prow[16];
 #pragma unroll
-  for (int c = 0; c < 16; ++c) prow[c] = s_prow[bs][c];
-  if (p != JJ) {  // rows JJ and p interchanged
-    if (tid == JJ) {
+  for (int c = JJ; c < 16; ++c) prow[c] = s_crow[bs][pw][c];
 #pragma unroll
-      for (int c = 0; c < 16; ++c) v[0][c] = prow[c];
-    }
-#pragma unroll
-    for (int k = 0; k < kPanelRows; ++k)
-      if (p == tid + 256 * k) {
-#pragma unroll
-        for (int c = 0; c < 16; ++c) v[k][c] = s_orow[bs][c];
-      }
-  }
+  for (int k = 0; k < kPanelRows; ++k) pos[k] = pos[k] == p ? JJ : (pos[k] == JJ ? p : pos[k]);
   const double pv = prow[JJ];
 #pragma unroll
-  for (int k = 0; k < kPanelRows; ++k) {
-    const int i = tid + 256 * k;
-    if (i > JJ && i < m) {
+  for (int k = 0; k < kPanelRows; ++k)
+    if (pos[k] > JJ && pos[k] < m) {
       const double l = v[k][JJ] / pv;
       v[k][JJ] = l;
 #pragma unroll
       for (int c = JJ + 1; c < 16; ++c) v[k][c] -= l * prow[c];
     }
-  }
 }
 
 template <int JJ>
-__device__ __forceinline__ void panel_steps(double (&v)[kPanelRows][16], int m, int k0, int Np, int tid, double (*s_cv)[4], int (*s_ci)[4],
-                                            double (*s_prow)[16], double (*s_orow)[16], int *ipk) {
+__device__ __forceinline__ void panel_steps(double (&v)[kPanelRows][16], int (&pos)[kPanelRows], int m, int k0, int Np, int tid,
+                                            double (*s_cv)[4], int (*s_ci)[4], double (*s_crow)[4][16], int *ipk) {
   if constexpr (JJ < 16) {
-    panel_step<JJ>(v, m, k0, Np, tid, s_cv, s_ci, s_prow, s_orow, ipk);
-    panel_steps<JJ + 1>(v, m, k0, Np, tid, s_cv, s_ci, s_prow, s_orow, ipk);
+    panel_step<JJ>(v, pos, m, k0, Np, tid, s_cv, s_ci, s_crow, ipk);
+    panel_steps<JJ + 1>(v, pos, m, k0, Np, tid, s_cv, s_ci, s_crow, ipk);
   }
 }
 
 // Panel k0 of every block's system (assembled in global memory by dyson_kernel in mode 1), one
-// workgroup of four waves per block with the panel's rows in registers, two workgroup barriers per
+// workgroup of four waves per block with the panel's rows in registers, one workgroup barrier per
 // column (panel_step). Blocks whose system is smaller than k0 + 16 rows, or solved by the
 // large-block path, exit.
 __global__ void __launch_bounds__(256) lu_panel_kernel(DysonBatch Bt, int k0) {
-  __shared__ double s_cv[2][4], s_prow[2][16], s_orow[2][16];
+  __shared__ double s_cv[2][4], s_crow[2][4][16];
   __shared__ int s_ci[2][4], ipk[16];
   const int pidx = find_seg(Bt.prefix, Bt.nprobs, blockIdx.x);
   const DysonArgs &A = Bt.probs[pidx];
@@ -575,21 +641,22 @@ __global__ void __launch_bounds__(256) lu_panel_kernel(DysonBatch Bt, int k0) {
   double *S = A.sys + A.soff[b];
   const int tid = threadIdx.x, m = Np - k0;
   double v[kPanelRows][16];
+  int pos[kPanelRows];
 #pragma unroll
   for (int k = 0; k < kPanelRows; ++k) {
     const int i = tid + 256 * k;
+    pos[k] = i;
 #pragma unroll
     for (int c = 0; c < 16; ++c) v[k][c] = i < m ? S[(size_t)(k0 + i) * Np + k0 + c] : 0.0;
   }
-  panel_steps<0>(v, m, k0, Np, tid, s_cv, s_ci, s_prow, s_orow, ipk);
+  panel_steps<0>(v, pos, m, k0, Np, tid, s_cv, s_ci, s_crow, ipk);
+  __syncthreads();  // every thread has read its rows before any is written to its final position
 #pragma unroll
-  for (int k = 0; k < kPanelRows; ++k) {
-    const int i = tid + 256 * k;
-    if (i < m) {
+  for (int k = 0; k < kPanelRows; ++k)
+    if (pos[k] < m) {
 #pragma unroll
-      for (int c = 0; c < 16; ++c) S[(size_t)(k0 + i) * Np + k0 + c] = v[k][c];
+      for (int c = 0; c < 16; ++c) S[(size_t)(k0 + pos[k]) * Np + k0 + c] = v[k][c];
     }
-  }
   __syncthreads();
   if (tid < 16) A.ipiv[A.poff[b] + k0 + tid] = ipk[tid];
 }
