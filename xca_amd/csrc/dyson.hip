@@ -742,32 +742,25 @@ __global__ void __launch_bounds__(256) lu_update_kernel(DysonBatch Bt, int k0) {
   for (int ti = c1 + 16 * wv; ti < Np; ti += 64) trailing_tile(S, Np, k0, ti, col0, S + (size_t)ti * Np + k0, Np, lane);
 }
 
-// Per problem (one workgroup each): Z = -Tr G_new(beta) from the DLR coefficients of the diagonal
-// entries (one wave per entry for the LU solve; the entries summed in a fixed order by one thread),
-// deta = ln|Z| / beta, eta += deta; the difference accumulator is cleared for the update kernel that
-// follows.
-__global__ void __launch_bounds__(1024) trace_kernel(IterBatch Bt) {
-  __shared__ double s_lu[kMaxRank * kMaxRank];
-  __shared__ int s_piv[kMaxRank];
+// Per problem (one wave each): Z = -Tr G_new(beta) = -sum_t zw[t] Tr G_new(t), the diagonal entries
+// summed per node in block order and the node terms in node order by one lane; deta = ln|Z| / beta,
+// eta += deta; the difference accumulator is cleared for the update kernel that follows.
+__global__ void __launch_bounds__(64) trace_kernel(IterBatch Bt) {
+  __shared__ double part[64];
   const IterArgs &A = Bt.probs[blockIdx.x];
-  const int tid = threadIdx.x, r = A.r, lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
-  for (int x = tid; x < r * r; x += blockDim.x) s_lu[x] = A.lu[x];
-  for (int x = tid; x < r; x += blockDim.x) s_piv[x] = A.piv[x];
+  const int t = threadIdx.x, r = A.r;
+  double s = 0.0;
+  if (t < r)
+    for (int b = 0; b < A.nb; ++b) {
+      const int n = A.dims[b];
+      const double *g = A.Gnew + A.goff[b] + (size_t)t * n * n;
+      for (int i = 0; i < n; ++i) s += g[i * n + i];
+    }
+  part[t] = t < r ? A.zw[t] * s : 0.0;
   __syncthreads();
-  for (int d = wv; d < A.ndiag; d += nwv) {  // one wave per diagonal entry
-    int b = 0, i = d;
-    while (i >= A.dims[b]) { i -= A.dims[b]; ++b; }
-    const int n = A.dims[b];
-    const double x = lu_solve_wave(s_lu, s_piv, r, lane < r ? A.Gnew[A.goff[b] + (size_t)lane * n * n + i * n + i] : 0.0, lane);
-    const double kb = lane < r ? A.kbeta[lane] : 0.0;
-    double acc = 0.0;
-    for (int k = 0; k < r; ++k) acc = fma(lane_value(kb, k), lane_value(x, k), acc);
-    if (lane == 0) A.trs[(size_t)d * r] = acc;
-  }
-  __syncthreads();
-  if (tid == 0) {
+  if (t == 0) {
     double tr = 0.0;
-    for (int d = 0; d < A.ndiag; ++d) tr += A.trs[(size_t)d * r];
+    for (int q = 0; q < r; ++q) tr += part[q];
     const double Z = -tr;
     const double deta = log(fabs(Z)) / A.beta;
     A.status[kStZ] = Z;
@@ -795,29 +788,31 @@ __global__ void __launch_bounds__(256) scale_diff_kernel(IterBatch Bt) {
     atomicMax(reinterpret_cast<unsigned long long *>(A.status + kStDiff), (unsigned long long)__double_as_longlong(d));
 }
 
-// Converged (diff < tol): G stays (the solver stops without mixing). Otherwise G = mix G_new + (1 - mix) G.
-__global__ void __launch_bounds__(256) mix_kernel(IterBatch Bt) {
-  const IterArgs &A = Bt.probs[blockIdx.y];
-  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool conv = A.status[kStDiff] < A.tol;
-  if (x == 0) A.status[kStConv] = conv ? 1.0 : 0.0;
-  if (!conv && x < A.total) A.G[x] = A.mix * A.Gnew[x] + (1.0 - A.mix) * A.G[x];
-}
-
-// max |G| per (problem, block), in a fixed order
-__global__ void __launch_bounds__(256) block_max_kernel(IterBatch Bt) {
+// Per (problem, block): converged (diff < tol): G stays (the solver stops without mixing), otherwise
+// G = mix G_new + (1 - mix) G; then max |G| of the block, in a fixed order. The first block's
+// workgroup records the convergence flag.
+__global__ void __launch_bounds__(256) mix_max_kernel(IterBatch Bt) {
   __shared__ double part[256];
   const int pidx = find_seg(Bt.prefix, Bt.nprobs, blockIdx.x);
   const IterArgs &A = Bt.probs[pidx];
   const int b = blockIdx.x - Bt.prefix[pidx];
+  const bool conv = A.status[kStDiff] < A.tol;
   double m = 0.0;
-  for (int64_t x = A.goff[b] + threadIdx.x; x < A.gend[b]; x += blockDim.x) m = fmax(m, fabs(A.G[x]));
+  for (int64_t x = A.goff[b] + threadIdx.x; x < A.gend[b]; x += blockDim.x) {
+    double g = A.G[x];
+    if (!conv) {
+      g = A.mix * A.Gnew[x] + (1.0 - A.mix) * A.G[x];
+      A.G[x] = g;
+    }
+    m = fmax(m, fabs(g));
+  }
   part[threadIdx.x] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int q = 0; q < (int)blockDim.x; ++q) t = fmax(t, part[q]);
     A.status[kStBmax + b] = t;
+    if (b == 0) A.status[kStConv] = conv ? 1.0 : 0.0;
   }
 }
 
@@ -861,16 +856,15 @@ hipError_t launch_dyson(const DysonBatch &B, hipStream_t s) {
 
 hipError_t launch_iterate(const IterBatch &B, hipStream_t s) {
   if (B.nprobs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(trace_kernel, dim3(B.nprobs), dim3(1024), 0, s, B);
+  hipLaunchKernelGGL(trace_kernel, dim3(B.nprobs), dim3(64), 0, s, B);
   const dim3 grid((unsigned)((B.max_total + 255) / 256), B.nprobs);
   hipLaunchKernelGGL(scale_diff_kernel, grid, dim3(256), 0, s, B);
-  hipLaunchKernelGGL(mix_kernel, grid, dim3(256), 0, s, B);
   return hipGetLastError();
 }
 
 hipError_t launch_block_max(const IterBatch &B, hipStream_t s) {
   if (B.total_blocks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(block_max_kernel, dim3(B.total_blocks), dim3(256), 0, s, B);
+  hipLaunchKernelGGL(mix_max_kernel, dim3(B.total_blocks), dim3(256), 0, s, B);
   return hipGetLastError();
 }
 

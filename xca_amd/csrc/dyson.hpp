@@ -65,11 +65,10 @@ struct IterArgs {
   const int64_t *goff, *gend;
   const double *lu;
   const int *piv;
-  const double *kbeta;      // K(1, w_k): evaluation at tau = beta
+  const double *zw;         // (r) G(beta) = sum_t zw[t] G(t): zw = cf2it^{-T} K(1, w) (DLR evaluation at tau = beta)
   const double *tau;        // (r) absolute times
   double *G;                // current iterate
   double *Gnew;             // Dyson solution, normalised in place
-  double *trs;              // (ndiag, r) scratch
   double *status;           // status_stride(nb) doubles
 };
 
@@ -81,7 +80,9 @@ struct IterBatch {
 };
 
 hipError_t launch_dyson(const DysonBatch &B, hipStream_t s);
+// Z, eta and the rescaling of G_new with max |G_new - G| (then launch_block_max)
 hipError_t launch_iterate(const IterBatch &B, hipStream_t s);
+// the mixing (unless converged) and the block maxima of the new G with the convergence flag
 hipError_t launch_block_max(const IterBatch &B, hipStream_t s);
 hipError_t launch_fill(int64_t n, double v, double *p, hipStream_t s);
 

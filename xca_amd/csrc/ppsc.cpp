@@ -43,7 +43,7 @@ IterArgs iter_args(xca_ctx *c, double *status, double tol, double mix) {
   A.r = c->M.P.r; A.nb = c->M.P.nb; A.ndiag = d->ndiag;
   A.beta = c->M.P.beta; A.tol = tol; A.mix = mix; A.total = c->M.G_size;
   A.dims = d->dims.ptr; A.goff = d->goff.ptr; A.gend = d->gend.ptr; A.lu = d->lu.ptr; A.piv = d->piv.ptr;
-  A.kbeta = d->kbeta.ptr; A.tau = d->tau.ptr; A.G = c->G.ptr; A.Gnew = d->gnew.ptr; A.trs = d->trs.ptr; A.status = status;
+  A.zw = d->zw.ptr; A.tau = d->tau.ptr; A.G = c->G.ptr; A.Gnew = d->gnew.ptr; A.status = status;
   return A;
 }
 
@@ -82,7 +82,7 @@ int64_t dyson_bytes(const xca_ctx *c) {
     rhs += Np * P.dims[b];
     piv += Np;
   }
-  return 8 * (6 * c->M.G_size + sys + rhs + P.r * ndiag + 6 * (int64_t)P.r * P.r) + 4 * piv + large_dyson_bytes(P.r, n_large);
+  return 8 * (6 * c->M.G_size + sys + rhs + 6 * (int64_t)P.r * P.r + 2 * P.r) + 4 * piv + large_dyson_bytes(P.r, n_large);
 }
 
 void require_dyson(xca_ctx *c) {
@@ -134,11 +134,23 @@ extern "C" int xca_dyson_init(xca_ctx *c, const double *G0) {
     d->cf2it.upload(o.cf2it, s);
     d->tcf2it.upload(o.tcf2it, s);
     d->W.upload(o.W, s);
-    std::vector<double> tau(r), kbeta(r);
+    std::vector<double> tau(r), zw(r);
     for (int t = 0; t < r; ++t) tau[t] = dlr::rel2abs(P.it[t]) * P.beta;
-    for (int k = 0; k < r; ++k) kbeta[k] = dlr::kernel(1.0, P.rf[k]);
+    // Tr G(beta) = K(1, w)^T cf2it^{-1} Tr G(t) = zw . Tr G(t) with cf2it^T zw = K(1, w), solved through
+    // the LU factors (P cf2it = L U): U^T y = K(1, w), L^T z = y, zw = P^T z. cf2it is far from
+    // orthogonal (condition ~1e10 at eps = 1e-10): a transposed solve keeps zw . g as accurate as
+    // the evaluation of vals2coefs(g) at beta, where an explicit inverse would not.
+    for (int k = 0; k < r; ++k) zw[k] = dlr::kernel(1.0, P.rf[k]);
+    for (int i = 0; i < r; ++i) {
+      for (int k = 0; k < i; ++k) zw[i] -= o.lu[(size_t)k * r + i] * zw[k];
+      zw[i] /= o.lu[(size_t)i * r + i];
+    }
+    for (int i = r - 1; i >= 0; --i)
+      for (int k = i + 1; k < r; ++k) zw[i] -= o.lu[(size_t)k * r + i] * zw[k];
+    for (int k = r - 1; k >= 0; --k)
+      if (o.piv[k] != k) std::swap(zw[k], zw[o.piv[k]]);
     d->tau.upload(tau, s);
-    d->kbeta.upload(kbeta, s);
+    d->zw.upload(zw, s);
     // G0 and its coefficients (dyson_it_ppsc ctor: g0c = vals2coefs(g0))
     d->G0_h.assign(G0, G0 + c->M.G_size);
     std::vector<double> g0c(d->G0_h), Wg0c(c->M.G_size);
@@ -156,7 +168,6 @@ extern "C" int xca_dyson_init(xca_ctx *c, const double *G0) {
     d->sys.alloc(so);
     d->rhs.alloc(ro);
     d->ipiv.alloc(po);
-    d->trs.alloc((size_t)r * d->ndiag);
     d->status.alloc(status_stride(nb));
     d->large_flag.upload(large_flag, s);
     d->large = large_dyson_init(c, large_blocks);

@@ -16,8 +16,8 @@ struct XcaDyson {
   xca::DevBuf<int> large_flag;  // (nb) 1 for the blocks of `large`
   xca::DevBuf<int> dims, piv, ipiv;
   xca::DevBuf<int64_t> goff, gend, soff, poff, roff;
-  xca::DevBuf<double> lu, cf2it, tcf2it, W, tau, kbeta;
-  xca::DevBuf<double> G0, g0c, Wg0c, ggs, wf, sys, rhs, gnew, trs, sig_in;
+  xca::DevBuf<double> lu, cf2it, tcf2it, W, tau, zw;
+  xca::DevBuf<double> G0, g0c, Wg0c, ggs, wf, sys, rhs, gnew, sig_in;
   xca::DevBuf<double> status;  // iteration status of the single-problem loop and the eta scalar of xca_dyson_solve
   std::vector<double> G0_h;
   int ndiag = 0;
