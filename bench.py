@@ -194,11 +194,12 @@ def solve_latency(n, cpu=False):
     return out
 
 
-def serving(n, seed=7, cpu_sample=0, threads=16):
+def serving(n, seed=7, cpu_sample=0, threads=16, fixed_rate=43.0):
     """Poisson stream of n requests (1-3 orbital Kanamori, U ~ U[1,6], J = U/8, half filling,
     beta ~ U[10,100], Lambda = 20 beta, eps = 1e-10, NCA 70 % / OCA 30 %, full self-consistency to
     1e-10). The arrival rate is 1.5x the single-stream solve rate measured on the first 16 requests
-    solved one at a time."""
+    solved one at a time; the same stream also runs at a fixed arrival rate (its latencies compare
+    across builds at one offered load) and with every request at t = 0 (capacity)."""
     from xca_amd import models
     from xca_amd.serving import Server, run_stream
     t0 = time.perf_counter()
@@ -222,6 +223,13 @@ def serving(n, seed=7, cpu_sample=0, threads=16):
            "converged": int(conv), "mean_iterations": float(np.mean(iters)), "max_active": out["max_active"],
            "iterations_run": out["stats"]["ticks"], "max_active_slots": 64}
     del sv
+    if fixed_rate > 0:
+        sv = Server(max_active=64, tile_variant=0)
+        fx = run_stream(sv, reqs, rate=fixed_rate, seed=seed)
+        del sv
+        res["fixed_rate"] = {"arrival_rate_per_s": fixed_rate, "solves_per_s": fx["solves_per_s"], "p50_s": fx["p50_s"],
+                             "p99_s": fx["p99_s"], "span_s": fx["span_s"], "max_active": fx["max_active"]}
+        log(f"serving at {fixed_rate:.1f} arrivals/s: {fx['solves_per_s']:.2f} solves/s p50 {fx['p50_s']:.3f} s p99 {fx['p99_s']:.3f} s")
     # capacity: every request arrives at t = 0, so the batch runs full from the start
     sv = Server(max_active=64, tile_variant=0)
     sat = run_stream(sv, reqs, rate=1e9, seed=seed)
@@ -504,6 +512,8 @@ def main():
                     help="host threads for the CPU legs (one oracle evaluator each), capped at the cores this process may use")
     ap.add_argument("--solves", type=int, default=50, help="NCA self-consistent solves timed for the latency figures (0: skip)")
     ap.add_argument("--serving", type=int, default=512, help="requests in the served stream (0: skip)")
+    ap.add_argument("--serving-fixed-rate", type=float, default=43.0,
+                    help="arrival rate of the fixed-load run of the served stream (0: skip)")
     ap.add_argument("--spgf", type=int, default=5, help="timed single-particle Green's function evaluations (0: skip)")
     ap.add_argument("--plain", type=int, default=1, help="time the plain one-problem-at-a-time GPU forms (0: skip)")
     ap.add_argument("--legs", default="tca3,tca3c,oca5", help="larger configurations measured beside the headline ('none' to skip)")
@@ -606,7 +616,8 @@ def main():
         out["solve"] = solve_latency(args.solves, cpu=not args.no_cpu)
         log(f"solve latency p50 {out['solve']['p50_ms']:.2f} ms p99 {out['solve']['p99_ms']:.2f} ms")
     if args.serving > 0:
-        out["serving"] = serving(args.serving, cpu_sample=0 if args.no_cpu else 16, threads=args.cpu_cores)
+        out["serving"] = serving(args.serving, cpu_sample=0 if args.no_cpu else 16, threads=args.cpu_cores,
+                                 fixed_rate=args.serving_fixed_rate)
         log(f"serving {out['serving']['solves_per_s']:.2f} solves/s p50 {out['serving']['p50_s']:.3f} s p99 {out['serving']['p99_s']:.3f} s")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(P, G, max_order, args.cpu_budget, args.cpu_cores)
