@@ -742,25 +742,30 @@ __global__ void __launch_bounds__(256) lu_update_kernel(DysonBatch Bt, int k0) {
   for (int ti = c1 + 16 * wv; ti < Np; ti += 64) trailing_tile(S, Np, k0, ti, col0, S + (size_t)ti * Np + k0, Np, lane);
 }
 
-// Per problem (one wave each): Z = -Tr G_new(beta) = -sum_t zw[t] Tr G_new(t), the diagonal entries
-// summed per node in block order and the node terms in node order by one lane; deta = ln|Z| / beta,
-// eta += deta; the difference accumulator is cleared for the update kernel that follows.
-__global__ void __launch_bounds__(64) trace_kernel(IterBatch Bt) {
-  __shared__ double part[64];
+// Per problem (one workgroup of four waves each): Z = -Tr G_new(beta) = -sum_t zw[t] Tr G_new(t).
+// Wave w sums the diagonal entries of the blocks b = w, w + 4, ... per node (lane t), the four wave
+// sums are added per node in wave order and the node terms in node order by one lane, so the sum has
+// a fixed order; deta = ln|Z| / beta, eta += deta; the difference accumulator is cleared for the
+// update kernel that follows.
+__global__ void __launch_bounds__(256) trace_kernel(IterBatch Bt) {
+  __shared__ double part[4][64];
   const IterArgs &A = Bt.probs[blockIdx.x];
-  const int t = threadIdx.x, r = A.r;
+  const int t = threadIdx.x & 63, w = threadIdx.x >> 6, r = A.r;
   double s = 0.0;
   if (t < r)
-    for (int b = 0; b < A.nb; ++b) {
+    for (int b = w; b < A.nb; b += 4) {
       const int n = A.dims[b];
       const double *g = A.Gnew + A.goff[b] + (size_t)t * n * n;
+#pragma unroll 4
       for (int i = 0; i < n; ++i) s += g[i * n + i];
     }
-  part[t] = t < r ? A.zw[t] * s : 0.0;
+  part[w][t] = s;
   __syncthreads();
-  if (t == 0) {
+  if (threadIdx.x < 64) part[0][t] = t < r ? A.zw[t] * (((part[0][t] + part[1][t]) + part[2][t]) + part[3][t]) : 0.0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
     double tr = 0.0;
-    for (int q = 0; q < r; ++q) tr += part[q];
+    for (int q = 0; q < r; ++q) tr += part[0][q];
     const double Z = -tr;
     const double deta = log(fabs(Z)) / A.beta;
     A.status[kStZ] = Z;
@@ -856,7 +861,7 @@ hipError_t launch_dyson(const DysonBatch &B, hipStream_t s) {
 
 hipError_t launch_iterate(const IterBatch &B, hipStream_t s) {
   if (B.nprobs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(trace_kernel, dim3(B.nprobs), dim3(64), 0, s, B);
+  hipLaunchKernelGGL(trace_kernel, dim3(B.nprobs), dim3(256), 0, s, B);
   const dim3 grid((unsigned)((B.max_total + 255) / 256), B.nprobs);
   hipLaunchKernelGGL(scale_diff_kernel, grid, dim3(256), 0, s, B);
   return hipGetLastError();
