@@ -117,3 +117,25 @@ def test_full_serving_stream_equals_alone():
         if k % 128 == 127:
             print(f"[gpu] full stream: {k + 1} alone solves compared, worst {worst:.1e}", flush=True)
     assert worst <= 1e-12
+
+
+def test_server_with_large_block_problem_equals_alone():
+    """A 5-orbital d-shell request ((N, S_z) blocks up to 100 states: Sigma on the large-block engine,
+    device-wide Dyson) served beside small requests: every result equals the same problem solved alone
+    (xca_ppsc_solve), the large one included, and the admission estimate bounds its device bytes."""
+    from xca_amd.serving import Server, run_stream
+    big = models.oca_5orb_slater(beta=0.25, lam=20.0, eps=1e-10)
+    assert int(big.dims.max()) == 100
+    reqs = [(models.anderson_nca(), 1), (big, 1), (models.oca_2orb(), 2), (models.anderson_nca(), 1)]
+    tol, maxiter, mix = 1e-9, 60, 0.7
+    sv = Server(max_active=8, tile_variant=0)
+    out = run_stream(sv, reqs, rate=1e4, seed=3, tol=tol, maxiter=maxiter, mix=mix, check_footprint=True)
+    print(f"[gpu] stream with a 5-orbital request: max active {out['max_active']} footprint ratio {out['max_footprint_ratio']:.3f}",
+          flush=True)
+    assert 0 < out["max_footprint_ratio"] <= 1.0
+    for (P, order), res in zip(reqs, out["results"]):
+        g, eta, iters, diff = _alone(P, order, tol, maxiter, mix)
+        e = _rel(res["G"], g)
+        print(f"  {P.name}: iters {res['iters']} vs {iters} eta {res['eta']:.12f} vs {eta:.12f} rel diff {e:.1e}", flush=True)
+        assert res["converged"] and res["iters"] == iters
+        assert e == 0.0 and res["eta"] == eta

@@ -44,8 +44,20 @@ int64_t plan_bytes_estimate(xca_ctx *c, int m, const int *topo);
 
 constexpr int kStatusStride = 72;  // doubles per problem status slot (blocks <= 64)
 
+// Descriptors of the Dyson and update launches of one problem whose Sigma runs on the large-block
+// engine (a batch of one each).
+struct WideIter {
+  DevBuf<DysonArgs> dyson;
+  DevBuf<IterArgs> iter;
+  DevBuf<int> prefix;
+  DysonBatch DB{};
+  IterBatch IB{};
+};
+
 struct BatchItem {
   xca_ctx *c = nullptr;
+  bool wide = false;            // blocks above 12 rows (or XCA_WIDE): its iteration runs on its own stream
+  std::unique_ptr<WideIter> wi;
   int max_order = 1, maxiter = 100;
   double tol = 1e-10, mix = 1.0, eta0 = 0.0;
   int iters = 0;
@@ -54,6 +66,7 @@ struct BatchItem {
   int slot = -1;
   uint64_t serial = 0;         // unique per admission into a batch (identifies the item in the plan key)
   std::vector<double> bmax;    // block maxima of the current G (zero-block flags of the next Sigma)
+  std::vector<int> zero_seen;  // zero-block pattern of the plans in use
   std::vector<DevPlan *> plans;
   std::vector<double> scales;
   std::vector<double> history;  // per iteration: max |G_new - G|, Z before the normalisation, eta
@@ -84,9 +97,6 @@ class Batch {
     if (free_slots_.empty()) throw std::runtime_error("xca: batch is full");
     if (it->c->M.P.nb + kStBmax > kStatusStride) throw std::invalid_argument("xca: more than 64 blocks");
     require_dyson(it->c);
-    if (it->c->dyson->large)
-      throw std::invalid_argument("xca: the device self-consistency loop takes Dyson systems r * (block dimension) <= 784; "
-                                  "solve larger blocks with xca_dyson_solve in a host loop");
     it->slot = free_slots_.back();
     free_slots_.pop_back();
     it->serial = ++serial_;
@@ -103,6 +113,22 @@ class Batch {
     it->iters = 0;
     it->done = it->converged = false;
     it->eta = it->eta0;
+    it->wide = c->wide;
+    if (it->wide) {
+      auto w = std::make_unique<WideIter>();
+      double *st = status_.ptr + (size_t)it->slot * kStatusStride;
+      std::vector<DysonArgs> hd(1, dyson_args(c, st + kStEta, c->sigma.ptr));
+      std::vector<IterArgs> hi(1, iter_args(c, st, it->tol, it->mix));
+      std::vector<int> pre{0, P.nb};
+      w->dyson.upload(hd, stream_);
+      w->iter.upload(hi, stream_);
+      w->prefix.upload(pre, stream_);
+      w->DB = DysonBatch{w->dyson.ptr, w->prefix.ptr, 1, P.nb, dyson_np_max_small(c)};
+      w->IB = IterBatch{w->iter.ptr, w->prefix.ptr, 1, P.nb, c->M.G_size};
+      it->wi = std::move(w);
+      // G = G0 and the cleared status precede the first launches on the problem's own stream
+      HIP_CHECK(hipStreamSynchronize(stream_));
+    }
     items_.push_back(it);
   }
 
@@ -112,6 +138,7 @@ class Batch {
     items_.erase(f);
     free_slots_.push_back(it->slot);
     it->slot = -1;
+    it->wi.reset();
   }
 
   // One iteration of every active problem. use_graph: replay the captured launch sequence when the
@@ -120,7 +147,11 @@ class Batch {
     if (items_.empty()) return;
     HIP_CHECK(hipSetDevice(device_));
     prepare();
-    if (use_graph && graph_ok_) {
+    bool any_tile = false;
+    for (BatchItem *it : items_) any_tile = any_tile || !it->wide;
+    if (!any_tile) {
+      // only problems on the large-block engine: nothing for the shared launch sequence
+    } else if (use_graph && graph_ok_) {
       HIP_CHECK(hipGraphLaunch(exec_, stream_));
     } else if (use_graph && eager_done_) {
       HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
@@ -139,7 +170,17 @@ class Batch {
       HIP_CHECK(launch_all());
       eager_done_ = true;
     }
+    // problems on the large-block engine: their iteration on their own streams, beside the shared
+    // sequence; their status words are copied once both have finished
+    for (BatchItem *it : items_)
+      if (it->wide) launch_wide_iteration(*it);
     HIP_CHECK(hipStreamSynchronize(stream_));
+    for (BatchItem *it : items_)
+      if (it->wide) {
+        HIP_CHECK(hipStreamSynchronize(it->c->stream));
+        HIP_CHECK(hipMemcpy(host_status_ + (size_t)it->slot * kStatusStride, status_.ptr + (size_t)it->slot * kStatusStride,
+                            sizeof(double) * kStatusStride, hipMemcpyDeviceToHost));
+      }
     for (BatchItem *it : items_) {
       const double *st = host_status_ + (size_t)it->slot * kStatusStride;
       it->iters += 1;
@@ -163,14 +204,33 @@ class Batch {
   // (problem, plan) composition changed.
   void prepare() {
     std::vector<uint64_t> key;
+    bool evicted = false;
     for (BatchItem *it : items_) {
       xca_ctx *c = it->c;
       const Problem &P = c->M.P;
       c->G_zero.assign(P.nb, 0);
       for (int b = 0; b < P.nb; ++b) c->G_zero[b] = !(it->bmax[b] >= 1e-16);
       c->have_G = true;
+      if (it->zero_seen != c->G_zero) {
+        // a block crossed the zero threshold: the plans of the old pattern are released, so that a
+        // problem never holds more than one pattern's plans (its admission estimate). Every launch
+        // that used them has finished (each tick ends with a synchronisation). Contexts whose plans
+        // a Sigma launch sequence also references keep them.
+        if (!it->zero_seen.empty() && !c->sigall) {
+          c->plans.clear();
+          c->wide_plans.clear();
+          evicted = true;
+        }
+        it->zero_seen = c->G_zero;
+      }
       it->plans.clear();
       it->scales.clear();
+      key.push_back(it->serial);
+      if (it->wide) {  // Sigma on the large-block engine, planned by its own launches
+        key.push_back(0);
+        continue;
+      }
+      key.pop_back();
       for (int order = 1; order <= it->max_order; ++order)
         for (auto &st : connected_topologies(order)) {
           DevPlan *dp = get_plan(c, order, st.second.data(), nullptr, 0);
@@ -197,7 +257,7 @@ class Batch {
       for (DevPlan *dp : it->plans) key.push_back((uint64_t)(uintptr_t)dp);
       key.push_back(0);
     }
-    if (key == key_) return;
+    if (key == key_ && !evicted) return;  // after an eviction, new plans may reuse old addresses
     key_ = key;
     drop_graph();
     eager_done_ = false;
@@ -206,6 +266,9 @@ class Batch {
 
   void build_descriptors() {
     hipStream_t s = stream_;
+    std::vector<BatchItem *> tiles;  // the problems of the shared launch sequence
+    for (BatchItem *it : items_)
+      if (!it->wide) tiles.push_back(it);
     // edge tables and chain segments, grouped by padded rank and tile width
     std::map<int, std::vector<TabSeg>> tabs;                       // RB -> segments
     std::map<std::pair<int, int>, std::vector<ChainArgs>> chains;  // (RB, variant) -> segments
@@ -216,7 +279,7 @@ class Batch {
     std::vector<int> red_prefix{0}, dys_prefix{0};
     max_sig_ = 0;
     int64_t max_total = 0;
-    for (BatchItem *it : items_) {
+    for (BatchItem *it : tiles) {
       xca_ctx *c = it->c;
       const Problem &P = c->M.P;
       for (size_t q = 0; q < it->plans.size(); ++q) {
@@ -227,14 +290,14 @@ class Batch {
       }
     }
     // red plans after the loop above so the vector does not reallocate under stored pointers
-    for (BatchItem *it : items_)
+    for (BatchItem *it : tiles)
       for (size_t q = 0; q < it->plans.size(); ++q) {
         DevVariant *V = it->plans[q]->var[it->plans[q]->chosen].get();
         h_redplans_.push_back(RedPlan{V->slot.ptr, V->red_chunks.ptr, V->red_range.ptr, V->partial.ptr, it->scales[q]});
       }
     redplans_.upload(h_redplans_, s);
     int rp = 0;
-    for (BatchItem *it : items_) {
+    for (BatchItem *it : tiles) {
       xca_ctx *c = it->c;
       const Problem &P = c->M.P;
       RedProb rq{};
@@ -257,15 +320,15 @@ class Batch {
     redprobs_.upload(h_redprobs_, s);
     h_red_prefix_ = red_prefix;
     red_prefix_.upload(h_red_prefix_, s);
-    red_ = RedBatch{redprobs_.ptr, red_prefix_.ptr, (int)items_.size(), red_prefix.back()};
+    red_ = RedBatch{redprobs_.ptr, red_prefix_.ptr, (int)tiles.size(), red_prefix.back()};
     dyson_.upload(h_dyson_, s);
     h_dys_prefix_ = dys_prefix;
     dys_prefix_.upload(h_dys_prefix_, s);
     int np_max = 16;
-    for (BatchItem *it : items_) np_max = std::max(np_max, dyson_np_max(it->c));
-    dys_ = DysonBatch{dyson_.ptr, dys_prefix_.ptr, (int)items_.size(), dys_prefix.back(), np_max};
+    for (BatchItem *it : tiles) np_max = std::max(np_max, dyson_np_max(it->c));
+    dys_ = DysonBatch{dyson_.ptr, dys_prefix_.ptr, (int)tiles.size(), dys_prefix.back(), np_max};
     iter_.upload(h_iter_, s);
-    iter_b_ = IterBatch{iter_.ptr, dys_prefix_.ptr, (int)items_.size(), dys_prefix.back(), max_total};
+    iter_b_ = IterBatch{iter_.ptr, dys_prefix_.ptr, (int)tiles.size(), dys_prefix.back(), max_total};
     // tab groups
     tab_groups_.clear();
     for (auto &kv : tabs) {
@@ -311,6 +374,29 @@ class Batch {
       for (DevPlan *dp : it->plans)
         if (dp->tabjobs.ptr == t.jobs) return dp->ntabjobs;
     return 0;
+  }
+
+  // One iteration of a problem on the large-block engine, on its context's stream: Sigma topology by
+  // topology (net factor -s per topology), the Dyson solve (blocks up to 12 rows in the one-workgroup
+  // kernel, larger ones device-wide, both hermitised; eta of the device-wide path from the last
+  // status read), then the same update launches as the shared sequence.
+  void launch_wide_iteration(BatchItem &it) {
+    xca_ctx *c = it.c;
+    hipStream_t s = c->stream;
+    HIP_CHECK(launch_fill(c->M.G_size, 0.0, c->sigma.ptr, s));
+    c->outer_events = true;
+    try {
+      for (int order = 1; order <= it.max_order; ++order)
+        for (auto &tp : connected_topologies(order)) accumulate(c, order, tp.second.data(), nullptr, 0, -(double)tp.first);
+    } catch (...) {
+      c->outer_events = false;
+      throw;
+    }
+    c->outer_events = false;
+    HIP_CHECK(launch_dyson(it.wi->DB, s));
+    if (c->dyson->large) large_dyson_solve(c, c->dyson->large, c->sigma.ptr, it.eta, c->dyson->gnew.ptr);
+    HIP_CHECK(launch_iterate(it.wi->IB, s));
+    HIP_CHECK(launch_block_max(it.wi->IB, s));
   }
 
   hipError_t launch_all() {
@@ -387,89 +473,6 @@ class Batch {
 
 using namespace xca;
 
-namespace {
-
-// Self-consistency of a problem with blocks above 12 rows: each iteration is Sigma topology by
-// topology on the large-block engine (net factor -s per topology), the Dyson solve (blocks up to 12
-// rows in the one-workgroup kernel, larger ones device-wide, both hermitised), then the same
-// normalisation / convergence / mixing and block-maxima launches as the batched engine, and one
-// copy of the status words to pinned host memory. G stays on the device throughout; eta goes to the
-// device-wide Dyson path from the last status read.
-struct LargeLoop {
-  double *host = nullptr;
-  ~LargeLoop() {
-    if (host) (void)hipHostFree(host);
-  }
-};
-
-void ppsc_solve_large(xca_ctx *c, BatchItem &it) {
-  XcaDyson *d = c->dyson;
-  const Problem &P = c->M.P;
-  const int nb = P.nb, stride = status_stride(nb);
-  hipStream_t s = c->stream;
-  LargeLoop hl;
-  HIP_CHECK(hipHostMalloc((void **)&hl.host, sizeof(double) * stride, hipHostMallocDefault));
-  double *st = d->status.ptr;
-  HIP_CHECK(hipMemcpyAsync(c->G.ptr, d->G0.ptr, c->M.G_size * sizeof(double), hipMemcpyDeviceToDevice, s));
-  HIP_CHECK(launch_fill(stride, 0.0, st, s));
-  HIP_CHECK(launch_fill(1, it.eta0, st + kStEta, s));
-  std::vector<double> bmax(nb, 0.0);
-  for (int b = 0; b < nb; ++b)
-    for (int64_t x = c->M.G_off[b]; x < c->M.G_off[b] + (int64_t)P.r * P.dims[b] * P.dims[b]; ++x)
-      bmax[b] = std::max(bmax[b], std::fabs(d->G0_h[x]));
-  // one-problem descriptors of the Dyson and iteration launches
-  std::vector<DysonArgs> hd(1, dyson_args(c, st + kStEta, c->sigma.ptr));
-  std::vector<IterArgs> hi(1, iter_args(c, st, it.tol, it.mix));
-  std::vector<int> pre{0, nb};
-  DevBuf<DysonArgs> dd;
-  DevBuf<IterArgs> di;
-  DevBuf<int> dp;
-  dd.upload(hd, s);
-  di.upload(hi, s);
-  dp.upload(pre, s);
-  const DysonBatch DB{dd.ptr, dp.ptr, 1, nb, dyson_np_max_small(c)};
-  const IterBatch IB{di.ptr, dp.ptr, 1, nb, c->M.G_size};
-  it.iters = 0;
-  it.done = it.converged = false;
-  it.eta = it.eta0;
-  it.history.clear();
-  while (!it.done) {
-    c->G_zero.assign(nb, 0);
-    for (int b = 0; b < nb; ++b) c->G_zero[b] = !(bmax[b] >= 1e-16);
-    c->have_G = true;
-    HIP_CHECK(launch_fill(c->M.G_size, 0.0, c->sigma.ptr, s));
-    c->outer_events = true;
-    try {
-      for (int order = 1; order <= it.max_order; ++order)
-        for (auto &tp : connected_topologies(order)) accumulate(c, order, tp.second.data(), nullptr, 0, -(double)tp.first);
-    } catch (...) {
-      c->outer_events = false;
-      throw;
-    }
-    c->outer_events = false;
-    HIP_CHECK(launch_dyson(DB, s));
-    if (d->large) large_dyson_solve(c, d->large, c->sigma.ptr, it.eta, d->gnew.ptr);
-    HIP_CHECK(launch_iterate(IB, s));
-    HIP_CHECK(launch_block_max(IB, s));
-    HIP_CHECK(hipMemcpyAsync(hl.host, st, sizeof(double) * stride, hipMemcpyDeviceToHost, s));
-    HIP_CHECK(hipStreamSynchronize(s));
-    const double *h = hl.host;
-    it.iters += 1;
-    it.eta = h[kStEta];
-    it.diff = h[kStDiff];
-    it.history.insert(it.history.end(), {h[kStDiff], h[kStZ], h[kStEta]});
-    for (int b = 0; b < nb; ++b) bmax[b] = h[kStBmax + b];
-    if (h[kStConv] != 0.0) {
-      it.done = true;
-      it.converged = true;
-    } else if (it.iters >= it.maxiter) {
-      it.done = true;
-    }
-  }
-}
-
-}  // namespace
-
 // Pseudo-particle self-consistency of one problem through the batched engine (a batch of one).
 // On return G_out holds the solver's G (the last mixed iterate, block_sparse_solver.py:338-346),
 // eta_out the pseudo-particle chemical potential.
@@ -487,15 +490,11 @@ extern "C" int xca_ppsc_solve(xca_ctx *c, double eta0, int max_order, double tol
     it.tol = tol;
     it.mix = mix;
     it.eta0 = eta0;
-    if (c->dyson->large) {  // blocks above 12 rows: the large-block engine and the device-wide Dyson path
-      ppsc_solve_large(c, it);
-    } else {
-      Batch batch(c->device, c->stream, 1);
-      batch.add(&it);
-      const bool graphs = !std::getenv("XCA_NO_GRAPH");
-      while (!it.done) batch.tick(graphs);
-      batch.remove(&it);
-    }
+    Batch batch(c->device, c->stream, 1);
+    batch.add(&it);
+    const bool graphs = !std::getenv("XCA_NO_GRAPH");
+    while (!it.done) batch.tick(graphs);
+    batch.remove(&it);
     HIP_CHECK(hipMemcpyAsync(G_out, c->G.ptr, c->M.G_size * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
     if (eta_out) *eta_out = it.eta;

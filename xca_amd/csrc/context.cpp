@@ -845,6 +845,7 @@ int64_t ctx_bytes(const xca_ctx *c) {
 // (the forced one and the widest as its fallback, or all of them when the autotuner chooses), its
 // programs, chunk descriptors, partial slots and reduction lists at that variant's tile count.
 int64_t plan_bytes_estimate(xca_ctx *c, int m, const int *topo) {
+  if (c->wide) return wide_plan_bytes_estimate(c, m, topo);  // Sigma on the large-block engine
   Plan plan;
   build_programs(c->M, c->ps, topo, m, 0, c->G_zero, plan);
   const Problem &P = c->M.P;
@@ -899,6 +900,8 @@ int64_t plan_bytes_estimate(xca_ctx *c, int m, const int *topo) {
 int64_t ctx_device_bytes(const xca_ctx *c) {
   int64_t b = ctx_bytes(c) + (c->dyson ? dyson_bytes(c) : 0);
   for (auto &kv : c->plans) b += kv.second->device_bytes;
+  for (auto &kv : c->wide_plans) b += kv.second->device_bytes;
+  if (c->widectx) b += 8 * (int64_t)c->widectx->M2.n;
   return b;
 }
 

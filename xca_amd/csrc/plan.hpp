@@ -130,6 +130,8 @@ struct WideCorrProg {
 // Large-block correlator chains of every batch (wide.cpp).
 std::unique_ptr<WidePlan> build_wide_corr(xca_ctx *c, const std::vector<EdgeTable> &tables, int64_t ftab_size,
                                           const std::vector<WideCorrProg> &progs, int nbatch, double *store, int64_t store_n);
+// Device bytes of the large-block Sigma plan of one topology (no allocation).
+int64_t wide_plan_bytes_estimate(xca_ctx *c, int m, const int *topo);
 // Launches [l0, l1) of a large-block plan on the context's stream.
 void wide_launch_range(xca_ctx *c, const WidePlan &wp, size_t l0, size_t l1);
 

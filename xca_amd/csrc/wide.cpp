@@ -584,6 +584,39 @@ static WidePlan *get_wide_plan(xca_ctx *c, int m, const int *topo, const int32_t
   return raw;
 }
 
+// Device bytes of the Sigma plan of one topology at the context's zero-block pattern, with every pole
+// tuple of every program (the allocations of get_wide_plan, without building it), plus the
+// problem-level matrix the engine keeps once per context.
+int64_t wide_plan_bytes_estimate(xca_ctx *c, int m, const int *topo) {
+  Plan plan;
+  build_programs(c->M, c->ps, topo, m, 0, c->G_zero, plan);
+  WideBuilder B(c, plan.tables);
+  B.size_tables();
+  int64_t items = 0, offs = 0, zperm = 0;
+  std::map<int64_t, int> mus;
+  for (const Program &pg : plan.progs) {
+    int fb[kMaxLines + 1];
+    for (int i = 0, f = pg.fb_ix; i < pg.m; ++i, f /= 2) fb[i] = f % 2;
+    int64_t ni = 1;
+    for (int x = 0; x < pg.m - 1; ++x) ni *= (int64_t)c->ps.get(fb[x + 1], (pg.cls >> x) & 1).size();
+    if (ni == 0) continue;
+    B.size_program(pg, ni);
+    items += ni;
+    for (const Op &op : pg.ops) {
+      if (op.type == OP_V) offs += ni * ((op.bar_line >= 0 ? 1 : 0) + (op.ks != 0 ? 1 : 0));
+      if (op.type == OP_E) offs += ni;
+      if (op.type == OP_Z && !mus.count(op.mubase)) {
+        mus[op.mubase] = 1;
+        zperm += (int64_t)op.bout * op.bin * op.qs;
+      }
+    }
+  }
+  auto n1 = [](int64_t n) { return std::max<int64_t>(n, 1); };
+  const int64_t work = n1(6 * B.max_ws + B.max_fh + B.max_tab + B.max_yz);
+  const int r = c->M.P.r;
+  return 8 * (work + n1(plan.ftab_size) + 2 * n1(items) + n1(offs) + n1(zperm)) + 8 * (int64_t)r * 2 * r;
+}
+
 std::unique_ptr<WidePlan> build_wide_corr(xca_ctx *c, const std::vector<EdgeTable> &tables, int64_t ftab_size,
                                           const std::vector<WideCorrProg> &progs, int nbatch, double *store, int64_t store_n) {
   init_wide_ctx(c);
