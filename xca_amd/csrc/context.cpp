@@ -253,7 +253,7 @@ std::vector<DTabJob> tab_jobs(const std::vector<EdgeTable> &tables, const xca_ct
   for (const auto &tb : tables) {
     int b = P.dims[tb.block], bb = b * b;
     if (bb > tile_columns(0)) throw std::invalid_argument("xca: block too large for the edge-table kernel");
-    int per = std::max(1, (bb <= 32 ? 32 : 64) / bb);
+    int per = std::max(1, (bb <= 16 ? 16 : bb <= 32 ? 32 : 64) / bb);
     for (int64_t e0 = 0; e0 < tb.nentries; e0 += per) {
       DTabJob j{};
       j.block = tb.block; j.b = b; j.nlines = tb.nlines; j.e0 = (int)e0;
