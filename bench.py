@@ -452,6 +452,14 @@ def workload_leg(name, steps, cpu, threads, cpu_budget, cpu_cores):
         ev.compute_self_energy(G, t)
     plain = time.perf_counter() - t0
     out["plain_gpu"] = {"per_topology_calls": {"s_per_eval": plain, "batched_over_plain": plain / s_per}}
+    if name == "oca5":
+        # g(tau) of the same G, orders 1 to max_order, on the large-block engine (chains in store
+        # mode, reflection, contraction), after a first-order warm-up evaluation
+        ev.single_particle_greens_function(G, 1)
+        t0 = time.perf_counter()
+        ev.single_particle_greens_function(G, max_order)
+        out["spgf"] = {"s_per_eval": time.perf_counter() - t0, "orders": f"1-{max_order}", "engine": "large-block"}
+        log(f"{name}: g(tau) orders 1-{max_order} in {out['spgf']['s_per_eval']:.2f} s")
     if cpu:
         po = _po()
         acc = []
