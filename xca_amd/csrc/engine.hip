@@ -973,8 +973,12 @@ __global__ void reset_queue_kernel(int *q) {
 // with the operators as sparse rows (wave-uniform, scalar loads) and R, L^T read with the time index
 // fastest (coalesced). Runs are summed over the chunk's tuples in a fixed order, so the result does
 // not depend on the schedule; every chunk writes its whole (r, nout) slot.
+constexpr int kCorrStageInts = 1024, kCorrStageVals = 512;  // per wave: a term's operators in LDS
+
 template <bool LARGE>
 __global__ void __launch_bounds__(256) corr_kernel(CorrArgs A) {
+  __shared__ int s_cidx[LARGE ? 4 : 1][LARGE ? kCorrStageInts : 1];
+  __shared__ double s_cval[LARGE ? 4 : 1][LARGE ? kCorrStageVals : 1];
   const CChunk ch = uniform_load(A.chunks + blockIdx.x);
   const int r = A.r, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, t = lane;
   double *slot = A.partial + (size_t)blockIdx.x * r * A.nout;
@@ -989,20 +993,49 @@ __global__ void __launch_bounds__(256) corr_kernel(CorrArgs A) {
       const int *mp = A.cidx + T.mu_csr, *mc = mp + d1 + 1;
       const int *kp = A.cidx + T.kap_csr, *kc = kp + db + 1;
       const double *mv = A.cval + T.mu_val, *kv = A.cval + T.kap_val;
+      bool staged = false;
+      int *smp = nullptr, *skp = nullptr;
+      double *smv = nullptr, *skv = nullptr;
+      if (LARGE) {
+        // copy the term's two sparse operators (row pointers, columns, values) into this wave's part
+        // of LDS, so that the loops below read them without global round trips
+        const int nmu = __builtin_amdgcn_readfirstlane(mp[d1]), nkap = __builtin_amdgcn_readfirstlane(kp[db]);
+        const int ni = (d1 + 1 + nmu) + (db + 1 + nkap), nv = nmu + nkap;
+        staged = ni <= kCorrStageInts && nv <= kCorrStageVals;
+        if (staged) {
+          __builtin_amdgcn_wave_barrier();  // the previous term's reads of these slots are done
+          smp = s_cidx[wv];
+          skp = smp + d1 + 1 + nmu;
+          smv = s_cval[wv];
+          skv = smv + nmu;
+          for (int x = lane; x < d1 + 1 + nmu; x += 64) smp[x] = mp[x];
+          for (int x = lane; x < db + 1 + nkap; x += 64) skp[x] = kp[x];
+          for (int x = lane; x < nmu; x += 64) smv[x] = mv[x];
+          for (int x = lane; x < nkap; x += 64) skv[x] = kv[x];
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+      }
       for (int k = ch.k0; k < ch.k1; ++k) {
         if (t >= r) break;
         const double *R = A.store + T.r_off + (size_t)k * d0 * db * r + t;  // element (x, a) at R[(x db + a) r]
         const double *L = A.store + T.l_off + (size_t)k * d1 * dc * r + t;  // element (i, j) at L[(i dc + j) r]
         if (LARGE) {  // the same sums in the same order, each element of the mu product in turn
+          // operator rows from this wave's LDS copy when the term's operators fit it
+          const int *cmp = staged ? smp : mp, *cmc = staged ? smp + d1 + 1 : mc;
+          const int *ckp = staged ? skp : kp, *ckc = staged ? skp + db + 1 : kc;
+          const double *cmv = staged ? smv : mv, *ckv = staged ? skv : kv;
           for (int i = 0; i < d1; ++i) {
-            const int e0 = __builtin_amdgcn_readfirstlane(mp[i]), e1 = __builtin_amdgcn_readfirstlane(mp[i + 1]);
+            const int e0 = __builtin_amdgcn_readfirstlane(cmp[i]), e1 = __builtin_amdgcn_readfirstlane(cmp[i + 1]);
+            if (e0 == e1) continue;  // a zero row of O_mu adds nothing
             const double *Li = L + (size_t)i * dc * r;
             for (int a = 0; a < db; ++a) {
               double nra = 0.0;
-              for (int e = e0; e < e1; ++e) nra += mv[e] * R[((size_t)__builtin_amdgcn_readfirstlane(mc[e]) * db + a) * r];
+              for (int e = e0; e < e1; ++e) nra += cmv[e] * R[((size_t)__builtin_amdgcn_readfirstlane(cmc[e]) * db + a) * r];
               double qv = 0.0;
-              const int f0 = __builtin_amdgcn_readfirstlane(kp[a]), f1 = __builtin_amdgcn_readfirstlane(kp[a + 1]);
-              for (int e = f0; e < f1; ++e) qv += kv[e] * Li[(size_t)__builtin_amdgcn_readfirstlane(kc[e]) * r];
+              const int f0 = __builtin_amdgcn_readfirstlane(ckp[a]), f1 = __builtin_amdgcn_readfirstlane(ckp[a + 1]);
+              for (int e = f0; e < f1; ++e) qv += ckv[e] * Li[(size_t)__builtin_amdgcn_readfirstlane(ckc[e]) * r];
               sum += nra * qv;
             }
           }
