@@ -993,25 +993,36 @@ __global__ void __launch_bounds__(256) corr_kernel(CorrArgs A) {
       const int *mp = A.cidx + T.mu_csr, *mc = mp + d1 + 1;
       const int *kp = A.cidx + T.kap_csr, *kc = kp + db + 1;
       const double *mv = A.cval + T.mu_val, *kv = A.cval + T.kap_val;
-      bool staged = false;
-      int *smp = nullptr, *skp = nullptr;
-      double *smv = nullptr, *skv = nullptr;
+      bool staged = false, single = false;
+      int *smp = nullptr, *skp = nullptr, *skj = nullptr;
+      double *smv = nullptr, *skv = nullptr, *skw = nullptr;
       if (LARGE) {
         // copy the term's two sparse operators (row pointers, columns, values) into this wave's part
         // of LDS, so that the loops below read them without global round trips
         const int nmu = __builtin_amdgcn_readfirstlane(mp[d1]), nkap = __builtin_amdgcn_readfirstlane(kp[db]);
-        const int ni = (d1 + 1 + nmu) + (db + 1 + nkap), nv = nmu + nkap;
+        const int ni = (d1 + 1 + nmu) + (db + 1 + nkap) + db, nv = nmu + nkap + db;
         staged = ni <= kCorrStageInts && nv <= kCorrStageVals;
         if (staged) {
           __builtin_amdgcn_wave_barrier();  // the previous term's reads of these slots are done
           smp = s_cidx[wv];
           skp = smp + d1 + 1 + nmu;
+          skj = skp + db + 1 + nkap;
           smv = s_cval[wv];
           skv = smv + nmu;
+          skw = skv + nkap;
+          bool one = true;  // every row of both operators holds at most one entry
           for (int x = lane; x < d1 + 1 + nmu; x += 64) smp[x] = mp[x];
           for (int x = lane; x < db + 1 + nkap; x += 64) skp[x] = kp[x];
           for (int x = lane; x < nmu; x += 64) smv[x] = mv[x];
           for (int x = lane; x < nkap; x += 64) skv[x] = kv[x];
+          for (int x = lane; x < d1; x += 64) one = one && mp[x + 1] - mp[x] <= 1;
+          for (int x = lane; x < db; x += 64) {  // O_kap row x as (column, value), (0, 0) when empty
+            const int f0 = kp[x], f1 = kp[x + 1];
+            one = one && f1 - f0 <= 1;
+            skj[x] = f1 > f0 ? kc[f0] : 0;
+            skw[x] = f1 > f0 ? kv[f0] : 0.0;
+          }
+          single = __ballot(!one) == 0;
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
           __builtin_amdgcn_wave_barrier();
           __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1026,6 +1037,23 @@ __global__ void __launch_bounds__(256) corr_kernel(CorrArgs A) {
           const int *cmp = staged ? smp : mp, *cmc = staged ? smp + d1 + 1 : mc;
           const int *ckp = staged ? skp : kp, *ckc = staged ? skp + db + 1 : kc;
           const double *cmv = staged ? smv : mv, *ckv = staged ? skv : kv;
+          if (single) {  // the same products and sums with at most one entry per operator row: the
+                         // row loads of R and L no longer wait on each other
+            for (int i = 0; i < d1; ++i) {
+              const int e0 = __builtin_amdgcn_readfirstlane(smp[i]);
+              if (__builtin_amdgcn_readfirstlane(smp[i + 1]) == e0) continue;
+              const double v = smv[e0];
+              const double *Rx = R + (size_t)__builtin_amdgcn_readfirstlane(smp[d1 + 1 + e0]) * db * r;
+              const double *Li = L + (size_t)i * dc * r;
+#pragma unroll 8
+              for (int a = 0; a < db; ++a) {
+                const double nra = 0.0 + v * Rx[(size_t)a * r];
+                const double qv = 0.0 + skw[a] * Li[(size_t)skj[a] * r];
+                sum += nra * qv;
+              }
+            }
+            continue;
+          }
           for (int i = 0; i < d1; ++i) {
             const int e0 = __builtin_amdgcn_readfirstlane(cmp[i]), e1 = __builtin_amdgcn_readfirstlane(cmp[i + 1]);
             if (e0 == e1) continue;  // a zero row of O_mu adds nothing
